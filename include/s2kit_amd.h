@@ -1,0 +1,72 @@
+/* s2kit_amd.h -- batched device API of libs2kit_amd.so.
+ *
+ * No counterpart in the reference, which transforms one field per call from
+ * host memory.  This is the throughput path: a plan per bandwidth holds the
+ * FP64 tables (generated on the GPU), and each call transforms many fields
+ * whose grids and coefficients already live in device memory.  Per field,
+ * layouts are exactly those of FSTSemiMemo / InvFSTSemiMemo
+ * (FST_semi_memo.c:28-54):
+ *   grid         re[f*4bw^2 + j*2bw + k], im likewise   (theta_j major, phi_k)
+ *   coefficients re[f*bw^2 + IndexOfHarmonicCoeff(m,l,bw)], im likewise
+ *
+ * Every function returns 0 on success or a negative S2AMD_E* code, with a
+ * message for s2amd_last_error() (per thread).  `stream` is a hipStream_t
+ * (NULL = the default stream); calls are asynchronous on it.  Calls that
+ * share a plan must not run concurrently (the plan owns the scratch).
+ */
+#ifndef S2KIT_AMD_H
+#define S2KIT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    S2AMD_OK = 0,
+    S2AMD_EINVAL = -1,  /* bad argument */
+    S2AMD_EDEVICE = -2, /* HIP runtime error / no device */
+    S2AMD_ENOMEM = -3   /* device allocation failed */
+};
+
+/* Arithmetic of the transforms (inputs and outputs are FP64 in every form). */
+typedef enum {
+    S2AMD_FP64 = 0, /* FP64 throughout: matches the reference to ~1e-13 */
+    S2AMD_FP32 = 1  /* FP32 FFTs, DCTs and Legendre sums; FP64 tables and I/O */
+} s2amd_precision;
+
+typedef struct s2amd_plan s2amd_plan;
+
+/* Plan for bandwidth bw (2..1024), up to max_fields fields per call,
+ * seminaive for orders < cutoff (cutoff = bw: all orders).  Generates the
+ * forward and transposed tables on the current device. */
+int s2amd_plan_create(s2amd_plan** plan, int bw, int cutoff, int max_fields, int precision);
+int s2amd_plan_destroy(s2amd_plan* plan);
+/* Device bytes held by the plan (tables + scratch). */
+size_t s2amd_plan_bytes(const s2amd_plan* plan);
+/* Device pointers to the plan's tables (layout of SemiNaive_Naive_Pml_Table /
+ * Transpose_..., order m at offset off[m]), for inspection. */
+int s2amd_plan_tables(const s2amd_plan* plan, const double** fwd, const double** inv, long* ndoubles);
+
+/* nfields grids -> coefficients (FSTSemiMemo per field). data_format: 0 COMPLEX, 1 REAL. */
+int s2amd_forward(s2amd_plan* plan, const double* re, const double* im, double* cre, double* cim, int nfields,
+                  int data_format, void* stream);
+/* nfields coefficient sets -> grids (InvFSTSemiMemo per field). */
+int s2amd_inverse(s2amd_plan* plan, const double* cre, const double* cim, double* re, double* im, int nfields,
+                  int data_format, void* stream);
+/* Zonal coefficients zr/zi [nfields][bw] (FZTSemiMemo per field). */
+int s2amd_zonal(s2amd_plan* plan, const double* re, const double* im, double* zr, double* zi, int nfields,
+                int data_format, void* stream);
+/* ConvOn2SphereSemiMemo per field: signal s, filter h (grids), output o. */
+int s2amd_conv(s2amd_plan* plan, const double* sre, const double* sim, const double* hre, const double* him,
+               double* ore, double* oim, int nfields, void* stream);
+
+/* Drop-in error policy: 1 (default) print and abort, 0 record only. */
+void s2amd_set_abort_on_error(int on);
+const char* s2amd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -150,7 +150,7 @@ def cospml_table(bw, m):
 
 def transpose_cospml_table(bw, m, t):
     t = f64(t)
-    tt = np.empty(table_size(m, bw))
+    tt = np.zeros(table_size(m, bw))  # odd bw: the transposed rows leave a tail unwritten
     lib().so_transpose_cospml_table(bw, m, _p(t), _p(tt))
     return tt
 

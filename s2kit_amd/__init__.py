@@ -1,0 +1,279 @@
+"""s2kit_amd -- MI355X-native memoised seminaive spherical harmonic transform.
+
+Python host side over libs2kit_amd.so (HIP kernels for gfx950 behind a C ABI).
+
+* ``FSTSemiMemo`` & co. are thin ctypes bindings of the library's S2kit
+  drop-ins: same arguments, same host-memory layouts and table formats as
+  S2kit 1.1's FST_semi_memo.h / cospml.h / util.h / weights.h.
+* ``Plan`` binds the batched device API (s2kit_amd.h): many fields per call,
+  device-resident data (torch tensors are used only as device allocations).
+
+Nothing here computes on the CPU: if the library is missing the import of a
+binding raises, and every transform runs on the GPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libs2kit_amd.so")
+
+COMPLEX, REAL = 0, 1
+FP64, FP32 = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_dpp = ctypes.POINTER(_dp)
+_i = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# every symbol the headers under include/ declare, with (restype, argtypes)
+SIGNATURES = {
+    # util.h
+    "IndexOfHarmonicCoeff": (_i, [_i, _i, _i]),
+    "TransMult": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _i]),
+    # weights.h
+    "GenerateWeightsForDLT": (None, [_i, _dp]),
+    # pml.h
+    "GeneratePmlTable": (None, [_i, _i, _dp, _dp]),
+    # cospml.h
+    "TableSize": (_i, [_i, _i]),
+    "Spharmonic_TableSize": (_i, [_i]),
+    "Reduced_SpharmonicTableSize": (_i, [_i, _i]),
+    "Reduced_Naive_TableSize": (_i, [_i, _i]),
+    "TableOffset": (_i, [_i, _i]),
+    "RowSize": (_i, [_i, _i]),
+    "Transpose_RowSize": (_i, [_i, _i, _i]),
+    "GenerateCosPmlTable": (None, [_i, _i, _dp, _dp]),
+    "TransposeCosPmlTable": (None, [_i, _i, _dp, _dp]),
+    "Spharmonic_Pml_Table": (_dpp, [_i, _dp, _dp]),
+    "Transpose_Spharmonic_Pml_Table": (_dpp, [_dpp, _i, _dp]),
+    "SemiNaive_Naive_Pml_Table": (_dpp, [_i, _i, _dp, _dp]),
+    "Transpose_SemiNaive_Naive_Pml_Table": (_dpp, [_dpp, _i, _i, _dp, _dp]),
+    # FST_semi_memo.h
+    "FSTSemiMemo": (None, [_dp, _dp, _dp, _dp, _i, _dpp, _dp, _i, _i, _vp, _vp, _dp]),
+    "InvFSTSemiMemo": (None, [_dp, _dp, _dp, _dp, _i, _dpp, _dp, _i, _i, _vp, _vp]),
+    "FZTSemiMemo": (None, [_dp, _dp, _dp, _dp, _i, _dp, _dp, _i, _vp, _dp]),
+    "ConvOn2SphereSemiMemo": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _i, _dp]),
+    # s2kit_amd.h
+    "s2amd_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i]),
+    "s2amd_plan_destroy": (_i, [_vp]),
+    "s2amd_plan_bytes": (ctypes.c_size_t, [_vp]),
+    "s2amd_plan_tables": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_long)]),
+    "s2amd_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "s2amd_inverse": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "s2amd_zonal": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "s2amd_conv": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "s2amd_set_abort_on_error": (None, [_i]),
+    "s2amd_last_error": (ctypes.c_char_p, []),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libs2kit_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build it with `make -C s2kit_amd/csrc` "
+                               "or __graft_entry__.build()")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class S2Error(RuntimeError):
+    pass
+
+
+def last_error():
+    return lib().s2amd_last_error().decode()
+
+
+def set_abort_on_error(on):
+    lib().s2amd_set_abort_on_error(1 if on else 0)
+
+
+def _p(a):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float64 numpy array")
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# S2kit drop-ins (host arrays)
+# ---------------------------------------------------------------------------
+class HostTables:
+    """Tables produced by the drop-in builders into caller (numpy) memory,
+    with the malloc'd pointer arrays the builders return."""
+
+    def __init__(self, bw, cutoff=None):
+        L = lib()
+        cutoff = bw if cutoff is None else cutoff
+        self.bw, self.cutoff = bw, cutoff
+        n = L.Reduced_Naive_TableSize(bw, cutoff) + L.Reduced_SpharmonicTableSize(bw, cutoff)
+        self.fwd = np.zeros(max(n, 1))
+        self.inv = np.zeros(max(n, 1))
+        ws = np.zeros(16 * bw)
+        if cutoff == bw:
+            self.pf = L.Spharmonic_Pml_Table(bw, _p(self.fwd), _p(ws))
+            self.pi = L.Transpose_Spharmonic_Pml_Table(self.pf, bw, _p(self.inv))
+        else:
+            self.pf = L.SemiNaive_Naive_Pml_Table(bw, cutoff, _p(self.fwd), _p(ws))
+            self.pi = L.Transpose_SemiNaive_Naive_Pml_Table(self.pf, bw, cutoff, _p(self.inv), _p(ws))
+        if not self.pf or not self.pi:
+            raise S2Error(last_error())
+
+    def offsets(self):
+        base = self.fwd.ctypes.data
+        return np.array([ctypes.cast(self.pf[m], _vp).value - base for m in range(self.bw)]) // 8
+
+    def __del__(self):
+        libc = ctypes.CDLL(None)
+        libc.free.argtypes = [_vp]
+        for p in (getattr(self, "pf", None), getattr(self, "pi", None)):
+            if p:
+                libc.free(ctypes.cast(p, _vp))
+
+
+def weights(bw):
+    w = np.empty(4 * bw)
+    lib().GenerateWeightsForDLT(bw, _p(w))
+    return w
+
+
+def fst(rdata, idata, bw, tables, w=None, data_format=COMPLEX, cutoff=None):
+    """FSTSemiMemo on host arrays; returns (rcoeffs, icoeffs)."""
+    cutoff = tables.cutoff if cutoff is None else cutoff
+    rdata, idata = _f64(rdata).ravel(), _f64(idata).ravel()
+    w = weights(bw) if w is None else _f64(w)
+    rc, ic = np.empty(bw * bw), np.empty(bw * bw)
+    lib().FSTSemiMemo(_p(rdata), _p(idata), _p(rc), _p(ic), bw, tables.pf, None, data_format, cutoff, None, None,
+                      _p(w))
+    return rc, ic
+
+
+def inv_fst(rcoeffs, icoeffs, bw, tables, data_format=COMPLEX, cutoff=None):
+    """InvFSTSemiMemo on host arrays; returns (rdata, idata)."""
+    cutoff = tables.cutoff if cutoff is None else cutoff
+    rc, ic = _f64(rcoeffs), _f64(icoeffs)
+    n = 2 * bw
+    rd, idd = np.empty(n * n), np.empty(n * n)
+    lib().InvFSTSemiMemo(_p(rc), _p(ic), _p(rd), _p(idd), bw, tables.pi, None, data_format, cutoff, None, None)
+    return rd, idd
+
+
+def fzt(rdata, idata, bw, tables, w=None, data_format=COMPLEX):
+    """FZTSemiMemo on host arrays; returns (rres, ires) of length bw."""
+    rdata, idata = _f64(rdata).ravel(), _f64(idata).ravel()
+    w = weights(bw) if w is None else _f64(w)
+    rr, ir = np.empty(bw), np.empty(bw)
+    lib().FZTSemiMemo(_p(rdata), _p(idata), _p(rr), _p(ir), bw, tables.pf[0], None, data_format, None, _p(w))
+    return rr, ir
+
+
+def conv(rdata, idata, rfilter, ifilter, bw):
+    """ConvOn2SphereSemiMemo on host arrays; returns (rres, ires)."""
+    a = [_f64(x).ravel() for x in (rdata, idata, rfilter, ifilter)]
+    n = 2 * bw
+    rr, ir = np.empty(n * n), np.empty(n * n)
+    lib().ConvOn2SphereSemiMemo(*[_p(x) for x in a], _p(rr), _p(ir), bw, None)
+    return rr, ir
+
+
+def trans_mult(rdc, idc, rfc, ifc, bw):
+    a = [_f64(x) for x in (rdc, idc, rfc, ifc)]
+    rr, ir = np.empty(bw * bw), np.empty(bw * bw)
+    lib().TransMult(*[_p(x) for x in a], _p(rr), _p(ir), bw)
+    return rr, ir
+
+
+def cospml_table(bw, m):
+    t = np.empty(lib().TableSize(m, bw))
+    lib().GenerateCosPmlTable(bw, m, _p(t), None)
+    return t
+
+
+def transpose_cospml_table(bw, m, t):
+    t = _f64(t)
+    tt = np.empty(lib().TableSize(m, bw))
+    lib().TransposeCosPmlTable(bw, m, _p(t), _p(tt))
+    return tt
+
+
+def pml_table(bw, m):
+    t = np.empty(2 * bw * (bw - m))
+    lib().GeneratePmlTable(bw, m, _p(t), None)
+    return t
+
+
+# ---------------------------------------------------------------------------
+# batched device API
+# ---------------------------------------------------------------------------
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Plan:
+    """Batched transforms of up to ``max_fields`` fields at bandwidth ``bw``
+    on device memory.  Arguments are torch float64 CUDA tensors laid out per
+    field like the drop-ins: grids [F, 2bw, 2bw], coefficients [F, bw*bw]."""
+
+    def __init__(self, bw, max_fields, precision=FP64, cutoff=None):
+        cutoff = bw if cutoff is None else cutoff
+        h = ctypes.c_void_p()
+        rc = lib().s2amd_plan_create(ctypes.byref(h), bw, cutoff, max_fields, precision)
+        if rc != 0:
+            raise S2Error(last_error())
+        self.h, self.bw, self.cutoff, self.max_fields, self.precision = h, bw, cutoff, max_fields, precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().s2amd_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def nbytes(self):
+        return lib().s2amd_plan_bytes(self.h)
+
+    @staticmethod
+    def _stream(stream):
+        import torch
+        s = torch.cuda.current_stream() if stream is None else stream
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise S2Error(last_error())
+
+    def forward(self, re, im, cre, cim, nfields=None, data_format=COMPLEX, stream=None):
+        n = re.shape[0] if nfields is None else nfields
+        self._check(lib().s2amd_forward(self.h, _ptr(re), _ptr(im), _ptr(cre), _ptr(cim), n, data_format,
+                                        self._stream(stream)))
+
+    def inverse(self, cre, cim, re, im, nfields=None, data_format=COMPLEX, stream=None):
+        n = cre.shape[0] if nfields is None else nfields
+        self._check(lib().s2amd_inverse(self.h, _ptr(cre), _ptr(cim), _ptr(re), _ptr(im), n, data_format,
+                                        self._stream(stream)))
+
+    def zonal(self, re, im, zr, zi, nfields=None, data_format=COMPLEX, stream=None):
+        n = re.shape[0] if nfields is None else nfields
+        self._check(lib().s2amd_zonal(self.h, _ptr(re), _ptr(im), _ptr(zr), _ptr(zi), n, data_format,
+                                      self._stream(stream)))
+
+    def conv(self, sre, sim, hre, him, ore, oim, nfields=None, stream=None):
+        n = sre.shape[0] if nfields is None else nfields
+        self._check(lib().s2amd_conv(self.h, _ptr(sre), _ptr(sim), _ptr(hre), _ptr(him), _ptr(ore), _ptr(oim), n,
+                                     self._stream(stream)))

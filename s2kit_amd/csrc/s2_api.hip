@@ -1,0 +1,816 @@
+// C-ABI of libs2kit_amd.so: S2kit drop-ins (host pointers) and the batched
+// device API (s2kit_amd.h).
+//
+// Drop-in calls run on the current HIP device, one at a time (a process-wide
+// mutex), on a stream owned by the library.  Caller tables are copied to the
+// device once and cached: the cache key is (bw, cutoff, direction, the
+// caller's per-order pointers) plus a fingerprint of 64 sampled values per
+// order, re-checked on every call, so a table rebuilt in the same memory for
+// other parameters is uploaded again.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "s2_internal.h"
+
+extern "C" {
+#include "s2kit/FST_semi_memo.h"
+#include "s2kit/cospml.h"
+#include "s2kit/pml.h"
+#include "s2kit/util.h"
+#include "s2kit/weights.h"
+#include "s2kit_amd.h"
+}
+
+namespace s2 {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static int g_abort_on_error = 1;
+
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+static void dropin_fail(const char* fn, const std::string& msg) {
+    set_error(std::string(fn) + ": " + msg);
+    std::fprintf(stderr, "libs2kit_amd: %s: %s\n", fn, msg.c_str());
+    if (g_abort_on_error) std::abort();
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    void reserve(size_t bytes) {
+        if (bytes <= n) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        S2_HIP(hipMalloc(&p, bytes));
+        n = bytes;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// drop-in context
+// ---------------------------------------------------------------------------
+struct CachedTables {
+    int bw, cutoff, dir;  // dir 0 forward, 1 transposed, 2 order-0 only (FZT)
+    int device;
+    std::vector<const double*> ptrs;
+    std::vector<unsigned long long> fp;
+    DevTables t;
+};
+
+struct CachedConsts {
+    int bw, device;
+    std::vector<double> weights;  // caller weights the prescaled copy was made from (empty: GPU weights)
+    DevConsts c;
+};
+
+struct Ctx {
+    std::mutex mu;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    DBuf re, im, cre, cim, X, Cb, aux1, aux2, aux3, aux4;
+    std::list<CachedTables> tables;  // most recent first
+    std::list<CachedConsts> consts;
+    size_t table_budget = (size_t)8 << 30;
+};
+
+static Ctx& ctx() {
+    static Ctx* c = new Ctx();  // never destroyed: HIP may be torn down first at exit
+    return *c;
+}
+
+static void ensure_device(Ctx& c) {
+    int dev = 0;
+    S2_HIP(hipGetDevice(&dev));
+    if (c.stream && c.device == dev) return;
+    if (c.stream) {
+        // the current device changed: drop per-device state
+        for (auto& e : c.tables) tables_destroy(e.t);
+        for (auto& e : c.consts) consts_destroy(e.c);
+        c.tables.clear();
+        c.consts.clear();
+        for (DBuf* b : {&c.re, &c.im, &c.cre, &c.cim, &c.X, &c.Cb, &c.aux1, &c.aux2, &c.aux3, &c.aux4})
+            b->release();
+    }
+    c.device = dev;
+    S2_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+}
+
+static unsigned long long fingerprint(const double* p, long n) {
+    unsigned long long h = 1469598103934665603ULL ^ (unsigned long long)n;
+    if (n <= 0) return h;
+    const long step = std::max(1L, n / 64);
+    for (long i = 0; i < n; i += step) {
+        unsigned long long v;
+        std::memcpy(&v, p + i, 8);
+        h = (h ^ v) * 1099511628211ULL;
+    }
+    unsigned long long v;
+    std::memcpy(&v, p + n - 1, 8);
+    return (h ^ v) * 1099511628211ULL;
+}
+
+static long order_len(int m, int bw, int cutoff) {
+    return (m < cutoff) ? table_size(m, bw) : 2L * bw * (bw - m);
+}
+
+// Device copy of a caller's table set (one direction).  norders = bw, or 1
+// for the order-0 table of FZTSemiMemo.
+static const DevTables& device_tables(Ctx& c, int bw, int cutoff, int dir, const double* const* ptrs,
+                                      int norders) {
+    std::vector<unsigned long long> fp(norders);
+    for (int m = 0; m < norders; ++m) {
+        if (!ptrs[m]) throw ArgError{"table pointer for order " + std::to_string(m) + " is NULL"};
+        fp[m] = fingerprint(ptrs[m], order_len(m, bw, cutoff));
+    }
+    for (auto it = c.tables.begin(); it != c.tables.end(); ++it) {
+        if (it->bw != bw || it->cutoff != cutoff || it->dir != dir || it->device != c.device) continue;
+        if ((int)it->ptrs.size() != norders || !std::equal(it->ptrs.begin(), it->ptrs.end(), ptrs)) continue;
+        if (it->fp != fp) continue;
+        c.tables.splice(c.tables.begin(), c.tables, it);
+        return c.tables.front().t;
+    }
+    CachedTables e;
+    e.bw = bw;
+    e.cutoff = cutoff;
+    e.dir = dir;
+    e.device = c.device;
+    e.ptrs.assign(ptrs, ptrs + norders);
+    e.fp = fp;
+    e.t.bw = bw;
+    e.t.cutoff = cutoff;
+    e.t.h_off.assign(bw, 0);
+    long pos = 0;
+    for (int m = 0; m < bw; ++m) {
+        e.t.h_off[m] = pos;
+        if (m < norders) pos += order_len(m, bw, cutoff);
+    }
+    e.t.total = pos;
+    double* d = nullptr;
+    S2_HIP(hipMalloc(&d, sizeof(double) * std::max(1L, pos)));
+    for (int m = 0; m < norders; ++m)
+        S2_HIP(hipMemcpyAsync(d + e.t.h_off[m], ptrs[m], sizeof(double) * order_len(m, bw, cutoff),
+                              hipMemcpyHostToDevice, c.stream));
+    S2_HIP(hipMalloc(&e.t.off, sizeof(long) * bw));
+    S2_HIP(hipMemcpyAsync(e.t.off, e.t.h_off.data(), sizeof(long) * bw, hipMemcpyHostToDevice, c.stream));
+    S2_HIP(hipStreamSynchronize(c.stream));
+    if (dir == 1)
+        e.t.inv = d;
+    else
+        e.t.fwd = d;
+    e.t.owned = true;
+    // evict least recently used entries above the budget
+    size_t used = sizeof(double) * pos;
+    for (auto& x : c.tables) used += sizeof(double) * x.t.total;
+    while (!c.tables.empty() && used > c.table_budget) {
+        used -= sizeof(double) * c.tables.back().t.total;
+        tables_destroy(c.tables.back().t);
+        c.tables.pop_back();
+    }
+    c.tables.push_front(std::move(e));
+    return c.tables.front().t;
+}
+
+// Register a device table set that this library produced for the caller's
+// host copy (so the first transform finds it without an upload).
+static void register_tables(Ctx& c, DevTables&& t, int dir, double** ptrs, int norders) {
+    CachedTables e;
+    e.bw = t.bw;
+    e.cutoff = t.cutoff;
+    e.dir = dir;
+    e.device = c.device;
+    e.ptrs.assign(ptrs, ptrs + norders);
+    e.fp.resize(norders);
+    for (int m = 0; m < norders; ++m) e.fp[m] = fingerprint(ptrs[m], order_len(m, t.bw, t.cutoff));
+    if (dir == 1 && t.fwd) {
+        (void)hipFree(t.fwd);
+        t.fwd = nullptr;
+    }
+    if (dir == 0 && t.inv) {
+        (void)hipFree(t.inv);
+        t.inv = nullptr;
+    }
+    e.t = std::move(t);
+    c.tables.push_front(std::move(e));
+}
+
+// Prescaled weights and twiddles for bw; `weights` (4bw, host) if the caller
+// supplied them, else the GPU-generated ones.
+static const DevConsts& device_consts(Ctx& c, int bw, const double* weights) {
+    for (auto it = c.consts.begin(); it != c.consts.end(); ++it) {
+        if (it->bw != bw || it->device != c.device) continue;
+        bool same = weights ? (it->weights.size() == (size_t)4 * bw &&
+                               std::memcmp(it->weights.data(), weights, sizeof(double) * 4 * bw) == 0)
+                            : it->weights.empty();
+        if (!same) continue;
+        c.consts.splice(c.consts.begin(), c.consts, it);
+        return c.consts.front().c;
+    }
+    CachedConsts e;
+    e.bw = bw;
+    e.device = c.device;
+    if (weights) e.weights.assign(weights, weights + 4 * bw);
+    consts_create(e.c, bw, weights, c.stream);
+    c.consts.push_front(std::move(e));
+    while (c.consts.size() > 8) {
+        consts_destroy(c.consts.back().c);
+        c.consts.pop_back();
+    }
+    return c.consts.front().c;
+}
+
+static void check_common(int bw, int fmt) {
+    if (bw < 2 || bw > 1024) throw ArgError{"bw must be in [2, 1024], got " + std::to_string(bw)};
+    if (fmt != 0 && fmt != 1) throw ArgError{"data_format must be COMPLEX (0) or REAL (1)"};
+}
+
+static void need(const void* p, const char* what) {
+    if (!p) throw ArgError{std::string(what) + " is NULL"};
+}
+
+template <typename F>
+static void run_dropin(const char* fn, F&& body) {
+    Ctx& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    try {
+        body(c);  // validates its arguments, then calls ensure_device()
+        if (c.stream) S2_HIP(hipStreamSynchronize(c.stream));
+    } catch (const ArgError& e) {
+        dropin_fail(fn, e.msg);
+    } catch (const HipError& e) {
+        dropin_fail(fn, std::string(hipGetErrorString(e.code)) + " (" + e.where + ")");
+    } catch (const std::bad_alloc&) {
+        dropin_fail(fn, "host allocation failed");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched plans
+// ---------------------------------------------------------------------------
+struct Plan {
+    int bw, cutoff, max_fields, precision, device;
+    DevConsts c;
+    DevTables t;
+    DBuf X, Cb, cr, ci, zr, zi;
+};
+
+static int plan_status(const char* fn, const std::exception_ptr& ep) {
+    try {
+        std::rethrow_exception(ep);
+    } catch (const ArgError& e) {
+        set_error(std::string(fn) + ": " + e.msg);
+        return S2AMD_EINVAL;
+    } catch (const HipError& e) {
+        set_error(std::string(fn) + ": " + hipGetErrorString(e.code) + " (" + e.where + ")");
+        return e.code == hipErrorOutOfMemory ? S2AMD_ENOMEM : S2AMD_EDEVICE;
+    } catch (const std::bad_alloc&) {
+        set_error(std::string(fn) + ": host allocation failed");
+        return S2AMD_ENOMEM;
+    } catch (...) {
+        set_error(std::string(fn) + ": unexpected error");
+        return S2AMD_EDEVICE;
+    }
+}
+
+template <typename F>
+static int run_plan(const char* fn, F&& body) {
+    try {
+        body();
+        return S2AMD_OK;
+    } catch (...) {
+        return plan_status(fn, std::current_exception());
+    }
+}
+
+static void plan_check_call(const Plan* p, int nfields, int fmt) {
+    if (!p) throw ArgError{"plan is NULL"};
+    if (nfields < 0 || nfields > p->max_fields)
+        throw ArgError{"nfields must be in [0, " + std::to_string(p->max_fields) + "]"};
+    if (fmt != 0 && fmt != 1) throw ArgError{"data_format must be 0 or 1"};
+    int dev = -1;
+    S2_HIP(hipGetDevice(&dev));
+    if (dev != p->device) throw ArgError{"plan was created on another device"};
+}
+
+}  // namespace s2
+
+using namespace s2;
+
+// ===========================================================================
+// drop-in: layout arithmetic (host)
+// ===========================================================================
+extern "C" int TableSize(const int m, const int bw) { return s2::table_size(m, bw); }
+
+extern "C" int Spharmonic_TableSize(const int bw) {
+    if (bw <= 512) return ((4 * bw * bw * bw) + (6 * bw * bw) - (8 * bw)) / 24 + bw;
+    return Reduced_SpharmonicTableSize(bw, bw);
+}
+
+extern "C" int Reduced_SpharmonicTableSize(const int bw, const int m) {
+    int s = 0;
+    for (int i = 0; i < m; ++i) s += s2::table_size(i, bw);
+    return s;
+}
+
+extern "C" int Reduced_Naive_TableSize(const int bw, const int m) {
+    int s = 0;
+    for (int i = m; i < bw; ++i) s += bw - i;
+    return 2 * bw * s;
+}
+
+extern "C" int TableOffset(int m, int l) { return s2::table_offset(m, l); }
+extern "C" int RowSize(const int m, const int l) { return s2::row_size(m, l); }
+extern "C" int Transpose_RowSize(const int row, const int m, const int bw) {
+    return s2::transpose_row_size(row, m, bw);
+}
+extern "C" int IndexOfHarmonicCoeff(const int m, const int l, const int bw) { return s2::index_of_coeff(m, l, bw); }
+
+// ===========================================================================
+// drop-in: tables and weights (GPU, copied back to the caller's buffers)
+// ===========================================================================
+extern "C" void GenerateWeightsForDLT(const int bw, double* weights) {
+    run_dropin("GenerateWeightsForDLT", [&](Ctx& c) {
+        check_common(bw, 0);
+        need(weights, "weights");
+        ensure_device(c);
+        c.aux1.reserve(sizeof(double) * 4 * bw);
+        gen_weights(bw, c.aux1.as<double>(), c.stream);
+        S2_HIP(hipMemcpyAsync(weights, c.aux1.p, sizeof(double) * 4 * bw, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void GenerateCosPmlTable(const int bw, const int m, double* tablespace, double* workspace) {
+    (void)workspace;
+    run_dropin("GenerateCosPmlTable", [&](Ctx& c) {
+        check_common(bw, 0);
+        if (m < 0 || m >= bw) throw ArgError{"m must be in [0, bw)"};
+        need(tablespace, "tablespace");
+        const long n = table_size(m, bw);
+        ensure_device(c);
+        c.aux1.reserve(sizeof(double) * n);
+        gen_cospml_one(bw, m, c.aux1.as<double>(), c.stream);
+        S2_HIP(hipMemcpyAsync(tablespace, c.aux1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void TransposeCosPmlTable(const int bw, const int m, double* cos_pml_table, double* result) {
+    run_dropin("TransposeCosPmlTable", [&](Ctx& c) {
+        check_common(bw, 0);
+        if (m < 0 || m >= bw) throw ArgError{"m must be in [0, bw)"};
+        need(cos_pml_table, "cos_pml_table");
+        need(result, "result");
+        const long n = table_size(m, bw);
+        ensure_device(c);
+        c.aux1.reserve(sizeof(double) * n);
+        c.aux2.reserve(sizeof(double) * n);
+        S2_HIP(hipMemcpyAsync(c.aux1.p, cos_pml_table, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        transpose_cospml_one(bw, m, c.aux1.as<double>(), c.aux2.as<double>(), c.stream);
+        S2_HIP(hipMemcpyAsync(result, c.aux2.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void GeneratePmlTable(const int bw, const int m, double* pml_table, double* workspace) {
+    (void)workspace;
+    run_dropin("GeneratePmlTable", [&](Ctx& c) {
+        check_common(bw, 0);
+        if (m < 0 || m >= bw) throw ArgError{"m must be in [0, bw)"};
+        need(pml_table, "pml_table");
+        const long n = 2L * bw * (bw - m);
+        ensure_device(c);
+        c.aux1.reserve(sizeof(double) * n);
+        gen_pml_one(bw, m, c.aux1.as<double>(), c.stream);
+        S2_HIP(hipMemcpyAsync(pml_table, c.aux1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+// Generate (bw, cutoff) tables on the GPU, copy direction `dir` into the
+// caller's resultspace, return its malloc'd pointer array (nptr entries),
+// and keep the device copy registered.
+static double** produce_tables(const char* fn, int bw, int cutoff, int dir, double* resultspace, int nptr) {
+    double** out = nullptr;
+    run_dropin(fn, [&](Ctx& c) {
+        check_common(bw, 0);
+        if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+        need(resultspace, "resultspace");
+        ensure_device(c);
+        DevTables t;
+        tables_generate(t, bw, cutoff, dir == 0, dir == 1, c.stream);
+        const double* src = dir == 0 ? t.fwd : t.inv;
+        S2_HIP(hipMemcpyAsync(resultspace, src, sizeof(double) * t.total, hipMemcpyDeviceToHost, c.stream));
+        S2_HIP(hipStreamSynchronize(c.stream));
+        out = static_cast<double**>(std::malloc(sizeof(double*) * nptr));
+        if (!out) throw std::bad_alloc();
+        for (int m = 0; m < nptr; ++m) out[m] = (m < bw) ? resultspace + t.h_off[m] : nullptr;
+        register_tables(c, std::move(t), dir, out, bw);
+    });
+    return out;
+}
+
+extern "C" double** Spharmonic_Pml_Table(const int bw, double* resultspace, double* workspace) {
+    (void)workspace;
+    return produce_tables("Spharmonic_Pml_Table", bw, bw, 0, resultspace, bw);
+}
+
+extern "C" double** SemiNaive_Naive_Pml_Table(const int bw, const int m, double* resultspace, double* workspace) {
+    (void)workspace;
+    return produce_tables("SemiNaive_Naive_Pml_Table", bw, m, 0, resultspace, bw + 1);
+}
+
+// Transposed tables from the caller's forward tables: seminaive orders are
+// transposed on the GPU from the caller's values; naive orders are
+// regenerated (cospml.c:506-515 regenerates them too).
+static double** transpose_tables(const char* fn, double** table, int bw, int cutoff, double* resultspace, int nptr) {
+    double** out = nullptr;
+    run_dropin(fn, [&](Ctx& c) {
+        check_common(bw, 0);
+        if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+        need(table, "table");
+        need(resultspace, "resultspace");
+        ensure_device(c);
+        const DevTables& src = device_tables(c, bw, cutoff, 0, table, bw);
+        DevTables t;
+        t.bw = bw;
+        t.cutoff = cutoff;
+        t.h_off = src.h_off;
+        t.total = src.total;
+        S2_HIP(hipMalloc(&t.off, sizeof(long) * bw));
+        S2_HIP(hipMemcpyAsync(t.off, t.h_off.data(), sizeof(long) * bw, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMalloc(&t.inv, sizeof(double) * std::max(1L, t.total)));
+        for (int m = 0; m < bw; ++m) {
+            if (m < cutoff)
+                transpose_cospml_one(bw, m, src.fwd + src.h_off[m], t.inv + t.h_off[m], c.stream);
+            else
+                gen_pml_one(bw, m, t.inv + t.h_off[m], c.stream);
+        }
+        S2_HIP(hipMemcpyAsync(resultspace, t.inv, sizeof(double) * t.total, hipMemcpyDeviceToHost, c.stream));
+        S2_HIP(hipStreamSynchronize(c.stream));
+        out = static_cast<double**>(std::malloc(sizeof(double*) * nptr));
+        if (!out) throw std::bad_alloc();
+        for (int m = 0; m < nptr; ++m) out[m] = (m < bw) ? resultspace + t.h_off[m] : nullptr;
+        register_tables(c, std::move(t), 1, out, bw);
+    });
+    return out;
+}
+
+extern "C" double** Transpose_Spharmonic_Pml_Table(double** spharmonic_pml_table, const int bw, double* resultspace) {
+    return transpose_tables("Transpose_Spharmonic_Pml_Table", spharmonic_pml_table, bw, bw, resultspace, bw);
+}
+
+extern "C" double** Transpose_SemiNaive_Naive_Pml_Table(double** seminaive_naive_pml_table, const int bw,
+                                                        const int m, double* resultspace, double* workspace) {
+    (void)workspace;
+    return transpose_tables("Transpose_SemiNaive_Naive_Pml_Table", seminaive_naive_pml_table, bw, m, resultspace,
+                            bw + 1);
+}
+
+// ===========================================================================
+// drop-in: transforms
+// ===========================================================================
+extern "C" void FSTSemiMemo(double* rdata, double* idata, double* rcoeffs, double* icoeffs, const int bw,
+                            double** seminaive_naive_table, double* workspace, DataFormat data_format,
+                            const int cutoff, fftw_plan* DCT_plan, fftw_plan* FFT_plan, double* weights) {
+    (void)workspace;
+    (void)DCT_plan;
+    (void)FFT_plan;
+    run_dropin("FSTSemiMemo", [&](Ctx& c) {
+        check_common(bw, (int)data_format);
+        if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+        need(rdata, "rdata");
+        need(idata, "idata");
+        need(rcoeffs, "rcoeffs");
+        need(icoeffs, "icoeffs");
+        need(seminaive_naive_table, "seminaive_naive_table");
+        need(weights, "weights");
+        const int N = 2 * bw;
+        const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
+        ensure_device(c);
+        const DevTables& t = device_tables(c, bw, cutoff, 0, seminaive_naive_table, bw);
+        const DevConsts& k0 = device_consts(c, bw, weights);
+        c.re.reserve(g);
+        c.im.reserve(g);
+        c.cre.reserve(k);
+        c.cim.reserve(k);
+        c.X.reserve(2 * g);
+        c.Cb.reserve(sizeof(double2) * N * bw);
+        S2_HIP(hipMemcpyAsync(c.re.p, rdata, g, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.im.p, idata, g, hipMemcpyHostToDevice, c.stream));
+        forward_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.cre.as<double>(), c.cim.as<double>(), 1,
+                     (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+        S2_HIP(hipMemcpyAsync(rcoeffs, c.cre.p, k, hipMemcpyDeviceToHost, c.stream));
+        S2_HIP(hipMemcpyAsync(icoeffs, c.cim.p, k, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void InvFSTSemiMemo(double* rcoeffs, double* icoeffs, double* rdata, double* idata, const int bw,
+                               double** transpose_seminaive_naive_table, double* workspace,
+                               DataFormat data_format, const int cutoff, fftw_plan* inv_DCT_plan,
+                               fftw_plan* inv_FFT_plan) {
+    (void)workspace;
+    (void)inv_DCT_plan;
+    (void)inv_FFT_plan;
+    run_dropin("InvFSTSemiMemo", [&](Ctx& c) {
+        check_common(bw, (int)data_format);
+        if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+        need(rdata, "rdata");
+        need(idata, "idata");
+        need(rcoeffs, "rcoeffs");
+        need(icoeffs, "icoeffs");
+        need(transpose_seminaive_naive_table, "transpose_seminaive_naive_table");
+        const int N = 2 * bw;
+        const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
+        ensure_device(c);
+        const DevTables& t = device_tables(c, bw, cutoff, 1, transpose_seminaive_naive_table, bw);
+        const DevConsts& k0 = device_consts(c, bw, nullptr);
+        c.re.reserve(g);
+        c.im.reserve(g);
+        c.cre.reserve(k);
+        c.cim.reserve(k);
+        c.X.reserve(2 * g);
+        c.Cb.reserve(sizeof(double2) * N * bw);
+        S2_HIP(hipMemcpyAsync(c.cre.p, rcoeffs, k, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.cim.p, icoeffs, k, hipMemcpyHostToDevice, c.stream));
+        inverse_fp64(k0, t, c.cre.as<double>(), c.cim.as<double>(), c.re.as<double>(), c.im.as<double>(), 1,
+                     (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+        S2_HIP(hipMemcpyAsync(rdata, c.re.p, g, hipMemcpyDeviceToHost, c.stream));
+        S2_HIP(hipMemcpyAsync(idata, c.im.p, g, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void FZTSemiMemo(double* rdata, double* idata, double* rres, double* ires, const int bw,
+                            double* cos_pml_table, double* workspace, DataFormat data_format, fftw_plan* DCT_plan,
+                            double* weights) {
+    (void)workspace;
+    (void)DCT_plan;
+    run_dropin("FZTSemiMemo", [&](Ctx& c) {
+        check_common(bw, (int)data_format);
+        need(rdata, "rdata");
+        need(idata, "idata");
+        need(rres, "rres");
+        need(ires, "ires");
+        need(cos_pml_table, "cos_pml_table");
+        need(weights, "weights");
+        const int N = 2 * bw;
+        const size_t g = sizeof(double) * N * N;
+        const double* p0 = cos_pml_table;
+        ensure_device(c);
+        const DevTables& t = device_tables(c, bw, bw, 2, &p0, 1);
+        const DevConsts& k0 = device_consts(c, bw, weights);
+        c.re.reserve(g);
+        c.im.reserve(g);
+        c.cre.reserve(sizeof(double) * bw);
+        c.cim.reserve(sizeof(double) * bw);
+        c.X.reserve(2 * g);
+        c.Cb.reserve(sizeof(double2) * N * bw);
+        S2_HIP(hipMemcpyAsync(c.re.p, rdata, g, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.im.p, idata, g, hipMemcpyHostToDevice, c.stream));
+        zonal_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.cre.as<double>(), c.cim.as<double>(), 1,
+                   (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+        S2_HIP(hipMemcpyAsync(rres, c.cre.p, sizeof(double) * bw, hipMemcpyDeviceToHost, c.stream));
+        S2_HIP(hipMemcpyAsync(ires, c.cim.p, sizeof(double) * bw, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void TransMult(double* rdatacoeffs, double* idatacoeffs, double* rfiltercoeffs, double* ifiltercoeffs,
+                          double* rres, double* ires, const int bw) {
+    run_dropin("TransMult", [&](Ctx& c) {
+        check_common(bw, 0);
+        for (const void* p : {(const void*)rdatacoeffs, (const void*)idatacoeffs, (const void*)rfiltercoeffs,
+                              (const void*)ifiltercoeffs, (const void*)rres, (const void*)ires})
+            need(p, "coefficient array");
+        const size_t k = sizeof(double) * bw * bw, h = sizeof(double) * bw;
+        ensure_device(c);
+        const DevConsts& k0 = device_consts(c, bw, nullptr);
+        c.cre.reserve(k);
+        c.cim.reserve(k);
+        c.aux1.reserve(h);
+        c.aux2.reserve(h);
+        c.aux3.reserve(k);
+        c.aux4.reserve(k);
+        S2_HIP(hipMemcpyAsync(c.cre.p, rdatacoeffs, k, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.cim.p, idatacoeffs, k, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux1.p, rfiltercoeffs, h, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux2.p, ifiltercoeffs, h, hipMemcpyHostToDevice, c.stream));
+        trans_mult(k0, c.cre.as<double>(), c.cim.as<double>(), c.aux1.as<double>(), c.aux2.as<double>(), 0,
+                   c.aux3.as<double>(), c.aux4.as<double>(), 1, c.stream);
+        S2_HIP(hipMemcpyAsync(rres, c.aux3.p, k, hipMemcpyDeviceToHost, c.stream));
+        S2_HIP(hipMemcpyAsync(ires, c.aux4.p, k, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+// Library-generated (bw, cutoff = bw) tables for ConvOn2SphereSemiMemo.
+static const DevTables& conv_tables(Ctx& c, int bw) {
+    for (auto it = c.tables.begin(); it != c.tables.end(); ++it)
+        if (it->bw == bw && it->cutoff == bw && it->dir == 3 && it->device == c.device) {
+            c.tables.splice(c.tables.begin(), c.tables, it);
+            return c.tables.front().t;
+        }
+    CachedTables e;
+    e.bw = bw;
+    e.cutoff = bw;
+    e.dir = 3;
+    e.device = c.device;
+    tables_generate(e.t, bw, bw, true, true, c.stream);
+    c.tables.push_front(std::move(e));
+    return c.tables.front().t;
+}
+
+extern "C" void ConvOn2SphereSemiMemo(double* rdata, double* idata, double* rfilter, double* ifilter, double* rres,
+                                      double* ires, const int bw, double* workspace) {
+    (void)workspace;
+    run_dropin("ConvOn2SphereSemiMemo", [&](Ctx& c) {
+        check_common(bw, 0);
+        for (const void* p : {(const void*)rdata, (const void*)idata, (const void*)rfilter, (const void*)ifilter,
+                              (const void*)rres, (const void*)ires})
+            need(p, "grid array");
+        const int N = 2 * bw;
+        const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
+        ensure_device(c);
+        const DevTables& t = conv_tables(c, bw);
+        const DevConsts& k0 = device_consts(c, bw, nullptr);
+        c.re.reserve(g);
+        c.im.reserve(g);
+        c.cre.reserve(k);
+        c.cim.reserve(k);
+        c.X.reserve(2 * g);
+        c.Cb.reserve(sizeof(double2) * N * bw);
+        c.aux1.reserve(sizeof(double) * bw);
+        c.aux2.reserve(sizeof(double) * bw);
+        c.aux3.reserve(k);
+        c.aux4.reserve(k);
+        // signal: FSTSemiMemo(REAL, cutoff = bw)
+        S2_HIP(hipMemcpyAsync(c.re.p, rdata, g, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.im.p, idata, g, hipMemcpyHostToDevice, c.stream));
+        forward_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.cre.as<double>(), c.cim.as<double>(), 1,
+                     Fmt::Real, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+        // filter: FZTSemiMemo(REAL)
+        S2_HIP(hipMemcpyAsync(c.re.p, rfilter, g, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.im.p, ifilter, g, hipMemcpyHostToDevice, c.stream));
+        zonal_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.aux1.as<double>(), c.aux2.as<double>(), 1,
+                   Fmt::Real, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+        trans_mult(k0, c.cre.as<double>(), c.cim.as<double>(), c.aux1.as<double>(), c.aux2.as<double>(), 0,
+                   c.aux3.as<double>(), c.aux4.as<double>(), 1, c.stream);
+        inverse_fp64(k0, t, c.aux3.as<double>(), c.aux4.as<double>(), c.re.as<double>(), c.im.as<double>(), 1,
+                     Fmt::Real, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+        S2_HIP(hipMemcpyAsync(rres, c.re.p, g, hipMemcpyDeviceToHost, c.stream));
+        S2_HIP(hipMemcpyAsync(ires, c.im.p, g, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+// ===========================================================================
+// batched device API
+// ===========================================================================
+struct s2amd_plan : s2::Plan {};
+
+extern "C" void s2amd_set_abort_on_error(int on) { g_abort_on_error = on ? 1 : 0; }
+extern "C" const char* s2amd_last_error(void) { return s2::last_error(); }
+
+extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_fields, int precision) {
+    return run_plan("s2amd_plan_create", [&] {
+        if (!out) throw ArgError{"plan output pointer is NULL"};
+        *out = nullptr;
+        check_common(bw, 0);
+        if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+        if (max_fields < 1) throw ArgError{"max_fields must be >= 1"};
+        if (precision != S2AMD_FP64) throw ArgError{"unknown precision"};
+        std::unique_ptr<s2amd_plan> p(new s2amd_plan());
+        p->bw = bw;
+        p->cutoff = cutoff;
+        p->max_fields = max_fields;
+        p->precision = precision;
+        S2_HIP(hipGetDevice(&p->device));
+        hipStream_t s = nullptr;
+        S2_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            consts_create(p->c, bw, nullptr, s);
+            tables_generate(p->t, bw, cutoff, true, true, s);
+            const size_t N = 2 * (size_t)bw, F = max_fields;
+            p->X.reserve(sizeof(double2) * N * N * F);
+            p->Cb.reserve(sizeof(double2) * N * bw * F);
+            p->cr.reserve(sizeof(double) * bw * bw * F);
+            p->ci.reserve(sizeof(double) * bw * bw * F);
+            p->zr.reserve(sizeof(double) * bw * F);
+            p->zi.reserve(sizeof(double) * bw * F);
+            S2_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            consts_destroy(p->c);
+            tables_destroy(p->t);
+            for (DBuf* b : {&p->X, &p->Cb, &p->cr, &p->ci, &p->zr, &p->zi}) b->release();
+            throw;
+        }
+        S2_HIP(hipStreamDestroy(s));
+        *out = p.release();
+    });
+}
+
+extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
+    return run_plan("s2amd_plan_destroy", [&] {
+        if (!p) return;
+        consts_destroy(p->c);
+        tables_destroy(p->t);
+        for (DBuf* b : {&p->X, &p->Cb, &p->cr, &p->ci, &p->zr, &p->zi}) b->release();
+        delete p;
+    });
+}
+
+extern "C" size_t s2amd_plan_bytes(const s2amd_plan* p) {
+    if (!p) return 0;
+    return 2 * sizeof(double) * p->t.total + p->X.n + p->Cb.n + p->cr.n + p->ci.n + p->zr.n + p->zi.n;
+}
+
+extern "C" int s2amd_plan_tables(const s2amd_plan* p, const double** fwd, const double** inv, long* n) {
+    return run_plan("s2amd_plan_tables", [&] {
+        if (!p) throw ArgError{"plan is NULL"};
+        if (fwd) *fwd = p->t.fwd;
+        if (inv) *inv = p->t.inv;
+        if (n) *n = p->t.total;
+    });
+}
+
+extern "C" int s2amd_forward(s2amd_plan* p, const double* re, const double* im, double* cre, double* cim,
+                             int nfields, int data_format, void* stream) {
+    return run_plan("s2amd_forward", [&] {
+        plan_check_call(p, nfields, data_format);
+        if (nfields == 0) return;
+        need(re, "re");
+        need(im, "im");
+        need(cre, "cre");
+        need(cim, "cim");
+        forward_fp64(p->c, p->t, re, im, cre, cim, nfields, (Fmt)data_format, p->X.as<double2>(),
+                     p->Cb.as<double2>(), (hipStream_t)stream);
+    });
+}
+
+extern "C" int s2amd_inverse(s2amd_plan* p, const double* cre, const double* cim, double* re, double* im,
+                             int nfields, int data_format, void* stream) {
+    return run_plan("s2amd_inverse", [&] {
+        plan_check_call(p, nfields, data_format);
+        if (nfields == 0) return;
+        need(re, "re");
+        need(im, "im");
+        need(cre, "cre");
+        need(cim, "cim");
+        inverse_fp64(p->c, p->t, cre, cim, re, im, nfields, (Fmt)data_format, p->X.as<double2>(),
+                     p->Cb.as<double2>(), (hipStream_t)stream);
+    });
+}
+
+extern "C" int s2amd_zonal(s2amd_plan* p, const double* re, const double* im, double* zr, double* zi, int nfields,
+                           int data_format, void* stream) {
+    return run_plan("s2amd_zonal", [&] {
+        plan_check_call(p, nfields, data_format);
+        if (nfields == 0) return;
+        need(re, "re");
+        need(im, "im");
+        need(zr, "zr");
+        need(zi, "zi");
+        zonal_fp64(p->c, p->t, re, im, zr, zi, nfields, (Fmt)data_format, p->X.as<double2>(), p->Cb.as<double2>(),
+                   (hipStream_t)stream);
+    });
+}
+
+extern "C" int s2amd_conv(s2amd_plan* p, const double* sre, const double* sim, const double* hre,
+                          const double* him, double* ore, double* oim, int nfields, void* stream) {
+    return run_plan("s2amd_conv", [&] {
+        plan_check_call(p, nfields, 1);
+        if (nfields == 0) return;
+        if (p->cutoff != p->bw) throw ArgError{"convolution needs a plan with cutoff = bw"};
+        for (const void* q : {(const void*)sre, (const void*)sim, (const void*)hre, (const void*)him,
+                              (const void*)ore, (const void*)oim})
+            need(q, "grid array");
+        hipStream_t s = (hipStream_t)stream;
+        double* cr = p->cr.as<double>();
+        double* ci = p->ci.as<double>();
+        forward_fp64(p->c, p->t, sre, sim, cr, ci, nfields, Fmt::Real, p->X.as<double2>(), p->Cb.as<double2>(), s);
+        zonal_fp64(p->c, p->t, hre, him, p->zr.as<double>(), p->zi.as<double>(), nfields, Fmt::Real,
+                   p->X.as<double2>(), p->Cb.as<double2>(), s);
+        trans_mult(p->c, cr, ci, p->zr.as<double>(), p->zi.as<double>(), p->bw, cr, ci, nfields, s);
+        inverse_fp64(p->c, p->t, cr, ci, ore, oim, nfields, Fmt::Real, p->X.as<double2>(), p->Cb.as<double2>(), s);
+    });
+}

@@ -1,0 +1,922 @@
+// HIP kernels of the memoised seminaive SHT for gfx950 (MI355X).
+//
+// Forward, per field (FSTSemiMemo, FST_semi_memo.c:68-202):
+//   k_phi_fwd   ring FFT along phi, order-major (transposed) store     (:81)
+//   k_dct_fwd   quadrature weights + DCT-II along theta per order row  (seminaive.c:161-176)
+//   k_leg_fwd   projections on the zero-stripped cosine tables         (seminaive.c:183-197)
+//   k_naive_fwd dense projection on sampled P_l^m for orders >= cutoff (naive.c:34-59)
+// Inverse (InvFSTSemiMemo, :228-351):
+//   k_leg_inv   cosine sums against the transposed tables              (seminaive.c:74-98)
+//   k_dct_inv   DCT-III, x sin for odd orders, signs, 1/sqrt(2pi)      (seminaive.c:107-114, :301-348)
+//   k_naive_inv dense synthesis for orders >= cutoff                   (naive.c:78-95)
+//   k_phi_inv   inverse ring FFT along phi                             (:350)
+// Tables (cospml.c:161-362, pml.c:41-79, weights.c:32-46) are generated on
+// the GPU in FP64 by k_gen_cospml / k_transpose / k_gen_pml / k_weights.
+//
+// FFTs are Stockham radix-2 passes in LDS for power-of-two lengths and a
+// direct DFT otherwise.  Both DCTs use one complex FFT of length N = 2B on a
+// reordered sequence (Makhoul); the DCTs are real-linear with real kernels,
+// so one complex row carries the real and imaginary transforms together.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "s2_internal.h"
+
+namespace s2 {
+
+// ---------------------------------------------------------------------------
+// complex helpers (double2 / float2)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Real;
+template <>
+struct Real<double2> {
+    using type = double;
+};
+template <>
+struct Real<float2> {
+    using type = float;
+};
+
+template <typename T>
+__device__ __forceinline__ T mk(typename Real<T>::type x, typename Real<T>::type y) {
+    T r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ T cadd(T a, T b) {
+    return mk<T>(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__device__ __forceinline__ T csub(T a, T b) {
+    return mk<T>(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__device__ __forceinline__ T cmul(T a, T b) {
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ T cconj(T a) {
+    return mk<T>(a.x, -a.y);
+}
+template <typename T>
+__device__ __forceinline__ T cscale(T a, typename Real<T>::type s) {
+    return mk<T>(a.x * s, a.y * s);
+}
+
+// ---------------------------------------------------------------------------
+// block-wide FFTs in LDS
+// ---------------------------------------------------------------------------
+
+// nfft independent length-N transforms stored back to back in a (b is
+// scratch of the same size).  SIGN = -1: forward, +1: inverse (unnormalised).
+// tw[t] = exp(-2 pi i t / N).  Returns the buffer holding the result.
+template <typename T, int SIGN>
+__device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T* __restrict__ tw) {
+    if (pow2) {
+        const int half = N >> 1;
+        const int total = nfft * half;
+        for (int s = 0, Ns = 1; s < logN; ++s, Ns <<= 1) {
+            const int tstride = half >> s;  // N / (2 Ns)
+            for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+                const int f = idx >> (logN - 1);
+                const int j = idx & (half - 1);
+                const T* in = a + f * N;
+                T* out = b + f * N;
+                const int k = j & (Ns - 1);
+                T w = tw[k * tstride];
+                if (SIGN > 0) w.y = -w.y;
+                const T x0 = in[j];
+                const T x1 = cmul(in[j + half], w);
+                const int o = ((j - k) << 1) + k;
+                out[o] = cadd(x0, x1);
+                out[o + Ns] = csub(x0, x1);
+            }
+            __syncthreads();
+            T* t = a;
+            a = b;
+            b = t;
+        }
+        return a;
+    }
+    // direct DFT for lengths that are not powers of two
+    const int total = nfft * N;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int f = idx / N, k = idx - f * N;
+        const T* in = a + f * N;
+        typename Real<T>::type sr = 0, si = 0;
+        int e = 0;
+        for (int j = 0; j < N; ++j) {
+            T w = tw[e];
+            if (SIGN > 0) w.y = -w.y;
+            const T v = in[j];
+            sr += v.x * w.x - v.y * w.y;
+            si += v.x * w.y + v.y * w.x;
+            e += k;
+            if (e >= N) e -= N;
+        }
+        b[f * N + k] = mk<T>(sr, si);
+    }
+    __syncthreads();
+    return b;
+}
+
+// Row of the order-major buffers that holds semi/naive row number q:
+// q < cut -> order q; then the negative orders -(cut-1) .. -1 in rows
+// N-cut+1 .. N-1 (rows bw+1 .. N-1 when cut = bw).
+__device__ __forceinline__ int row_of(int q, int N, int cut) { return q < cut ? q : N - 2 * cut + 1 + q; }
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+
+// grid (ceil(N/rpb), F): rpb rings of one field per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
+                                                 T* __restrict__ X, int N, int logN, int pow2, int rpb,
+                                                 const T* __restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* a = reinterpret_cast<T*>(smem);
+    T* b = a + rpb * N;
+    const int f = blockIdx.y, j0 = blockIdx.x * rpb;
+    const int nr = min(rpb, N - j0);
+    const size_t base = ((size_t)f * N + j0) * N;
+    for (int i = threadIdx.x; i < nr * N; i += blockDim.x)
+        a[i] = mk<T>((typename Real<T>::type)re[base + i], (typename Real<T>::type)im[base + i]);
+    __syncthreads();
+    T* r = block_fft<T, -1>(a, b, N, logN, pow2, nr, tw);
+    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
+        const int row = i / nr, jl = i - row * nr;
+        X[((size_t)f * N + row) * N + j0 + jl] = r[jl * N + row];
+    }
+}
+
+// Weighted DCT-II of every seminaive order row, first bw outputs kept.
+// grid (ceil(nrows/rpb), F).  w_semi holds the even then the odd weights,
+// premultiplied by sqrt(2 pi)/N (FFT scaling) and 1/sqrt(2N) (DCT scaling).
+template <typename T>
+__global__ __launch_bounds__(256) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
+                                                 const double* __restrict__ w_semi, const T* __restrict__ tw,
+                                                 const T* __restrict__ ph, int N, int logN, int pow2, int B,
+                                                 int cut, int nrows, int rpb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* a = reinterpret_cast<T*>(smem);
+    T* b = a + rpb * N;
+    const int f = blockIdx.y, q0 = blockIdx.x * rpb;
+    const int nr = min(rpb, nrows - q0);
+    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
+        const int rl = i / N, j = i - rl * N;
+        const int r = row_of(q0 + rl, N, cut);
+        const int am = r < B ? r : N - r;
+        const typename Real<T>::type w = (typename Real<T>::type)w_semi[(am & 1) * N + j];
+        const T x = cscale(X[((size_t)f * N + r) * N + j], w);
+        const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+        a[rl * N + p] = x;
+    }
+    __syncthreads();
+    T* V = block_fft<T, -1>(a, b, N, logN, pow2, nr, tw);
+    for (int i = threadIdx.x; i < nr * B; i += blockDim.x) {
+        const int rl = i / B, k = i - rl * B;
+        const int r = row_of(q0 + rl, N, cut);
+        const T* v = V + rl * N;
+        const T e = ph[k];
+        T c = cadd(cmul(e, v[k]), cmul(cconj(e), v[k ? N - k : 0]));
+        if (k == 0) c = cscale(c, (typename Real<T>::type)0.70710678118654752440);
+        Cb[((size_t)f * N + r) * B + k] = c;
+    }
+}
+
+// Seminaive projections of orders m < cut.  grid (cut, ceil(F/fc)); block m
+// handles rows m and N-m of fc fields.  zonal: write only degree l of order 0
+// into zr/zi [F][B] (FZTSemiMemo).
+__global__ __launch_bounds__(256) void k_leg_fwd(const double2* __restrict__ Cb, const double* __restrict__ T,
+                                                 const long* __restrict__ toff, double* __restrict__ cre,
+                                                 double* __restrict__ cim, int N, int B, int F, int fc, int fmt,
+                                                 int zonal) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* cs = reinterpret_cast<double2*>(smem);
+    const int m = blockIdx.x, f0 = blockIdx.y * fc;
+    const int nf = min(fc, F - f0);
+    const int nsign = (m > 0 && fmt == 0) ? 2 : 1;
+    for (int i = threadIdx.x; i < nf * nsign * B; i += blockDim.x) {
+        const int k = i % B, rest = i / B, s = rest % nsign, fl = rest / nsign;
+        const int r = s ? N - m : m;
+        cs[i] = Cb[((size_t)(f0 + fl) * N + r) * B + k];
+    }
+    __syncthreads();
+    const double* tm = T + toff[m];
+    const int L = B - m;
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    for (int it = threadIdx.x; it < L * nf * nsign; it += blockDim.x) {
+        const int li = it % L, rest = it / L, s = rest % nsign, fl = rest / nsign;
+        const int l = m + li;
+        const int rs = row_size(m, l);
+        const double* tp = tm + table_offset(m, l);
+        const double2* c = cs + (fl * nsign + s) * B + (li & 1);
+        double ar = 0.0, ai = 0.0;
+        for (int t = 0; t < rs; ++t) {
+            const double tv = tp[t];
+            const double2 cv = c[2 * t];
+            ar += cv.x * tv;
+            ai += cv.y * tv;
+        }
+        const int f = f0 + fl;
+        if (zonal) {
+            cre[(size_t)f * B + l] = ar;
+            cim[(size_t)f * B + l] = (fmt == 0) ? ai : 0.0;
+        } else if (s == 0) {
+            const int p = index_of_coeff(m, l, B);
+            cre[f * B2 + p] = ar;
+            cim[f * B2 + p] = ai;
+            if (fmt == 1 && m > 0) {  // REAL: f(l,-m) = (-1)^m conj f(l,m)
+                const int q = index_of_coeff(-m, l, B);
+                cre[f * B2 + q] = sg * ar;
+                cim[f * B2 + q] = -sg * ai;
+            }
+        } else {
+            const int q = index_of_coeff(-m, l, B);
+            cre[f * B2 + q] = sg * ar;
+            cim[f * B2 + q] = sg * ai;
+        }
+    }
+}
+
+// Naive projections of orders m >= cut (DLTNaive: even weights for every
+// order).  grid (B-cut, ceil(F/fc)).  w_naive = even weights * sqrt(2pi)/N.
+__global__ __launch_bounds__(256) void k_naive_fwd(const double2* __restrict__ X, const double* __restrict__ P,
+                                                   const long* __restrict__ toff,
+                                                   const double* __restrict__ w_naive, double* __restrict__ cre,
+                                                   double* __restrict__ cim, int N, int B, int cut, int F, int fc,
+                                                   int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* xs = reinterpret_cast<double2*>(smem);
+    const int m = cut + blockIdx.x, f0 = blockIdx.y * fc;
+    const int nf = min(fc, F - f0);
+    const int nsign = (m > 0 && fmt == 0) ? 2 : 1;
+    for (int i = threadIdx.x; i < nf * nsign * N; i += blockDim.x) {
+        const int j = i % N, rest = i / N, s = rest % nsign, fl = rest / nsign;
+        const int r = s ? N - m : m;
+        xs[i] = cscale(X[((size_t)(f0 + fl) * N + r) * N + j], w_naive[j]);
+    }
+    __syncthreads();
+    const double* pm = P + toff[m];
+    const int L = B - m;
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    for (int it = threadIdx.x; it < L * nf * nsign; it += blockDim.x) {
+        const int li = it % L, rest = it / L, s = rest % nsign, fl = rest / nsign;
+        const int l = m + li;
+        const double* pr = pm + (size_t)li * N;
+        const double2* x = xs + (fl * nsign + s) * N;
+        double ar = 0.0, ai = 0.0;
+        for (int j = 0; j < N; ++j) {
+            ar += x[j].x * pr[j];
+            ai += x[j].y * pr[j];
+        }
+        const int f = f0 + fl;
+        if (s == 0) {
+            const int p = index_of_coeff(m, l, B);
+            cre[f * B2 + p] = ar;
+            cim[f * B2 + p] = ai;
+            if (fmt == 1) {
+                const int q = index_of_coeff(-m, l, B);
+                cre[f * B2 + q] = sg * ar;
+                cim[f * B2 + q] = -sg * ai;
+            }
+        } else {
+            const int q = index_of_coeff(-m, l, B);
+            cre[f * B2 + q] = sg * ar;
+            cim[f * B2 + q] = sg * ai;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inverse
+// ---------------------------------------------------------------------------
+
+// Cosine coefficients of every seminaive order from the transposed tables.
+// grid (cut, ceil(F/fc)).  Output rows of Cb as the forward uses them.
+__global__ __launch_bounds__(256) void k_leg_inv(const double* __restrict__ cre, const double* __restrict__ cim,
+                                                 const double* __restrict__ TT, const long* __restrict__ toff,
+                                                 double2* __restrict__ Cb, int N, int B, int F, int fc, int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int m = blockIdx.x, f0 = blockIdx.y * fc;
+    const int nf = min(fc, F - f0);
+    const int nsign = (m > 0 && fmt == 0) ? 2 : 1;
+    const int L = B - m;
+    int* rowpos = reinterpret_cast<int*>(smem);                       // B+1 ints
+    double2* co = reinterpret_cast<double2*>(smem + ((B + 2) * sizeof(int) + 15) / 16 * 16);
+    const size_t B2 = (size_t)B * B;
+    for (int i = threadIdx.x; i < nf * nsign * L; i += blockDim.x) {
+        const int li = i % L, rest = i / L, s = rest % nsign, fl = rest / nsign;
+        const int p = index_of_coeff(s ? -m : m, m + li, B);
+        co[i] = make_double2(cre[(f0 + fl) * B2 + p], cim[(f0 + fl) * B2 + p]);
+    }
+    for (int i = threadIdx.x; i < B; i += blockDim.x) rowpos[i + 1] = transpose_row_size(i, m, B);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rowpos[0] = 0;
+        for (int i = 1; i <= B; ++i) rowpos[i] += rowpos[i - 1];
+    }
+    __syncthreads();
+    const double* tt = TT + toff[m];
+    const double scale = 0.5 / sqrt((double)B);
+    const double scale0 = 1.0 / sqrt((double)N);
+    for (int it = threadIdx.x; it < B * nf * nsign; it += blockDim.x) {
+        const int i = it % B, rest = it / B, s = rest % nsign, fl = rest / nsign;
+        double vr = 0.0, vi = 0.0;
+        if (!(i == B - 1 && (m & 1))) {
+            const int first = transpose_first(i, m);
+            const int rs = rowpos[i + 1] - rowpos[i];
+            const double* tp = tt + rowpos[i];
+            const double2* a = co + (fl * nsign + s) * L + first;
+            for (int k = 0; k < rs; ++k) {
+                const double tv = tp[k];
+                const double2 av = a[2 * k];
+                vr += av.x * tv;
+                vi += av.y * tv;
+            }
+            const double sc = (i == 0) ? scale0 : scale;
+            vr *= sc;
+            vi *= sc;
+        }
+        const int r = s ? N - m : m;
+        Cb[((size_t)(f0 + fl) * N + r) * B + i] = make_double2(vr, vi);
+    }
+}
+
+// DCT-III of every seminaive order row (inputs 0..B-1, zero above), then
+// x sin(theta) for odd orders, x (-1) on odd negative rows, x 1/sqrt(2pi).
+// REAL mirrors row N-m = conj(row m).  grid (ceil(nrows/rpb), F).
+template <typename T>
+__global__ __launch_bounds__(256) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
+                                                 const T* __restrict__ tw, const T* __restrict__ ph,
+                                                 const double* __restrict__ sinv, int N, int logN, int pow2, int B,
+                                                 int cut, int nrows, int rpb, int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* a = reinterpret_cast<T*>(smem);
+    T* b = a + rpb * N;
+    const int f = blockIdx.y, q0 = blockIdx.x * rpb;
+    const int nr = min(rpb, nrows - q0);
+    using R = typename Real<T>::type;
+    // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]), X[N] = 0, X[k >= B] = 0
+    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
+        const int rl = i / N, k = i - rl * N;
+        const int r = row_of(q0 + rl, N, cut);
+        const T* crow = Cb + ((size_t)f * N + r) * B;
+        const T xk = (k < B) ? crow[k] : mk<T>(0, 0);
+        const T xn = (k > 0 && N - k < B) ? crow[N - k] : mk<T>(0, 0);
+        const T t = mk<T>(xk.x + xn.y, xk.y - xn.x);
+        a[rl * N + k] = cmul(cconj(ph[k]), t);
+    }
+    __syncthreads();
+    T* u = block_fft<T, +1>(a, b, N, logN, pow2, nr, tw);
+    const R inv_s2pi = (R)0.39894228040143267794;  // 1/sqrt(2 pi)
+    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
+        const int rl = i / N, j = i - rl * N;
+        const int r = row_of(q0 + rl, N, cut);
+        const int am = r < B ? r : N - r;
+        const int n = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+        R sc = inv_s2pi;
+        if (am & 1) sc *= (R)sinv[j];
+        if (r > B && (r & 1)) sc = -sc;
+        const T y = cscale(u[rl * N + n], sc);
+        X[((size_t)f * N + r) * N + j] = y;
+        if (fmt == 1 && r > 0) X[((size_t)f * N + (N - r)) * N + j] = cconj(y);
+    }
+}
+
+// Dense synthesis of orders m >= cut (InvDLTNaive), with the same row
+// signs / scaling / REAL mirror as k_dct_inv.  grid (B-cut, ceil(F/fc)).
+__global__ __launch_bounds__(256) void k_naive_inv(const double* __restrict__ cre, const double* __restrict__ cim,
+                                                   const double* __restrict__ P, const long* __restrict__ toff,
+                                                   double2* __restrict__ X, int N, int B, int cut, int F, int fc,
+                                                   int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* co = reinterpret_cast<double2*>(smem);
+    const int m = cut + blockIdx.x, f0 = blockIdx.y * fc;
+    const int nf = min(fc, F - f0);
+    const int nsign = (m > 0 && fmt == 0) ? 2 : 1;
+    const int L = B - m;
+    const size_t B2 = (size_t)B * B;
+    for (int i = threadIdx.x; i < nf * nsign * L; i += blockDim.x) {
+        const int li = i % L, rest = i / L, s = rest % nsign, fl = rest / nsign;
+        const int p = index_of_coeff(s ? -m : m, m + li, B);
+        co[i] = make_double2(cre[(f0 + fl) * B2 + p], cim[(f0 + fl) * B2 + p]);
+    }
+    __syncthreads();
+    const double* pm = P + toff[m];
+    const double inv_s2pi = 0.39894228040143267794;
+    for (int it = threadIdx.x; it < N * nf * nsign; it += blockDim.x) {
+        const int j = it % N, rest = it / N, s = rest % nsign, fl = rest / nsign;
+        const double2* a = co + (fl * nsign + s) * L;
+        double yr = 0.0, yi = 0.0;
+        for (int li = 0; li < L; ++li) {
+            const double pv = pm[(size_t)li * N + j];
+            yr += a[li].x * pv;
+            yi += a[li].y * pv;
+        }
+        const int r = s ? N - m : m;
+        double sc = inv_s2pi;
+        if (r > B && (r & 1)) sc = -sc;
+        const double2 y = make_double2(yr * sc, yi * sc);
+        const int f = f0 + fl;
+        X[((size_t)f * N + r) * N + j] = y;
+        if (fmt == 1 && r > 0) X[((size_t)f * N + (N - r)) * N + j] = make_double2(y.x, -y.y);
+    }
+}
+
+// Inverse ring FFT (+i, unnormalised) per ring; row B (Nyquist) is zero.
+// grid (ceil(N/rpb), F).
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
+                                                 double* __restrict__ im, int N, int logN, int pow2, int B, int rpb,
+                                                 const T* __restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* a = reinterpret_cast<T*>(smem);
+    T* b = a + rpb * N;
+    const int f = blockIdx.y, j0 = blockIdx.x * rpb;
+    const int nr = min(rpb, N - j0);
+    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
+        const int r = i / nr, jl = i - r * nr;
+        a[jl * N + r] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + jl];
+    }
+    __syncthreads();
+    T* y = block_fft<T, +1>(a, b, N, logN, pow2, nr, tw);
+    const size_t base = ((size_t)f * N + j0) * N;
+    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
+        re[base + i] = (double)y[i].x;
+        im[base + i] = (double)y[i].y;
+    }
+}
+
+// m = 0 bin of each ring as a direct sum (FST_semi_memo.c:386-398), into
+// row 0 of X.  One wavefront per ring; grid (ceil(N/4), F), block 256.
+__global__ __launch_bounds__(256) void k_ring_sum(const double* __restrict__ re, const double* __restrict__ im,
+                                                  double2* __restrict__ X, int N) {
+    const int f = blockIdx.y;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= N) return;
+    const size_t base = ((size_t)f * N + j) * N;
+    double sr = 0.0, si = 0.0;
+    for (int k = lane; k < N; k += 64) {
+        sr += re[base + k];
+        si += im[base + k];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o);
+        si += __shfl_xor(si, o);
+    }
+    if (lane == 0) X[(size_t)f * N * N + j] = make_double2(sr, si);
+}
+
+// TransMult (util.c:68-103): filter degree l times signal coefficient, with
+// the reference's imaginary part x*v - y*u, times sqrt(4 pi/(2l+1)).
+__global__ void k_trans_mult(const double* dr, const double* di,
+                             const double* __restrict__ hr, const double* __restrict__ hi, long fstride,
+                             double* rr, double* ri, const int* __restrict__ deg, int B,
+                             int F) {
+    const size_t B2 = (size_t)B * B;
+    const size_t total = B2 * F;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t f = i / B2, p = i - f * B2;
+        const int l = deg[p];
+        const double x = hr[f * fstride + l], y = hi[f * fstride + l];
+        const double u = dr[i], v = di[i];
+        const double s = sqrt(4.0 * M_PI / (2.0 * l + 1.0));
+        rr[i] = (x * u - y * v) * s;
+        ri[i] = (x * v - y * u) * s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tables and weights (FP64)
+// ---------------------------------------------------------------------------
+
+// L2-normalised seed P_m^m(cos th) (pmm.c:21-34) and recurrence coefficients
+// (l2_norms.c:16-38), in the reference's operation order.
+__device__ double pmm_coeff(int m) {
+    double c = sqrt(m + 0.5);
+    for (int i = 0; i < m; ++i) c *= sqrt((m - (i / 2.0)) / ((double)m - i));
+    if (m) c *= pow(2.0, -m / 2.0);
+    if (m & 1) c = -c;
+    return c;
+}
+__device__ __forceinline__ double l2_an(int m, int l) {
+    return sqrt(((2.0 * l + 3.0) / (2.0 * l + 1.0)) * ((l - m + 1.0) / (l + m + 1.0))) *
+           ((2.0 * l + 1.0) / (l - m + 1.0));
+}
+__device__ __forceinline__ double l2_cn(int m, int l) {
+    if (l == 0) return 0.0;
+    return -1.0 *
+           sqrt(((2.0 * l + 3.0) / (2.0 * l - 1.0)) * ((l - m + 1.0) / (l + m + 1.0)) *
+                (((double)l - m) / ((double)l + m))) *
+           ((l + m) / (l - m + 1.0));
+}
+
+// cosT[k*B + i] = cos(pi (2i+1) k / (2B)), argument reduced mod 2 pi exactly.
+__global__ void k_cos_table(double* __restrict__ cosT, int B) {
+    const int total = B * B;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+        const int k = idx / B, i = idx - k * B;
+        const long e = ((long)(2 * i + 1) * k) % (4L * B);
+        cosT[idx] = cos(M_PI * (double)e / (2.0 * B));
+    }
+}
+
+// GenerateCosPmlTable (cospml.c:161-242) for orders m0 + blockIdx.x.  One
+// thread per Chebyshev node runs the recurrence; each degree's samples go
+// through LDS and the parity-selected DCT-II outputs are formed as direct
+// sums against cosT.  out + (toff ? toff[m] : 0) receives order m.
+__global__ void k_gen_cospml(double* __restrict__ out, const long* __restrict__ toff, const double* __restrict__ cosT,
+                             int B, int m0) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* P = reinterpret_cast<double*>(smem);
+    const int m = m0 + blockIdx.x;
+    double* tab = out + (toff ? toff[m] : 0);
+    const double cm = (m == 0) ? 0.0 : pmm_coeff(m);
+    const double fudge = 1.0 / sqrt((double)B);
+    for (int i = threadIdx.x; i < B; i += blockDim.x) {
+        const double th = (2.0 * i + 1.0) * M_PI / (2.0 * B);
+        double p = (m == 0) ? 0.70710678118654752440 : cm * pow(sin(th), (double)m);
+        if (m & 1) p /= sin(th);
+        P[i] = p;
+    }
+    // each thread keeps the recurrence state of its nodes in registers
+    // (B <= 1024 = 4 nodes per thread with 256 threads)
+    double cur[4], prev[4], xs[4];
+    for (int u = 0; u < 4; ++u) {
+        const int i = threadIdx.x + u * blockDim.x;
+        cur[u] = 0.0;
+        prev[u] = 0.0;
+        xs[u] = 0.0;
+        if (i < B) xs[u] = cos((2.0 * i + 1.0) * M_PI / (2.0 * B));
+    }
+    __syncthreads();
+    for (int u = 0; u < 4; ++u) {
+        const int i = threadIdx.x + u * blockDim.x;
+        if (i < B) cur[u] = P[i];
+    }
+    for (int l = m; l < B; ++l) {
+        const int tau = (l - m) & 1, rs = row_size(m, l);
+        double* dst = tab + table_offset(m, l);
+        for (int t = threadIdx.x; t < rs; t += blockDim.x) {
+            const int k = 2 * t + tau;
+            const double* ck = cosT + (size_t)k * B;
+            double s = 0.0;
+            for (int i = 0; i < B; ++i) s += P[i] * ck[i];
+            double c = 2.0 * s;
+            if (k == 0) c *= 0.70710678118654752440;
+            dst[t] = c * fudge;
+        }
+        __syncthreads();
+        if (l + 1 < B) {
+            const double an = l2_an(m, l), cn = l2_cn(m, l);
+            for (int u = 0; u < 4; ++u) {
+                const int i = threadIdx.x + u * blockDim.x;
+                if (i < B) {
+                    const double nx = an * (cur[u] * xs[u]) + cn * prev[u];
+                    prev[u] = cur[u];
+                    cur[u] = nx;
+                    P[i] = nx;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// TransposeCosPmlTable (cospml.c:301-362), restated by what it holds: row i
+// lists cosine coefficient i of degrees l0(i), l0(i)+2, ... (the degrees
+// InvDLTSemi pairs with it), Transpose_RowSize(i) of them.
+__global__ void k_transpose(const double* __restrict__ in, double* __restrict__ outp,
+                            const long* __restrict__ toff, int B, int m0) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int* rowpos = reinterpret_cast<int*>(smem);
+    const int m = m0 + blockIdx.x;
+    const long o = toff ? toff[m] : 0;
+    const double* t = in + o;
+    double* tt = outp + o;
+    if (m == B - 1) {  // one degree: the two layouts coincide
+        const int n = table_size(m, B);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) tt[i] = t[i];
+        return;
+    }
+    for (int i = threadIdx.x; i < B; i += blockDim.x)
+        rowpos[i + 1] = (i == B - 1 && (m & 1)) ? 0 : transpose_row_size(i, m, B);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rowpos[0] = 0;
+        for (int i = 1; i <= B; ++i) rowpos[i] += rowpos[i - 1];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < B; i += blockDim.x) {
+        const int rs = rowpos[i + 1] - rowpos[i];
+        const int l0 = m + transpose_first(i, m);
+        for (int k = 0; k < rs; ++k) tt[rowpos[i] + k] = t[table_offset(m, l0 + 2 * k) + i / 2];
+    }
+}
+
+// GeneratePmlTable (pml.c:41-79): P_l^m at the 2B grid latitudes, orders
+// m0 + blockIdx.x; each thread runs the recurrence at its own nodes.
+__global__ void k_gen_pml(double* __restrict__ out, const long* __restrict__ toff, int B, int m0) {
+    const int m = m0 + blockIdx.x;
+    const int N = 2 * B;
+    double* tab = out + (toff ? toff[m] : 0);
+    const double cm = (m == 0) ? 0.0 : pmm_coeff(m);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double th = (2.0 * j + 1.0) * M_PI / (2.0 * N);
+        const double x = cos((2.0 * j + 1.0) * M_PI / (2.0 * N));
+        double cur = (m == 0) ? 0.70710678118654752440 : cm * pow(sin(th), (double)m);
+        double prev = 0.0;
+        tab[j] = cur;
+        for (int l = m; l + 1 < B; ++l) {
+            const double nx = l2_an(m, l) * (cur * x) + l2_cn(m, l) * prev;
+            prev = cur;
+            cur = nx;
+            tab[(size_t)(l - m + 1) * N + j] = nx;
+        }
+    }
+}
+
+// GenerateWeightsForDLT (weights.c:32-46).
+__global__ void k_weights(double* __restrict__ w, int B) {
+    const int N = 2 * B;
+    const double c = M_PI / (4.0 * B);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double k = 2.0 * i + 1.0;
+        double s = 0.0;
+        for (int j = 0; j < B; ++j) s += 1.0 / (2.0 * j + 1.0) * sin(k * (2.0 * j + 1.0) * c);
+        s *= 2.0 * sin(k * c) / B;
+        w[i] = s;
+        w[i + N] = s * sin(k * c);
+    }
+}
+
+// Prescaled weights: w_semi = w * sqrt(2pi)/N / sqrt(2N) (2N entries),
+// w_naive = w[0:N] * sqrt(2pi)/N.
+__global__ void k_prescale_weights(const double* __restrict__ w, double* __restrict__ ws, double* __restrict__ wn,
+                                   int N) {
+    const double nc = sqrt(2.0 * M_PI) / N;
+    const double dc = 1.0 / sqrt(2.0 * N);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * N; i += gridDim.x * blockDim.x) {
+        ws[i] = w[i] * nc * dc;
+        if (i < N) wn[i] = w[i] * nc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static int ilog2_exact(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return ((1 << l) == n) ? l : -1;
+}
+
+static int rings_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
+
+void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s) {
+    c.bw = bw;
+    c.N = 2 * bw;
+    const int N = c.N;
+    const int lg = ilog2_exact(N);
+    c.pow2 = lg >= 0;
+    c.logN = lg < 0 ? 0 : lg;
+    double* d_w = nullptr;
+    S2_HIP(hipMalloc(&d_w, sizeof(double) * 4 * bw));
+    if (h_weights)
+        S2_HIP(hipMemcpyAsync(d_w, h_weights, sizeof(double) * 4 * bw, hipMemcpyHostToDevice, s));
+    else
+        k_weights<<<(N + 255) / 256, 256, 0, s>>>(d_w, bw);
+    S2_HIP(hipMalloc(&c.w_semi, sizeof(double) * 2 * N));
+    S2_HIP(hipMalloc(&c.w_naive, sizeof(double) * N));
+    k_prescale_weights<<<(2 * N + 255) / 256, 256, 0, s>>>(d_w, c.w_semi, c.w_naive, N);
+    S2_HIP(hipGetLastError());
+    std::vector<double2> tw(N), ph(N);
+    std::vector<float2> tw32(N), ph32(N);
+    std::vector<double> sv(N);
+    for (int t = 0; t < N; ++t) {
+        const double a = 2.0 * M_PI * (double)t / N;
+        tw[t] = make_double2(cos(a), -sin(a));
+        const double b = M_PI * (double)t / (2.0 * N);
+        ph[t] = make_double2(cos(b), -sin(b));
+        tw32[t] = make_float2((float)tw[t].x, (float)tw[t].y);
+        ph32[t] = make_float2((float)ph[t].x, (float)ph[t].y);
+        sv[t] = sin((2.0 * t + 1.0) * M_PI / (2.0 * N));
+    }
+    std::vector<int> deg((size_t)bw * bw);
+    for (int m = -(bw - 1); m < bw; ++m)
+        for (int l = m < 0 ? -m : m; l < bw; ++l) deg[index_of_coeff(m, l, bw)] = l;
+    S2_HIP(hipMalloc(&c.tw, sizeof(double2) * N));
+    S2_HIP(hipMalloc(&c.ph, sizeof(double2) * N));
+    S2_HIP(hipMalloc(&c.tw32, sizeof(float2) * N));
+    S2_HIP(hipMalloc(&c.ph32, sizeof(float2) * N));
+    S2_HIP(hipMalloc(&c.sinv, sizeof(double) * N));
+    S2_HIP(hipMalloc(&c.deg, sizeof(int) * deg.size()));
+    S2_HIP(hipMemcpyAsync(c.tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, s));
+    S2_HIP(hipMemcpyAsync(c.ph, ph.data(), sizeof(double2) * N, hipMemcpyHostToDevice, s));
+    S2_HIP(hipMemcpyAsync(c.tw32, tw32.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s));
+    S2_HIP(hipMemcpyAsync(c.ph32, ph32.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s));
+    S2_HIP(hipMemcpyAsync(c.sinv, sv.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    S2_HIP(hipMemcpyAsync(c.deg, deg.data(), sizeof(int) * deg.size(), hipMemcpyHostToDevice, s));
+    S2_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+    S2_HIP(hipFree(d_w));
+}
+
+void consts_destroy(DevConsts& c) {
+    for (void* p : {(void*)c.w_semi, (void*)c.w_naive, (void*)c.tw, (void*)c.ph, (void*)c.sinv, (void*)c.deg,
+                    (void*)c.tw32, (void*)c.ph32})
+        if (p) (void)hipFree(p);
+    c = DevConsts{};
+}
+
+static void check_bw(int bw) {
+    if (bw < 2 || bw > 1024) throw ArgError{"bandwidth must be in [2, 1024]"};
+}
+
+void gen_weights(int bw, double* d_w, hipStream_t s) {
+    check_bw(bw);
+    k_weights<<<(2 * bw + 255) / 256, 256, 0, s>>>(d_w, bw);
+    S2_HIP(hipGetLastError());
+}
+
+static double* make_cos_table(int bw, hipStream_t s) {
+    double* cosT = nullptr;
+    S2_HIP(hipMalloc(&cosT, sizeof(double) * bw * bw));
+    k_cos_table<<<std::min(4096, (bw * bw + 255) / 256), 256, 0, s>>>(cosT, bw);
+    S2_HIP(hipGetLastError());
+    return cosT;
+}
+
+void gen_cospml_one(int bw, int m, double* d_out, hipStream_t s) {
+    check_bw(bw);
+    double* cosT = make_cos_table(bw, s);
+    k_gen_cospml<<<1, 256, sizeof(double) * bw, s>>>(d_out, nullptr, cosT, bw, m);
+    S2_HIP(hipGetLastError());
+    S2_HIP(hipStreamSynchronize(s));
+    S2_HIP(hipFree(cosT));
+}
+
+void transpose_cospml_one(int bw, int m, const double* d_in, double* d_out, hipStream_t s) {
+    check_bw(bw);
+    S2_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * table_size(m, bw), s));
+    k_transpose<<<1, 256, sizeof(int) * (bw + 1), s>>>(d_in, d_out, nullptr, bw, m);
+    S2_HIP(hipGetLastError());
+}
+
+void gen_pml_one(int bw, int m, double* d_out, hipStream_t s) {
+    check_bw(bw);
+    k_gen_pml<<<1, 256, 0, s>>>(d_out, nullptr, bw, m);
+    S2_HIP(hipGetLastError());
+}
+
+void tables_generate(DevTables& t, int bw, int cutoff, bool want_fwd, bool want_inv, hipStream_t s) {
+    check_bw(bw);
+    if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+    t.bw = bw;
+    t.cutoff = cutoff;
+    t.h_off.assign(bw, 0);
+    long pos = 0;
+    for (int m = 0; m < bw; ++m) {
+        t.h_off[m] = pos;
+        pos += (m < cutoff) ? table_size(m, bw) : 2L * bw * (bw - m);
+    }
+    t.total = pos;
+    S2_HIP(hipMalloc(&t.off, sizeof(long) * bw));
+    S2_HIP(hipMemcpyAsync(t.off, t.h_off.data(), sizeof(long) * bw, hipMemcpyHostToDevice, s));
+    double* fwd = nullptr;
+    S2_HIP(hipMalloc(&fwd, sizeof(double) * pos));
+    if (cutoff > 0) {
+        double* cosT = make_cos_table(bw, s);
+        k_gen_cospml<<<cutoff, 256, sizeof(double) * bw, s>>>(fwd, t.off, cosT, bw, 0);
+        S2_HIP(hipGetLastError());
+        S2_HIP(hipStreamSynchronize(s));
+        S2_HIP(hipFree(cosT));
+    }
+    if (cutoff < bw) {
+        k_gen_pml<<<bw - cutoff, 256, 0, s>>>(fwd, t.off, bw, cutoff);
+        S2_HIP(hipGetLastError());
+    }
+    if (want_inv) {
+        S2_HIP(hipMalloc(&t.inv, sizeof(double) * pos));
+        // for odd bw the transposed rows do not fill an order's span; keep the tail zero
+        S2_HIP(hipMemsetAsync(t.inv, 0, sizeof(double) * pos, s));
+        if (cutoff > 0) {
+            k_transpose<<<cutoff, 256, sizeof(int) * (bw + 1), s>>>(fwd, t.inv, t.off, bw, 0);
+            S2_HIP(hipGetLastError());
+        }
+        if (cutoff < bw) {
+            const long o = t.h_off[cutoff];
+            S2_HIP(hipMemcpyAsync(t.inv + o, fwd + o, sizeof(double) * (pos - o), hipMemcpyDeviceToDevice, s));
+        }
+    }
+    if (want_fwd) {
+        t.fwd = fwd;
+    } else {
+        S2_HIP(hipStreamSynchronize(s));
+        S2_HIP(hipFree(fwd));
+    }
+    t.owned = true;
+}
+
+void tables_destroy(DevTables& t) {
+    if (t.owned) {
+        if (t.fwd) (void)hipFree(t.fwd);
+        if (t.inv) (void)hipFree(t.inv);
+    }
+    if (t.off) (void)hipFree(t.off);
+    t = DevTables{};
+}
+
+static void check_launch_shape(const DevConsts& c) {
+    if (c.N > 2048) throw ArgError{"bandwidth above 1024 is not supported"};
+}
+
+void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* cre,
+                  double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s) {
+    check_launch_shape(c);
+    const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
+    const int rpb = rings_per_block(N);
+    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    k_phi_fwd<double2><<<dim3((N + rpb - 1) / rpb, F), 256, lds_fft, s>>>(re, im, X, N, c.logN, c.pow2, rpb, c.tw);
+    S2_HIP(hipGetLastError());
+    if (cut > 0) {
+        const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+        k_dct_fwd<double2><<<dim3((nrows + rpb - 1) / rpb, F), 256, lds_fft, s>>>(
+            X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+        S2_HIP(hipGetLastError());
+        const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
+        k_leg_fwd<<<dim3(cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
+            Cb, t.fwd, t.off, cre, cim, N, B, F, fc, f, 0);
+        S2_HIP(hipGetLastError());
+    }
+    if (cut < B) {
+        const int fc = std::max(1, 65536 / (2 * N * (int)sizeof(double2)));
+        k_naive_fwd<<<dim3(B - cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * N * std::min(fc, F), s>>>(
+            X, t.fwd, t.off, c.w_naive, cre, cim, N, B, cut, F, fc, f);
+        S2_HIP(hipGetLastError());
+    }
+}
+
+void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, const double* cim, double* re,
+                  double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s) {
+    check_launch_shape(c);
+    const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
+    const int rpb = rings_per_block(N);
+    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    if (cut > 0) {
+        const int head = ((B + 2) * (int)sizeof(int) + 15) / 16 * 16;  // row offsets
+        const int fc = std::max(1, (65536 - head) / (2 * B * (int)sizeof(double2)));
+        const size_t lds = head + sizeof(double2) * 2 * B * std::min(fc, F);
+        k_leg_inv<<<dim3(cut, (F + fc - 1) / fc), 256, lds, s>>>(cre, cim, t.inv, t.off, Cb, N, B, F, fc, f);
+        S2_HIP(hipGetLastError());
+        const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+        k_dct_inv<double2><<<dim3((nrows + rpb - 1) / rpb, F), 256, lds_fft, s>>>(
+            Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+        S2_HIP(hipGetLastError());
+    }
+    if (cut < B) {
+        const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
+        k_naive_inv<<<dim3(B - cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
+            cre, cim, t.inv, t.off, X, N, B, cut, F, fc, f);
+        S2_HIP(hipGetLastError());
+    }
+    k_phi_inv<double2><<<dim3((N + rpb - 1) / rpb, F), 256, lds_fft, s>>>(X, re, im, N, c.logN, c.pow2, B, rpb,
+                                                                          c.tw);
+    S2_HIP(hipGetLastError());
+}
+
+void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* zr, double* zi,
+                int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s) {
+    check_launch_shape(c);
+    if (t.cutoff < 1) throw ArgError{"the zonal transform needs the order-0 cosine table"};
+    const int N = c.N, B = c.bw;
+    k_ring_sum<<<dim3((N + 3) / 4, F), 256, 0, s>>>(re, im, X, N);
+    S2_HIP(hipGetLastError());
+    const int rpb = rings_per_block(N);
+    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    k_dct_fwd<double2><<<dim3(1, F), 256, lds_fft, s>>>(X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B,
+                                                        t.cutoff, 1, 1);
+    S2_HIP(hipGetLastError());
+    const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
+    k_leg_fwd<<<dim3(1, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
+        Cb, t.fwd, t.off, zr, zi, N, B, F, fc, (int)fmt, 1);
+    S2_HIP(hipGetLastError());
+}
+
+void trans_mult(const DevConsts& c, const double* dr, const double* di, const double* hr, const double* hi,
+                long fstride, double* rr, double* ri, int F, hipStream_t s) {
+    const size_t total = (size_t)c.bw * c.bw * F;
+    const int blocks = (int)std::min<size_t>(8192, (total + 255) / 256);
+    k_trans_mult<<<blocks, 256, 0, s>>>(dr, di, hr, hi, fstride, rr, ri, c.deg, c.bw, F);
+    S2_HIP(hipGetLastError());
+}
+
+}  // namespace s2

@@ -1,0 +1,107 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol,
+and its host-side integer layout functions and argument checks behave like
+the reference's (no GPU work is started here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADERS = [os.path.join(ROOT, "include", "s2kit", h) for h in
+           ("FST_semi_memo.h", "cospml.h", "util.h", "weights.h", "pml.h")] + \
+          [os.path.join(ROOT, "include", "s2kit_amd.h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            if not m.group(0).lstrip().startswith(("typedef", "#", "return")):
+                names.add(m.group(1))
+    return names
+
+
+@pytest.fixture(scope="module")
+def S():
+    import s2kit_amd
+    s2kit_amd.lib()
+    return s2kit_amd
+
+
+def test_every_declared_symbol_is_exported(S):
+    names = declared_functions()
+    assert {"FSTSemiMemo", "InvFSTSemiMemo", "FZTSemiMemo", "ConvOn2SphereSemiMemo", "TableSize",
+            "Transpose_RowSize", "s2amd_forward"} <= names
+    L = S.lib()
+    for n in sorted(names):
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
+        assert n in S.SIGNATURES, f"{n} has no binding in s2kit_amd.SIGNATURES"
+    assert set(S.SIGNATURES) == names
+
+
+def test_no_fftw_or_oracle_symbols(S):
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", S.LIBPATH], capture_output=True, text=True).stdout
+    assert "fftw_" not in out and " so_" not in out
+    und = subprocess.run(["nm", "-D", "--undefined-only", S.LIBPATH], capture_output=True, text=True).stdout
+    assert "fftw" not in und and "s2oracle" not in und
+
+
+@pytest.mark.parametrize("bw", [8, 9, 17, 23, 64, 256, 512])
+def test_layout_functions_match_oracle(S, O, bw):
+    L = S.lib()
+    for m in range(bw):
+        assert L.TableSize(m, bw) == O.table_size(m, bw)
+        for l in range(m, bw, max(1, bw // 32)):
+            assert L.TableOffset(m, l) == O.table_offset(m, l)
+            assert L.RowSize(m, l) == O.row_size(m, l)
+        for i in range(bw):
+            assert L.Transpose_RowSize(i, m, bw) == O.transpose_row_size(i, m, bw)
+    for m in range(0, bw + 1, max(1, bw // 16)):
+        assert L.Reduced_SpharmonicTableSize(bw, m) == O.reduced_sph_table_size(bw, m)
+        assert L.Reduced_Naive_TableSize(bw, m) == O.reduced_naive_table_size(bw, m)
+    assert L.Spharmonic_TableSize(bw) == O.sph_table_size(bw)
+    for m in range(-(bw - 1), bw, max(1, bw // 16)):
+        for l in range(abs(m), bw):
+            assert L.IndexOfHarmonicCoeff(m, l, bw) == O.index_of_coeff(m, l, bw)
+
+
+def test_dropin_rejects_bad_arguments_before_touching_the_device(S):
+    L = S.lib()
+    S.set_abort_on_error(False)
+    try:
+        z = np.zeros(16)
+        p = z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        L.FSTSemiMemo(p, p, p, p, 0, None, None, 0, 0, None, None, p)
+        assert "bw must be in [2, 1024]" in S.last_error()
+        L.InvFSTSemiMemo(p, p, p, p, 8, None, None, 0, 8, None, None)
+        assert "transpose_seminaive_naive_table is NULL" in S.last_error()
+        L.FSTSemiMemo(p, p, p, p, 8, None, None, 3, 8, None, None, p)
+        assert "data_format" in S.last_error()
+        L.FSTSemiMemo(p, p, p, p, 8, None, None, 0, 9, None, None, p)
+        assert "cutoff" in S.last_error()
+        L.ConvOn2SphereSemiMemo(p, p, p, None, p, p, 8, None)
+        assert "NULL" in S.last_error()
+        L.GenerateCosPmlTable(8, 8, p, None)
+        assert "m must be in [0, bw)" in S.last_error()
+    finally:
+        S.set_abort_on_error(True)
+
+
+def test_plan_rejects_bad_arguments(S):
+    L = S.lib()
+    h = ctypes.c_void_p()
+    assert L.s2amd_plan_create(ctypes.byref(h), 1, 1, 4, 0) == -1
+    assert "bw" in S.last_error()
+    assert L.s2amd_plan_create(ctypes.byref(h), 16, 17, 4, 0) == -1
+    assert L.s2amd_plan_create(ctypes.byref(h), 16, 16, 0, 0) == -1
+    assert L.s2amd_plan_create(ctypes.byref(h), 16, 16, 4, 9) == -1
+    assert "precision" in S.last_error()
+    assert L.s2amd_forward(None, None, None, None, None, 1, 0, None) == -1
+    assert "plan is NULL" in S.last_error()
+    assert L.s2amd_plan_destroy(None) == 0

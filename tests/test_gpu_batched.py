@@ -1,0 +1,153 @@
+"""GPU parity of the batched device API (s2kit_amd.h) against the CPU
+oracle, plus size-independent properties at the full B=256 shape.
+
+FP64 plans: 1e-11 relative (normwise) against the oracle, per field.
+FP32 plans: 1e-6 relative against the oracle (the mixed-precision bar).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import s2kit_amd
+    s2kit_amd.lib()
+    return s2kit_amd, torch, torch.device("cuda:0")
+
+
+def crel(ar, ai, br, bi):
+    a = np.asarray(ar) + 1j * np.asarray(ai)
+    b = np.asarray(br) + 1j * np.asarray(bi)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def to_dev(torch, dev, arrs):
+    return torch.tensor(np.stack(arrs), dtype=torch.float64, device=dev)
+
+
+@pytest.mark.parametrize("bw,cutoff,fmt,prec", [(8, 8, 0, 0), (17, 17, 0, 0), (64, 64, 0, 0), (64, 16, 0, 0),
+                                                (64, 64, 1, 0), (128, 128, 0, 0)])
+def test_plan_matches_oracle(env, O, bw, cutoff, fmt, prec):
+    S, torch, dev = env
+    print(f"plan bw={bw} cutoff={cutoff} fmt={fmt} prec={prec}", flush=True)
+    tol = 1e-11 if prec == 0 else 1e-6
+    F = 3
+    plan = S.Plan(bw, F, precision=prec, cutoff=cutoff)
+    kind = 2 if fmt == 1 else 0
+    coeffs = [O.make_coeffs(100 + f, bw, kind) for f in range(F)]
+    cre = to_dev(torch, dev, [c[0] for c in coeffs])
+    cim = to_dev(torch, dev, [c[1] for c in coeffs])
+    n = 2 * bw
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    plan.inverse(cre, cim, gre, gim, data_format=fmt)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+    plan.forward(gre, gim, c2r, c2i, data_format=fmt)
+    torch.cuda.synchronize()
+    gre_h, gim_h = gre.cpu().numpy(), gim.cpu().numpy()
+    c2r_h, c2i_h = c2r.cpu().numpy(), c2i.cpu().numpy()
+    for f in range(F):
+        ord_, oid = O.inv_fst(coeffs[f][0], coeffs[f][1], bw, data_format=fmt, cutoff=cutoff)
+        assert crel(gre_h[f].ravel(), gim_h[f].ravel(), ord_, oid) < tol
+        orc, oic = O.fst(gre_h[f].ravel(), gim_h[f].ravel(), bw, data_format=fmt, cutoff=cutoff)
+        assert crel(c2r_h[f], c2i_h[f], orc, oic) < tol
+    plan.close()
+
+
+@pytest.mark.parametrize("bw,cutoff", [(16, 16), (64, 64), (64, 16), (256, 256)])
+def test_plan_tables_match_oracle(env, O, bw, cutoff):
+    import ctypes
+    S, torch, dev = env
+    plan = S.Plan(bw, 1, cutoff=cutoff)
+    f, i, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_long()
+    assert S.lib().s2amd_plan_tables(plan.h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(n)) == 0
+    ot = O.Tables(bw, cutoff)
+    assert n.value == ot.fwd.size
+    hf, hi = np.empty(n.value), np.empty(n.value)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    for host, src in ((hf, f), (hi, i)):
+        assert hip.hipMemcpy(host.ctypes.data, src.value, 8 * n.value, 2) == 0  # device to host
+    scale = np.abs(ot.fwd).max()
+    assert np.abs(hf - ot.fwd).max() / scale < 1e-13
+    assert np.abs(hi - ot.inv).max() / scale < 1e-13
+    plan.close()
+
+
+def test_plan_conv_and_zonal(env, O):
+    S, torch, dev = env
+    bw, F = 64, 2
+    plan = S.Plan(bw, F)
+    rng = np.random.default_rng(31)
+    n = 2 * bw
+    sig = [O.inv_fst(*O.make_coeffs(40 + f, bw, 0), bw) for f in range(F)]
+    filt = []
+    for f in range(F):
+        sigma = 5 + rng.uniform() * (bw / 2 - 5)
+        l = np.arange(bw)
+        hc = np.zeros(bw * bw)
+        hc[:bw] = np.exp(-l * (l + 1) / (2 * sigma ** 2))
+        filt.append(O.inv_fst(hc, np.zeros(bw * bw), bw))
+    sre = to_dev(torch, dev, [s[0].reshape(n, n) for s in sig])
+    sim = to_dev(torch, dev, [s[1].reshape(n, n) for s in sig])
+    hre = to_dev(torch, dev, [h[0].reshape(n, n) for h in filt])
+    him = to_dev(torch, dev, [h[1].reshape(n, n) for h in filt])
+    ore, oim = torch.empty_like(sre), torch.empty_like(sim)
+    plan.conv(sre, sim, hre, him, ore, oim)
+    zr = torch.empty((F, bw), dtype=torch.float64, device=dev)
+    zi = torch.empty_like(zr)
+    plan.zonal(hre, him, zr, zi, data_format=0)
+    torch.cuda.synchronize()
+    for f in range(F):
+        orr, oir = O.conv(sig[f][0], sig[f][1], filt[f][0], filt[f][1], bw)
+        assert crel(ore[f].cpu().numpy().ravel(), oim[f].cpu().numpy().ravel(), orr, oir) < 1e-11
+        ozr, ozi = O.fzt(filt[f][0], filt[f][1], bw, data_format=0)
+        assert crel(zr[f].cpu().numpy(), zi[f].cpu().numpy(), ozr, ozi) < 1e-11
+    plan.close()
+
+
+def test_plan_argument_errors(env):
+    S, torch, dev = env
+    plan = S.Plan(16, 2)
+    x = torch.zeros((3, 32, 32), dtype=torch.float64, device=dev)
+    c = torch.zeros((3, 256), dtype=torch.float64, device=dev)
+    with pytest.raises(S.S2Error, match="nfields"):
+        plan.forward(x, x, c, c)
+    plan.forward(x, x, c, c, nfields=0)  # no-op
+    plan.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-11)])
+def test_full_size_round_trip_bw256(env, O, prec, tol):
+    # B=256, 8 power-law fields: inverse then forward is the identity on
+    # band-limited coefficients; and the transform is linear.
+    S, torch, dev = env
+    bw, F = 256, 8
+    plan = S.Plan(bw, F, precision=prec)
+    coeffs = [O.make_coeffs(500 + f, bw, 1) for f in range(F)]
+    cre = to_dev(torch, dev, [c[0] for c in coeffs])
+    cim = to_dev(torch, dev, [c[1] for c in coeffs])
+    n = 2 * bw
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    plan.inverse(cre, cim, gre, gim)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+    plan.forward(gre, gim, c2r, c2i)
+    # linearity: forward(2 g0 - g1) = 2 c0 - c1
+    lre = (2 * gre[0] - gre[1]).unsqueeze(0).contiguous()
+    lim = (2 * gim[0] - gim[1]).unsqueeze(0).contiguous()
+    l2r, l2i = torch.empty_like(cre[:1]), torch.empty_like(cim[:1])
+    plan.forward(lre, lim, l2r, l2i, nfields=1)
+    torch.cuda.synchronize()
+    a = torch.complex(c2r, c2i)
+    b = torch.complex(cre, cim)
+    err = float((a - b).abs().max() / b.abs().max())
+    lin = torch.complex(l2r[0], l2i[0]) - (2 * a[0] - a[1])
+    lerr = float(lin.abs().max() / b.abs().max())
+    print(f"bw=256 prec={prec}: round-trip rel err {err:.3e}, linearity {lerr:.3e}", flush=True)
+    assert err < tol and lerr < tol
+    plan.close()
