@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Throughput of the batched forward + inverse SHT on one MI355X.
+
+Workload: 1024 independent fields at bandwidth 256 (512x512 Driscoll-Healy
+grids, complex FP64 in and out), power-law coefficients (E|c|^2 = (l+1)^-2)
+synthesised on the GPU.  One step = forward transform of all fields (grids
+-> coefficients) followed by the inverse (coefficients -> grids), inputs
+already resident in HBM.  value = fwd+inv pairs per second = fields * steps /
+timed seconds.
+
+Also reported on the same JSON line:
+  errors      the timed form's results against the FP64 GPU form on the same
+              input (normwise max|d|/max|ref|), and the FP64 round trip;
+  roofline    the dominant kernel's algorithmic HBM bytes per launch over its
+              average duration (HIP events on the launch stream, recorded
+              during the timed steps), against 8 TB/s;
+  cpu_baseline  the CPU FP64 oracle (a port of the reference path) timed on
+              the host, one process per core, on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def stage_bytes(stage, bw, nfields, table_doubles, elem):
+    """Algorithmic HBM bytes of one launch of `stage` over nfields fields:
+    compulsory reads + writes of its inputs/outputs (grids and coefficients
+    FP64 complex; the order-major intermediates in `elem` bytes per complex
+    value), plus one pass over the table for the Legendre stages."""
+    n = 2 * bw
+    rows = 2 * bw - 1  # order rows used (all but the Nyquist row)
+    grid = 16 * n * n
+    coef = 16 * bw * bw
+    xrow = elem * n * rows
+    crow = elem * bw * rows
+    per_field = {
+        "phi_fwd": grid + elem * n * n,
+        "dct_fwd": xrow + crow,
+        "leg_fwd": crow + coef,
+        "leg_inv": coef + crow,
+        "dct_inv": crow + xrow,
+        "phi_inv": elem * n * rows + grid,
+    }[stage]
+    table = 8 * table_doubles if stage in ("leg_fwd", "leg_inv") else 0
+    return per_field * nfields + table
+
+
+def cpu_worker(args):
+    """One CPU process: `pairs` oracle inverse+forward transforms at bw."""
+    bw, pairs, seed = args
+    from oracle import oracle as O
+    O.tables(bw)  # precomputed and not timed, as test_s2_semi_memo.c:6-8
+    rc, ic = O.make_coeffs(seed, bw, 1)
+    t0 = time.perf_counter()
+    for _ in range(pairs):
+        rd, idd = O.inv_fst(rc, ic, bw)
+        rc2, ic2 = O.fst(rd, idd, bw)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(bw, pairs_per_proc, procs):
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")  # before any GPU initialisation in this process
+    with ctx.Pool(procs) as pool:
+        t0 = time.perf_counter()
+        times = pool.map(cpu_worker, [(bw, pairs_per_proc, 7 + i) for i in range(procs)])
+        wall = time.perf_counter() - t0
+    total = pairs_per_proc * procs
+    return {
+        "value": total / max(times),
+        "unit": "fwd+inv pairs/s",
+        "cores": procs,
+        "kind": "port",
+        "sample": f"{total} fwd+inv pairs at bw={bw} (power-law coefficients), {procs} single-threaded "
+                  f"processes x {pairs_per_proc} pairs, tables precomputed; slowest process {max(times):.2f} s, "
+                  f"pool wall {wall:.2f} s; own FFT/DCT (no FFTW on this host)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--bw", type=int, default=256)
+    ap.add_argument("--fields", type=int, default=1024)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
+    ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(fields would shard across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    bw, F = args.bw, args.fields
+    cpu = None
+    if args.cpu_pairs > 0:
+        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(bw, args.cpu_pairs, procs)
+
+    import torch
+
+    import s2kit_amd as S
+    from s2kit_amd import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    n = 2 * bw
+    prec = {"fp64": S.FP64, "fp32": S.FP32}[args.precision]
+
+    # ---- workload: band-limited grids from power-law coefficients ---------
+    c0r, c0i = synth.power_law_coeffs(bw, F, 1234, dev)
+    p64 = S.Plan(bw, F, precision=S.FP64)
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    p64.inverse(c0r, c0i, gre, gim)
+    plan = p64 if prec == S.FP64 else S.Plan(bw, F, precision=prec)
+    cre, cim = torch.empty_like(c0r), torch.empty_like(c0i)
+    ore, oim = torch.empty_like(gre), torch.empty_like(gim)
+
+    def step():
+        plan.forward(gre, gim, cre, cim)
+        plan.inverse(cre, cim, ore, oim)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stages = plan.stage_times()
+    plan.set_timing(False)
+    value = F * args.steps / dt
+
+    # ---- errors of the timed form against the FP64 form --------------------
+    ref_cr, ref_ci = torch.empty_like(c0r), torch.empty_like(c0i)
+    p64.forward(gre, gim, ref_cr, ref_ci)
+    ref_gr, ref_gi = torch.empty_like(gre), torch.empty_like(gim)
+    p64.inverse(ref_cr, ref_ci, ref_gr, ref_gi)
+    step()
+    torch.cuda.synchronize()
+
+    def rel(ar, ai, br, bi):
+        return float((torch.complex(ar, ai) - torch.complex(br, bi)).abs().max() /
+                     torch.complex(br, bi).abs().max())
+
+    errors = {
+        "coeff_vs_fp64": rel(cre, cim, ref_cr, ref_ci),
+        "grid_vs_fp64": rel(ore, oim, ref_gr, ref_gi),
+        "fp64_round_trip": rel(ref_cr, ref_ci, c0r, c0i),
+    }
+
+    # ---- batch-1 FP64 form: one field per call -----------------------------
+    plain = None
+    if args.plain_fields > 0:
+        pf = min(args.plain_fields, F)
+        p1 = S.Plan(bw, 1, precision=S.FP64)
+        for f in range(2):
+            p1.forward(gre[f:f + 1], gim[f:f + 1], cre[f:f + 1], cim[f:f + 1])
+            p1.inverse(cre[f:f + 1], cim[f:f + 1], ore[f:f + 1], oim[f:f + 1])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for f in range(pf):
+            p1.forward(gre[f:f + 1], gim[f:f + 1], cre[f:f + 1], cim[f:f + 1])
+            p1.inverse(cre[f:f + 1], cim[f:f + 1], ore[f:f + 1], oim[f:f + 1])
+        torch.cuda.synchronize()
+        plain = pf / (time.perf_counter() - t1)
+        p1.close()
+
+    # ---- roofline of the dominant kernel ----------------------------------
+    import ctypes
+    f_ptr, i_ptr, ndbl = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_long()
+    S.lib().s2amd_plan_tables(plan.h, ctypes.byref(f_ptr), ctypes.byref(i_ptr), ctypes.byref(ndbl))
+    elem = 16 if prec == S.FP64 else 8
+    dom = max(stages, key=lambda k: stages[k][0])
+    dom_ms, dom_n = stages[dom]
+    avg_s = dom_ms / dom_n / 1e3
+    achieved = stage_bytes(dom, bw, F, ndbl.value, elem) / avg_s
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": achieved / 1e9,
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK,
+        "traffic": None,
+        "avg_launch_ms": dom_ms / dom_n,
+        "stages_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
+    }
+
+    out = {
+        "metric": "fwd+inv SHT pairs/s, bw=256 x1024 batch: mixed vs FP64, rel err<=1e-6",
+        "value": value,
+        "unit": "fwd+inv pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"batched fwd+inv SHT, bw={bw}, {F} fields, {n}x{n} complex FP64 grids, "
+                               "power-law (l+1)^-2 coefficients", "bw": bw, "fields": F,
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "errors": errors,
+        "fp64_batch1_pairs_per_s": plain,
+        "speedup_vs_fp64_batch1": (value / plain) if plain else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    p64.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -62,6 +62,15 @@ int s2amd_zonal(s2amd_plan* plan, const double* re, const double* im, double* zr
 int s2amd_conv(s2amd_plan* plan, const double* sre, const double* sim, const double* hre, const double* him,
                double* ore, double* oim, int nfields, void* stream);
 
+/* Per-stage kernel timing: when enabled, every launch of the plan is
+ * bracketed by HIP events recorded on the call's stream.  stage_times waits
+ * for the recorded events, returns the summed milliseconds and launch counts
+ * per stage (arrays of S2AMD_NSTAGES), and clears them. */
+#define S2AMD_NSTAGES 10
+int s2amd_plan_timing(s2amd_plan* plan, int enable);
+int s2amd_plan_stage_times(s2amd_plan* plan, double* ms, long* counts);
+const char* s2amd_stage_name(int stage);
+
 /* Drop-in error policy: 1 (default) print and abort, 0 record only. */
 void s2amd_set_abort_on_error(int on);
 const char* s2amd_last_error(void);

@@ -21,6 +21,7 @@ LIBPATH = os.path.join(_HERE, "libs2kit_amd.so")
 
 COMPLEX, REAL = 0, 1
 FP64, FP32 = 0, 1
+NSTAGES = 10
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _dpp = ctypes.POINTER(_dp)
@@ -64,6 +65,9 @@ SIGNATURES = {
     "s2amd_inverse": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "s2amd_zonal": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "s2amd_conv": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "s2amd_plan_timing": (_i, [_vp, _i]),
+    "s2amd_plan_stage_times": (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long)]),
+    "s2amd_stage_name": (ctypes.c_char_p, [_i]),
     "s2amd_set_abort_on_error": (None, [_i]),
     "s2amd_last_error": (ctypes.c_char_p, []),
 }
@@ -253,6 +257,17 @@ class Plan:
         import torch
         s = torch.cuda.current_stream() if stream is None else stream
         return ctypes.c_void_p(s.cuda_stream)
+
+    def set_timing(self, on):
+        self._check(lib().s2amd_plan_timing(self.h, 1 if on else 0))
+
+    def stage_times(self):
+        """{stage: (total_ms, launches)} since the last call; waits for the events."""
+        n = NSTAGES
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_long * n)()
+        self._check(lib().s2amd_plan_stage_times(self.h, ms, cnt))
+        return {lib().s2amd_stage_name(i).decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i]}
 
     def _check(self, rc):
         if rc != 0:

@@ -275,10 +275,29 @@ static void run_dropin(const char* fn, F&& body) {
 // ---------------------------------------------------------------------------
 struct Plan {
     int bw, cutoff, max_fields, precision, device;
+    bool fast = false;  // power-of-two grid path with MFMA GEMMs (s2_fast.hip)
     DevConsts c;
     DevTables t;
+    GemmTables g;
     DBuf X, Cb, cr, ci, zr, zi;
+    StageTimer timer;
 };
+
+static void plan_forward(Plan* p, const double* re, const double* im, double* cre, double* cim, int F, Fmt fmt,
+                         hipStream_t s) {
+    if (p->fast)
+        forward_fast(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
+    else
+        forward_fp64(p->c, p->t, re, im, cre, cim, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
+}
+
+static void plan_inverse(Plan* p, const double* cre, const double* cim, double* re, double* im, int F, Fmt fmt,
+                         hipStream_t s) {
+    if (p->fast)
+        inverse_fast(p->c, p->g, cre, cim, re, im, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
+    else
+        inverse_fp64(p->c, p->t, cre, cim, re, im, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
+}
 
 static int plan_status(const char* fn, const std::exception_ptr& ep) {
     try {
@@ -697,7 +716,9 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
         check_common(bw, 0);
         if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
         if (max_fields < 1) throw ArgError{"max_fields must be >= 1"};
-        if (precision != S2AMD_FP64) throw ArgError{"unknown precision"};
+        if (precision != S2AMD_FP64 && precision != S2AMD_FP32) throw ArgError{"unknown precision"};
+        if (precision == S2AMD_FP32 && !fast_supported(bw, cutoff, 1))
+            throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 1024] and cutoff = bw"};
         std::unique_ptr<s2amd_plan> p(new s2amd_plan());
         p->bw = bw;
         p->cutoff = cutoff;
@@ -709,9 +730,12 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
         try {
             consts_create(p->c, bw, nullptr, s);
             tables_generate(p->t, bw, cutoff, true, true, s);
+            p->fast = fast_supported(bw, cutoff, precision);
+            if (p->fast) gemm_tables_create(p->g, p->t, precision, s);
             const size_t N = 2 * (size_t)bw, F = max_fields;
-            p->X.reserve(sizeof(double2) * N * N * F);
-            p->Cb.reserve(sizeof(double2) * N * bw * F);
+            // large enough for the fast layouts and for the FP64 zonal transform
+            p->X.reserve(std::max(sizeof(double2) * N * N, fast_x_bytes(bw, precision)) * F);
+            p->Cb.reserve(std::max(sizeof(double2) * N * bw, fast_cb_bytes(bw, precision)) * F);
             p->cr.reserve(sizeof(double) * bw * bw * F);
             p->ci.reserve(sizeof(double) * bw * bw * F);
             p->zr.reserve(sizeof(double) * bw * F);
@@ -721,6 +745,7 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
             (void)hipStreamDestroy(s);
             consts_destroy(p->c);
             tables_destroy(p->t);
+            gemm_tables_destroy(p->g);
             for (DBuf* b : {&p->X, &p->Cb, &p->cr, &p->ci, &p->zr, &p->zi}) b->release();
             throw;
         }
@@ -734,6 +759,7 @@ extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
         if (!p) return;
         consts_destroy(p->c);
         tables_destroy(p->t);
+        gemm_tables_destroy(p->g);
         for (DBuf* b : {&p->X, &p->Cb, &p->cr, &p->ci, &p->zr, &p->zi}) b->release();
         delete p;
     });
@@ -741,7 +767,9 @@ extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
 
 extern "C" size_t s2amd_plan_bytes(const s2amd_plan* p) {
     if (!p) return 0;
-    return 2 * sizeof(double) * p->t.total + p->X.n + p->Cb.n + p->cr.n + p->ci.n + p->zr.n + p->zi.n;
+    const size_t el = p->precision == S2AMD_FP64 ? sizeof(double) : sizeof(float);
+    return 2 * sizeof(double) * p->t.total + el * (p->g.nf + p->g.ni) + p->X.n + p->Cb.n + p->cr.n + p->ci.n +
+           p->zr.n + p->zi.n;
 }
 
 extern "C" int s2amd_plan_tables(const s2amd_plan* p, const double** fwd, const double** inv, long* n) {
@@ -762,8 +790,7 @@ extern "C" int s2amd_forward(s2amd_plan* p, const double* re, const double* im, 
         need(im, "im");
         need(cre, "cre");
         need(cim, "cim");
-        forward_fp64(p->c, p->t, re, im, cre, cim, nfields, (Fmt)data_format, p->X.as<double2>(),
-                     p->Cb.as<double2>(), (hipStream_t)stream);
+        plan_forward(p, re, im, cre, cim, nfields, (Fmt)data_format, (hipStream_t)stream);
     });
 }
 
@@ -776,8 +803,7 @@ extern "C" int s2amd_inverse(s2amd_plan* p, const double* cre, const double* cim
         need(im, "im");
         need(cre, "cre");
         need(cim, "cim");
-        inverse_fp64(p->c, p->t, cre, cim, re, im, nfields, (Fmt)data_format, p->X.as<double2>(),
-                     p->Cb.as<double2>(), (hipStream_t)stream);
+        plan_inverse(p, cre, cim, re, im, nfields, (Fmt)data_format, (hipStream_t)stream);
     });
 }
 
@@ -791,7 +817,7 @@ extern "C" int s2amd_zonal(s2amd_plan* p, const double* re, const double* im, do
         need(zr, "zr");
         need(zi, "zi");
         zonal_fp64(p->c, p->t, re, im, zr, zi, nfields, (Fmt)data_format, p->X.as<double2>(), p->Cb.as<double2>(),
-                   (hipStream_t)stream);
+                   (hipStream_t)stream, &p->timer);
     });
 }
 
@@ -807,10 +833,34 @@ extern "C" int s2amd_conv(s2amd_plan* p, const double* sre, const double* sim, c
         hipStream_t s = (hipStream_t)stream;
         double* cr = p->cr.as<double>();
         double* ci = p->ci.as<double>();
-        forward_fp64(p->c, p->t, sre, sim, cr, ci, nfields, Fmt::Real, p->X.as<double2>(), p->Cb.as<double2>(), s);
+        StageTimer* tm = &p->timer;
+        plan_forward(p, sre, sim, cr, ci, nfields, Fmt::Real, s);
         zonal_fp64(p->c, p->t, hre, him, p->zr.as<double>(), p->zi.as<double>(), nfields, Fmt::Real,
-                   p->X.as<double2>(), p->Cb.as<double2>(), s);
-        trans_mult(p->c, cr, ci, p->zr.as<double>(), p->zi.as<double>(), p->bw, cr, ci, nfields, s);
-        inverse_fp64(p->c, p->t, cr, ci, ore, oim, nfields, Fmt::Real, p->X.as<double2>(), p->Cb.as<double2>(), s);
+                   p->X.as<double2>(), p->Cb.as<double2>(), s, tm);
+        trans_mult(p->c, cr, ci, p->zr.as<double>(), p->zi.as<double>(), p->bw, cr, ci, nfields, s, tm);
+        plan_inverse(p, cr, ci, ore, oim, nfields, Fmt::Real, s);
     });
 }
+
+extern "C" int s2amd_plan_timing(s2amd_plan* p, int enable) {
+    return run_plan("s2amd_plan_timing", [&] {
+        if (!p) throw ArgError{"plan is NULL"};
+        p->timer.collect();
+        p->timer.reset();
+        p->timer.on = enable != 0;
+    });
+}
+
+extern "C" int s2amd_plan_stage_times(s2amd_plan* p, double* ms, long* counts) {
+    return run_plan("s2amd_plan_stage_times", [&] {
+        if (!p) throw ArgError{"plan is NULL"};
+        p->timer.collect();
+        for (int i = 0; i < S2AMD_NSTAGES; ++i) {
+            if (ms) ms[i] = p->timer.ms[i];
+            if (counts) counts[i] = p->timer.count[i];
+        }
+        p->timer.reset();
+    });
+}
+
+extern "C" const char* s2amd_stage_name(int stage) { return s2::stage_name(stage); }

@@ -57,9 +57,71 @@ struct DevConsts {
     int* deg = nullptr;          // bw*bw: degree of each coefficient slot
     float2* tw32 = nullptr;
     float2* ph32 = nullptr;
+    float* w_semi32 = nullptr;   // w_semi rounded to FP32
+    float* sinv32 = nullptr;
+};
+
+// Zero-padded, parity-split Legendre matrices for the MFMA GEMMs of the fast
+// path (s2_fast.hip).  Entry m*2+p describes order m, degrees l = m+p+2i.
+struct GemmMeta {
+    int Lp;     // number of degrees in the class
+    int Kf;     // cosine terms of its last (longest) row
+    long offF;  // offset of AF(m,p) [roundup4(Kf)][roundup16(Lp)]
+    long offI;  // offset of AI(m,p) [roundup4(Lp)][roundup16(bw/2)]
+};
+
+struct GemmTables {
+    int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32
+    void* af = nullptr;
+    void* ai = nullptr;
+    GemmMeta* meta = nullptr;
+    std::vector<GemmMeta> h_meta;
+    long nf = 0, ni = 0;
 };
 
 enum class Fmt : int { Complex = 0, Real = 1 };
+
+// Optional per-stage timing with HIP events recorded on the launch stream.
+enum Stage : int {
+    ST_PHI_FWD = 0,
+    ST_DCT_FWD,
+    ST_LEG_FWD,
+    ST_NAIVE_FWD,
+    ST_LEG_INV,
+    ST_DCT_INV,
+    ST_NAIVE_INV,
+    ST_PHI_INV,
+    ST_ZONAL,
+    ST_MULT,
+    kStages
+};
+const char* stage_name(int st);
+
+struct StageTimer {
+    bool on = false;
+    struct Rec {
+        int st;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double ms[kStages] = {};
+    long count[kStages] = {};
+    hipEvent_t take();
+    void collect();  // waits for recorded events, accumulates, recycles them
+    void reset();
+    ~StageTimer();
+};
+
+// RAII span around one launch: records a start and a stop event on `s`.
+struct Span {
+    StageTimer* t;
+    int st;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    Span(StageTimer* t_, int st_, hipStream_t s_);
+    ~Span();
+};
 
 // ---- kernels' host launchers (s2_kernels.hip) -----------------------------
 void consts_create(DevConsts& c, int bw, const double* h_weights /* 4*bw or null: compute on GPU */,
@@ -81,14 +143,25 @@ void gen_weights(int bw, double* d_w, hipStream_t s);
 //   grid  re/im [F][N][N], coefficients re/im [F][bw*bw].
 // X: scratch [F][N][N] double2, Cb: scratch [F][N][bw] double2.
 void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* cre,
-                  double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s);
+                  double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm = nullptr);
 void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, const double* cim, double* re,
-                  double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s);
+                  double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm = nullptr);
 // FZT: zonal coefficients [F][bw] of grids [F][N][N].
 void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* zr, double* zi,
-                int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s);
+                int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm = nullptr);
 // TransMult over F fields; the filter of field f is hr/hi + f*fstride.
 void trans_mult(const DevConsts& c, const double* dr, const double* di, const double* hr, const double* hi,
-                long fstride, double* rr, double* ri, int F, hipStream_t s);
+                long fstride, double* rr, double* ri, int F, hipStream_t s, StageTimer* tm = nullptr);
+
+// ---- fast path (s2_fast.hip): power-of-two grids, cutoff = bw ----------
+bool fast_supported(int bw, int cutoff, int prec);
+void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t s);
+void gemm_tables_destroy(GemmTables& g);
+size_t fast_x_bytes(int bw, int prec);   // scratch X per field
+size_t fast_cb_bytes(int bw, int prec);  // scratch Cb per field
+void forward_fast(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
+                  double* cim, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm);
+void inverse_fast(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
+                  double* im, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm);
 
 }  // namespace s2

@@ -21,51 +21,10 @@
 #include <cstdio>
 #include <vector>
 
+#include "s2_complex.h"
 #include "s2_internal.h"
 
 namespace s2 {
-
-// ---------------------------------------------------------------------------
-// complex helpers (double2 / float2)
-// ---------------------------------------------------------------------------
-template <typename T>
-struct Real;
-template <>
-struct Real<double2> {
-    using type = double;
-};
-template <>
-struct Real<float2> {
-    using type = float;
-};
-
-template <typename T>
-__device__ __forceinline__ T mk(typename Real<T>::type x, typename Real<T>::type y) {
-    T r;
-    r.x = x;
-    r.y = y;
-    return r;
-}
-template <typename T>
-__device__ __forceinline__ T cadd(T a, T b) {
-    return mk<T>(a.x + b.x, a.y + b.y);
-}
-template <typename T>
-__device__ __forceinline__ T csub(T a, T b) {
-    return mk<T>(a.x - b.x, a.y - b.y);
-}
-template <typename T>
-__device__ __forceinline__ T cmul(T a, T b) {
-    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-template <typename T>
-__device__ __forceinline__ T cconj(T a) {
-    return mk<T>(a.x, -a.y);
-}
-template <typename T>
-__device__ __forceinline__ T cscale(T a, typename Real<T>::type s) {
-    return mk<T>(a.x * s, a.y * s);
-}
 
 // ---------------------------------------------------------------------------
 // block-wide FFTs in LDS
@@ -671,9 +630,77 @@ __global__ void k_prescale_weights(const double* __restrict__ w, double* __restr
     }
 }
 
+__global__ void k_to_float(const double* __restrict__ in, float* __restrict__ out, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+const char* stage_name(int st) {
+    static const char* names[kStages] = {"phi_fwd", "dct_fwd", "leg_fwd", "naive_fwd", "leg_inv",
+                                         "dct_inv", "naive_inv", "phi_inv", "zonal", "mult"};
+    return (st >= 0 && st < kStages) ? names[st] : "?";
+}
+
+hipEvent_t StageTimer::take() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    S2_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void StageTimer::collect() {
+    for (auto& r : recs) {
+        float t = 0.f;
+        S2_HIP(hipEventSynchronize(r.b));
+        S2_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        ms[r.st] += t;
+        count[r.st] += 1;
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    recs.clear();
+}
+
+void StageTimer::reset() {
+    for (int i = 0; i < kStages; ++i) {
+        ms[i] = 0.0;
+        count[i] = 0;
+    }
+}
+
+StageTimer::~StageTimer() {
+    for (auto& r : recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+Span::Span(StageTimer* t_, int st_, hipStream_t s_) : t(t_ && t_->on ? t_ : nullptr), st(st_), s(s_) {
+    if (!t) return;
+    a = t->take();
+    S2_HIP(hipEventRecord(a, s));
+}
+
+Span::~Span() {
+    if (!t) return;
+    try {
+        hipEvent_t b = t->take();
+        if (hipEventRecord(b, s) == hipSuccess) {
+            t->recs.push_back({st, a, b});
+            return;
+        }
+        t->pool.push_back(b);
+    } catch (...) {
+    }
+    t->pool.push_back(a);
+}
 static int ilog2_exact(int n) {
     int l = 0;
     while ((1 << l) < n) ++l;
@@ -702,6 +729,7 @@ void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s)
     std::vector<double2> tw(N), ph(N);
     std::vector<float2> tw32(N), ph32(N);
     std::vector<double> sv(N);
+    std::vector<float> sv32(N);
     for (int t = 0; t < N; ++t) {
         const double a = 2.0 * M_PI * (double)t / N;
         tw[t] = make_double2(cos(a), -sin(a));
@@ -710,6 +738,7 @@ void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s)
         tw32[t] = make_float2((float)tw[t].x, (float)tw[t].y);
         ph32[t] = make_float2((float)ph[t].x, (float)ph[t].y);
         sv[t] = sin((2.0 * t + 1.0) * M_PI / (2.0 * N));
+        sv32[t] = (float)sv[t];
     }
     std::vector<int> deg((size_t)bw * bw);
     for (int m = -(bw - 1); m < bw; ++m)
@@ -719,12 +748,17 @@ void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s)
     S2_HIP(hipMalloc(&c.tw32, sizeof(float2) * N));
     S2_HIP(hipMalloc(&c.ph32, sizeof(float2) * N));
     S2_HIP(hipMalloc(&c.sinv, sizeof(double) * N));
+    S2_HIP(hipMalloc(&c.sinv32, sizeof(float) * N));
+    S2_HIP(hipMalloc(&c.w_semi32, sizeof(float) * 2 * N));
     S2_HIP(hipMalloc(&c.deg, sizeof(int) * deg.size()));
     S2_HIP(hipMemcpyAsync(c.tw, tw.data(), sizeof(double2) * N, hipMemcpyHostToDevice, s));
     S2_HIP(hipMemcpyAsync(c.ph, ph.data(), sizeof(double2) * N, hipMemcpyHostToDevice, s));
     S2_HIP(hipMemcpyAsync(c.tw32, tw32.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s));
     S2_HIP(hipMemcpyAsync(c.ph32, ph32.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s));
     S2_HIP(hipMemcpyAsync(c.sinv, sv.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    S2_HIP(hipMemcpyAsync(c.sinv32, sv32.data(), sizeof(float) * N, hipMemcpyHostToDevice, s));
+    k_to_float<<<(2 * N + 255) / 256, 256, 0, s>>>(c.w_semi, c.w_semi32, 2 * N);
+    S2_HIP(hipGetLastError());
     S2_HIP(hipMemcpyAsync(c.deg, deg.data(), sizeof(int) * deg.size(), hipMemcpyHostToDevice, s));
     S2_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
     S2_HIP(hipFree(d_w));
@@ -732,7 +766,7 @@ void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s)
 
 void consts_destroy(DevConsts& c) {
     for (void* p : {(void*)c.w_semi, (void*)c.w_naive, (void*)c.tw, (void*)c.ph, (void*)c.sinv, (void*)c.deg,
-                    (void*)c.tw32, (void*)c.ph32})
+                    (void*)c.tw32, (void*)c.ph32, (void*)c.w_semi32, (void*)c.sinv32})
         if (p) (void)hipFree(p);
     c = DevConsts{};
 }
@@ -840,25 +874,33 @@ static void check_launch_shape(const DevConsts& c) {
 }
 
 void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* cre,
-                  double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s) {
+                  double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
     const int rpb = rings_per_block(N);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    {
+    Span sp(tm, ST_PHI_FWD, s);
     k_phi_fwd<double2><<<dim3((N + rpb - 1) / rpb, F), 256, lds_fft, s>>>(re, im, X, N, c.logN, c.pow2, rpb, c.tw);
     S2_HIP(hipGetLastError());
+    }
     if (cut > 0) {
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+        {
+        Span sp(tm, ST_DCT_FWD, s);
         k_dct_fwd<double2><<<dim3((nrows + rpb - 1) / rpb, F), 256, lds_fft, s>>>(
             X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
+        }
         const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
+        Span sp(tm, ST_LEG_FWD, s);
         k_leg_fwd<<<dim3(cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
             Cb, t.fwd, t.off, cre, cim, N, B, F, fc, f, 0);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
         const int fc = std::max(1, 65536 / (2 * N * (int)sizeof(double2)));
+        Span sp(tm, ST_NAIVE_FWD, s);
         k_naive_fwd<<<dim3(B - cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * N * std::min(fc, F), s>>>(
             X, t.fwd, t.off, c.w_naive, cre, cim, N, B, cut, F, fc, f);
         S2_HIP(hipGetLastError());
@@ -866,7 +908,7 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
 }
 
 void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, const double* cim, double* re,
-                  double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s) {
+                  double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
     const int rpb = rings_per_block(N);
@@ -875,27 +917,34 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         const int head = ((B + 2) * (int)sizeof(int) + 15) / 16 * 16;  // row offsets
         const int fc = std::max(1, (65536 - head) / (2 * B * (int)sizeof(double2)));
         const size_t lds = head + sizeof(double2) * 2 * B * std::min(fc, F);
+        {
+        Span sp(tm, ST_LEG_INV, s);
         k_leg_inv<<<dim3(cut, (F + fc - 1) / fc), 256, lds, s>>>(cre, cim, t.inv, t.off, Cb, N, B, F, fc, f);
         S2_HIP(hipGetLastError());
+        }
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+        Span sp(tm, ST_DCT_INV, s);
         k_dct_inv<double2><<<dim3((nrows + rpb - 1) / rpb, F), 256, lds_fft, s>>>(
             Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
         const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
+        Span sp(tm, ST_NAIVE_INV, s);
         k_naive_inv<<<dim3(B - cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
             cre, cim, t.inv, t.off, X, N, B, cut, F, fc, f);
         S2_HIP(hipGetLastError());
     }
+    Span sp(tm, ST_PHI_INV, s);
     k_phi_inv<double2><<<dim3((N + rpb - 1) / rpb, F), 256, lds_fft, s>>>(X, re, im, N, c.logN, c.pow2, B, rpb,
                                                                           c.tw);
     S2_HIP(hipGetLastError());
 }
 
 void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* zr, double* zi,
-                int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s) {
+                int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
+    Span sp(tm, ST_ZONAL, s);
     if (t.cutoff < 1) throw ArgError{"the zonal transform needs the order-0 cosine table"};
     const int N = c.N, B = c.bw;
     k_ring_sum<<<dim3((N + 3) / 4, F), 256, 0, s>>>(re, im, X, N);
@@ -912,7 +961,8 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
 }
 
 void trans_mult(const DevConsts& c, const double* dr, const double* di, const double* hr, const double* hi,
-                long fstride, double* rr, double* ri, int F, hipStream_t s) {
+                long fstride, double* rr, double* ri, int F, hipStream_t s, StageTimer* tm) {
+    Span sp(tm, ST_MULT, s);
     const size_t total = (size_t)c.bw * c.bw * F;
     const int blocks = (int)std::min<size_t>(8192, (total + 255) / 256);
     k_trans_mult<<<blocks, 256, 0, s>>>(dr, di, hr, hi, fstride, rr, ri, c.deg, c.bw, F);
