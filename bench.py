@@ -31,27 +31,25 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 
-def stage_bytes(stage, bw, nfields, table_doubles, elem):
-    """Algorithmic HBM bytes of one launch of `stage` over nfields fields:
-    compulsory reads + writes of its inputs/outputs (grids and coefficients
-    FP64 complex; the order-major intermediates in `elem` bytes per complex
-    value), plus one pass over the table for the Legendre stages."""
+def stage_bytes(stage, bw, nfields, elem, form):
+    """Algorithmic HBM bytes of one launch of `stage` over nfields fields: the
+    compulsory reads + writes of its inputs and outputs.  Grids and
+    coefficients are complex FP64 (16 B); the order-major intermediates hold
+    `elem` bytes per complex value.  Tables (shared by all fields of a launch)
+    are left out."""
     n = 2 * bw
     rows = 2 * bw - 1  # order rows used (all but the Nyquist row)
     grid = 16 * n * n
     coef = 16 * bw * bw
-    xrow = elem * n * rows
-    crow = elem * bw * rows
-    per_field = {
-        "phi_fwd": grid + elem * n * n,
-        "dct_fwd": xrow + crow,
-        "leg_fwd": crow + coef,
-        "leg_inv": coef + crow,
-        "dct_inv": crow + xrow,
-        "phi_inv": elem * n * rows + grid,
-    }[stage]
-    table = 8 * table_doubles if stage in ("leg_fwd", "leg_inv") else 0
-    return per_field * nfields + table
+    xrow = elem * n * rows  # phi-FFT output rows that the next stage reads
+    crow = elem * bw * rows  # cosine coefficients (seminaive forms)
+    if form == "mixed":
+        per_field = {"phi_fwd": grid + elem * n * n, "leg_fwd": xrow + coef, "leg_inv": coef + xrow,
+                     "phi_inv": xrow + grid}[stage]
+    else:
+        per_field = {"phi_fwd": grid + elem * n * n, "dct_fwd": xrow + crow, "leg_fwd": crow + coef,
+                     "leg_inv": coef + crow, "dct_inv": crow + xrow, "phi_inv": xrow + grid}[stage]
+    return per_field * nfields
 
 
 def cpu_worker(args):
@@ -93,7 +91,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bw", type=int, default=256)
     ap.add_argument("--fields", type=int, default=1024)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="mixed")
     ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
     ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
@@ -119,7 +117,7 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     n = 2 * bw
-    prec = {"fp64": S.FP64, "fp32": S.FP32}[args.precision]
+    prec = {"fp64": S.FP64, "fp32": S.FP32, "mixed": S.MIXED}[args.precision]
 
     # ---- workload: band-limited grids from power-law coefficients ---------
     c0r, c0i = synth.power_law_coeffs(bw, F, 1234, dev)
@@ -149,12 +147,12 @@ def main():
     plan.set_timing(False)
     value = F * args.steps / dt
 
-    # ---- errors of the timed form against the FP64 form --------------------
+    # ---- errors of the timed form against the FP64 form, same inputs --------
+    # forward: grids gre -> coefficients; inverse: coefficients c0 -> grids
     ref_cr, ref_ci = torch.empty_like(c0r), torch.empty_like(c0i)
     p64.forward(gre, gim, ref_cr, ref_ci)
-    ref_gr, ref_gi = torch.empty_like(gre), torch.empty_like(gim)
-    p64.inverse(ref_cr, ref_ci, ref_gr, ref_gi)
-    step()
+    plan.forward(gre, gim, cre, cim)
+    plan.inverse(c0r, c0i, ore, oim)
     torch.cuda.synchronize()
 
     def rel(ar, ai, br, bi):
@@ -162,10 +160,14 @@ def main():
                      torch.complex(br, bi).abs().max())
 
     errors = {
-        "coeff_vs_fp64": rel(cre, cim, ref_cr, ref_ci),
-        "grid_vs_fp64": rel(ore, oim, ref_gr, ref_gi),
+        "forward_vs_fp64": rel(cre, cim, ref_cr, ref_ci),
+        "inverse_vs_fp64": rel(ore, oim, gre, gim),
         "fp64_round_trip": rel(ref_cr, ref_ci, c0r, c0i),
     }
+    # the timed form's own round trip, coefficients -> grids -> coefficients
+    plan.forward(ore, oim, cre, cim)
+    torch.cuda.synchronize()
+    errors["round_trip"] = rel(cre, cim, c0r, c0i)
 
     # ---- batch-1 FP64 form: one field per call -----------------------------
     plain = None
@@ -185,14 +187,11 @@ def main():
         p1.close()
 
     # ---- roofline of the dominant kernel ----------------------------------
-    import ctypes
-    f_ptr, i_ptr, ndbl = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_long()
-    S.lib().s2amd_plan_tables(plan.h, ctypes.byref(f_ptr), ctypes.byref(i_ptr), ctypes.byref(ndbl))
-    elem = 16 if prec == S.FP64 else 8
+    elem = 16 if prec == S.FP64 else 8  # bytes per complex intermediate value
     dom = max(stages, key=lambda k: stages[k][0])
     dom_ms, dom_n = stages[dom]
     avg_s = dom_ms / dom_n / 1e3
-    achieved = stage_bytes(dom, bw, F, ndbl.value, elem) / avg_s
+    achieved = stage_bytes(dom, bw, F, elem, args.precision) / avg_s
     roofline = {
         "bound": "hbm",
         "kernel": dom,
@@ -216,7 +215,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"batched fwd+inv SHT, bw={bw}, {F} fields, {n}x{n} complex FP64 grids, "
                                "power-law (l+1)^-2 coefficients", "bw": bw, "fields": F,

@@ -32,8 +32,16 @@ enum {
 
 /* Arithmetic of the transforms (inputs and outputs are FP64 in every form). */
 typedef enum {
-    S2AMD_FP64 = 0, /* FP64 throughout: matches the reference to ~1e-13 */
-    S2AMD_FP32 = 1  /* FP32 FFTs, DCTs and Legendre sums; FP64 tables and I/O */
+    /* FP64 throughout (seminaive: DCT + cosine-series tables, FP64 MFMA).
+     * Matches the reference to ~1e-13. */
+    S2AMD_FP64 = 0,
+    /* seminaive in FP32 (FFTs, DCTs, FP32-MFMA projections). Measured error
+     * 1e-6 .. 4e-5 relative: the cosine-space sums cancel. */
+    S2AMD_FP32 = 1,
+    /* mixed: FP32 ring FFTs and the naive Legendre transform (naive.c) as an
+     * FP32-MFMA GEMM over m on tables from the FP64 recurrence; FP64 in/out.
+     * Error ~1e-7 relative.  bw a power of two in [32, 512]. */
+    S2AMD_MIXED = 2
 } s2amd_precision;
 
 typedef struct s2amd_plan s2amd_plan;

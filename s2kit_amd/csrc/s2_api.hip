@@ -285,7 +285,9 @@ struct Plan {
 
 static void plan_forward(Plan* p, const double* re, const double* im, double* cre, double* cim, int F, Fmt fmt,
                          hipStream_t s) {
-    if (p->fast)
+    if (p->precision == S2AMD_MIXED)
+        forward_mixed(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, s, &p->timer);
+    else if (p->fast)
         forward_fast(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
     else
         forward_fp64(p->c, p->t, re, im, cre, cim, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
@@ -293,7 +295,9 @@ static void plan_forward(Plan* p, const double* re, const double* im, double* cr
 
 static void plan_inverse(Plan* p, const double* cre, const double* cim, double* re, double* im, int F, Fmt fmt,
                          hipStream_t s) {
-    if (p->fast)
+    if (p->precision == S2AMD_MIXED)
+        inverse_mixed(p->c, p->g, cre, cim, re, im, F, fmt, p->X.p, s, &p->timer);
+    else if (p->fast)
         inverse_fast(p->c, p->g, cre, cim, re, im, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
     else
         inverse_fp64(p->c, p->t, cre, cim, re, im, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
@@ -716,9 +720,12 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
         check_common(bw, 0);
         if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
         if (max_fields < 1) throw ArgError{"max_fields must be >= 1"};
-        if (precision != S2AMD_FP64 && precision != S2AMD_FP32) throw ArgError{"unknown precision"};
+        if (precision != S2AMD_FP64 && precision != S2AMD_FP32 && precision != S2AMD_MIXED)
+            throw ArgError{"unknown precision"};
+        if (precision == S2AMD_MIXED && !mixed_supported(bw))
+            throw ArgError{"the mixed form needs a power-of-two grid with bw in [32, 512]"};
         if (precision == S2AMD_FP32 && !fast_supported(bw, cutoff, 1))
-            throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 1024] and cutoff = bw"};
+            throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 512] and cutoff = bw"};
         std::unique_ptr<s2amd_plan> p(new s2amd_plan());
         p->bw = bw;
         p->cutoff = cutoff;
@@ -730,12 +737,16 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
         try {
             consts_create(p->c, bw, nullptr, s);
             tables_generate(p->t, bw, cutoff, true, true, s);
-            p->fast = fast_supported(bw, cutoff, precision);
-            if (p->fast) gemm_tables_create(p->g, p->t, precision, s);
+            if (precision == S2AMD_MIXED) {
+                naive_tables_create(p->g, p->c, s);
+            } else {
+                p->fast = fast_supported(bw, cutoff, precision);
+                if (p->fast) gemm_tables_create(p->g, p->t, precision, s);
+            }
             const size_t N = 2 * (size_t)bw, F = max_fields;
             // large enough for the fast layouts and for the FP64 zonal transform
-            p->X.reserve(std::max(sizeof(double2) * N * N, fast_x_bytes(bw, precision)) * F);
-            p->Cb.reserve(std::max(sizeof(double2) * N * bw, fast_cb_bytes(bw, precision)) * F);
+            p->X.reserve(sizeof(double2) * N * N * F);  // largest layout (FP64 generic / zonal)
+            p->Cb.reserve(sizeof(double2) * N * bw * F);
             p->cr.reserve(sizeof(double) * bw * bw * F);
             p->ci.reserve(sizeof(double) * bw * bw * F);
             p->zr.reserve(sizeof(double) * bw * F);
@@ -767,7 +778,7 @@ extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
 
 extern "C" size_t s2amd_plan_bytes(const s2amd_plan* p) {
     if (!p) return 0;
-    const size_t el = p->precision == S2AMD_FP64 ? sizeof(double) : sizeof(float);
+    const size_t el = p->precision == S2AMD_FP64 ? sizeof(double) : sizeof(float);  // GEMM tables
     return 2 * sizeof(double) * p->t.total + el * (p->g.nf + p->g.ni) + p->X.n + p->Cb.n + p->cr.n + p->ci.n +
            p->zr.n + p->zi.n;
 }

@@ -26,6 +26,7 @@
 
 #include "s2_complex.h"
 #include "s2_internal.h"
+#include "s2_recurrence.h"
 
 namespace s2 {
 
@@ -515,6 +516,224 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Mixed form: naive DLT (naive.c:34-95) as FP32 MFMA GEMMs on the phi-FFT rows
+// ---------------------------------------------------------------------------
+// P_l^m(theta_{N-1-j}) = (-1)^(l-m) P_l^m(theta_j) and the quadrature weights
+// are symmetric, so each order folds the ring pair (j, N-1-j): degrees with
+// l-m even see x_j + x_{N-1-j}, odd ones x_j - x_{N-1-j}, over j < bw.
+//   NF(m,p) [j][i]  P_{m+p+2i}^m(theta_j) w_j sqrt(2pi)/N  (j < bw, i < roundup16(Lp))
+//   NI(m,p) [i][j]  P_{m+p+2i}^m(theta_j)                  (i < roundup4(Lp), j < bw)
+// Both come from the reference's FP64 recurrence (pml.c:41-79), rounded once.
+
+// XOR swizzle of an LDS row index: 16 consecutive columns reading the same
+// k land in 16 different 4-bank groups (bw is a multiple of 32 here).
+__device__ __forceinline__ int swz(int c, int B) { return ((c & 15) << 2) & (B - 1); }
+
+__global__ void k_pack_naive(const double* __restrict__ w, const GemmMeta* __restrict__ meta, float* __restrict__ NF,
+                             float* __restrict__ NI, int B) {
+    const int m = blockIdx.x;
+    const int N = 2 * B;
+    const double cm = (m == 0) ? 0.0 : pmm_coeff(m);
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int L16_0 = (g0.Lp + 15) & ~15, L16_1 = (g1.Lp + 15) & ~15;
+    for (int j = threadIdx.x; j < B; j += blockDim.x) {
+        const double th = (2.0 * j + 1.0) * M_PI / (2.0 * N);
+        const double x = cos((2.0 * j + 1.0) * M_PI / (2.0 * N));
+        const double wj = w[j];  // even weight times sqrt(2pi)/N
+        double cur = (m == 0) ? 0.70710678118654752440 : cm * pow(sin(th), (double)m);
+        double prev = 0.0;
+        for (int l = m; l < B; ++l) {
+            const int p = (l - m) & 1, i = (l - m) >> 1;
+            const GemmMeta& g = p ? g1 : g0;
+            NF[g.offF + (long)j * (p ? L16_1 : L16_0) + i] = (float)(cur * wj);
+            NI[g.offI + (long)i * B + j] = (float)cur;
+            if (l + 1 < B) {
+                const double nx = l2_an(m, l) * (cur * x) + l2_cn(m, l) * prev;
+                prev = cur;
+                cur = nx;
+            }
+        }
+    }
+}
+
+// Forward: coefficient[c][i] = sum_j fold_p(x)[c][j] * NF(m,p)[j][i].
+// grid (bw, ceil(cols/ncb)); LDS Sd[c][p][j] (swizzled j), 2*bw floats per column.
+__global__ __launch_bounds__(kThreads) void k_ngemm_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
+                                                        const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                                        double* __restrict__ cim, int lgB, int F, int ncb, int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* Sd = reinterpret_cast<float*>(smem);
+    const int B = 1 << lgB, N = 2 * B;
+    const int m = blockIdx.x;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    const int cbase = blockIdx.y * ncb;
+    if (cbase >= ncols) return;
+    // one (field, sign) row feeds two columns (re, im)
+    for (int idx = threadIdx.x; idx < (ncb >> 1) * B; idx += kThreads) {
+        const int pr = idx >> lgB, j = idx & (B - 1);
+        const int c = 2 * pr, col = cbase + c;
+        float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
+        if (col < ncols) {
+            const int f = col / cpf, rem = col - f * cpf;
+            const int r = (rem >> 1) ? N - m : m;
+            const float2* row = X + ((size_t)f * N + r) * N;
+            a = row[j];
+            b = row[N - 1 - j];
+        }
+        const int s0 = swz(c, B), s1 = swz(c + 1, B);
+        float* d0 = Sd + (size_t)c * 2 * B;
+        float* d1 = d0 + 2 * B;
+        d0[j ^ s0] = a.x + b.x;
+        d1[j ^ s1] = a.y + b.y;
+        d0[B + (j ^ s0)] = a.x - b.x;
+        d1[B + (j ^ s1)] = a.y - b.y;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nct = ncb >> 4;
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    for (int p = 0; p < 2; ++p) {
+        const GemmMeta g = meta[m * 2 + p];
+        if (g.Lp == 0) continue;
+        const int L16 = (g.Lp + 15) & ~15;
+        const int ntiles = (L16 >> 4) * nct;
+        for (int item = wave; item < ntiles; item += kThreads / 64) {
+            const int it = item / nct, ct = item - it * nct;
+            const int i0 = it * 16, c0 = ct * 16;
+            const int c = c0 + (lane & 15);
+            const float* arow = Sd + (size_t)c * 2 * B + p * B;
+            const int sc = swz(c, B);
+            const float* brow = NF + g.offF + i0 + (lane & 15);
+            fvec4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+            for (int j0 = 0; j0 < B; j0 += 8) {
+                const int ja = j0 + (lane >> 4), jb = ja + 4;
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[ja ^ sc], brow[(size_t)ja * L16], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[jb ^ sc], brow[(size_t)jb * L16], acc1, 0, 0, 0);
+            }
+            const int i = i0 + (lane & 15);
+            if (i >= g.Lp) continue;
+            const int l = m + p + 2 * i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = cbase + c0 + Mma<float>::row(lane, r);
+                if (col >= ncols) continue;
+                const int f = col / cpf, rem = col - f * cpf;
+                const int comp = rem & 1;
+                double* out = comp ? cim : cre;
+                const double v = (double)acc0[r] + (double)acc1[r];
+                if ((rem >> 1) == 0) {
+                    out[f * B2 + index_of_coeff(m, l, B)] = v;
+                    if (fmt == 1 && m > 0) out[f * B2 + index_of_coeff(-m, l, B)] = comp ? -sg * v : sg * v;
+                } else {
+                    out[f * B2 + index_of_coeff(-m, l, B)] = sg * v;
+                }
+            }
+        }
+    }
+}
+
+// Inverse: E/O[c][j] = sum_i coef_p[c][i] * NI(m,p)[i][j]; ring j gets E+O,
+// ring N-1-j gets E-O; x 1/sqrt(2pi), (-1) on odd negative rows, REAL mirror.
+// grid (bw, ceil(cols/ncb)); LDS Cd[c][q] (swizzled), q = i or roundup4(Lp0)+i.
+__global__ __launch_bounds__(kThreads) void k_ngemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
+                                                        const float* __restrict__ NI,
+                                                        const GemmMeta* __restrict__ meta, float2* __restrict__ X,
+                                                        int lgB, int F, int ncb, int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* Cd = reinterpret_cast<float*>(smem);
+    const int B = 1 << lgB, N = 2 * B;
+    const int m = blockIdx.x;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    const int cbase = blockIdx.y * ncb;
+    if (cbase >= ncols) return;
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
+    const int L = B - m;
+    const size_t B2 = (size_t)B * B;
+    for (int idx = threadIdx.x; idx < ncb * L; idx += kThreads) {
+        const int c = idx / L, li = idx - c * L;
+        const int col = cbase + c;
+        const int q = (li & 1) ? Li4_0 + (li >> 1) : (li >> 1);
+        float v = 0.f;
+        if (col < ncols) {
+            const int f = col / cpf, rem = col - f * cpf;
+            const double* src = (rem & 1) ? cim : cre;
+            v = (float)src[f * B2 + index_of_coeff((rem >> 1) ? -m : m, m + li, B)];
+        }
+        Cd[(size_t)c * B + (q ^ swz(c, B))] = v;
+    }
+    for (int idx = threadIdx.x; idx < ncb * 8; idx += kThreads) {
+        const int c = idx >> 3, k = idx & 7;
+        const int p = k >> 2, i = (p ? g1.Lp : g0.Lp) + (k & 3);
+        if (i < (p ? Li4_1 : Li4_0)) Cd[(size_t)c * B + (((p ? Li4_0 : 0) + i) ^ swz(c, B))] = 0.f;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nct = ncb >> 4, ntj = B >> 4;
+    const float inv_s2pi = 0.39894228040143267794f;
+    for (int item = wave; item < ntj * nct; item += kThreads / 64) {
+        const int jt = item / nct, ct = item - jt * nct;
+        const int j0 = jt * 16, c0 = ct * 16;
+        const int c = c0 + (lane & 15);
+        const int sc = swz(c, B);
+        const float* acol = Cd + (size_t)c * B;
+        fvec4 e = {0, 0, 0, 0}, o = {0, 0, 0, 0};
+        {
+            const float* brow = NI + g0.offI + j0 + (lane & 15);
+            fvec4 e1 = {0, 0, 0, 0};
+            int i0 = 0;
+            for (; i0 + 8 <= Li4_0; i0 += 8) {
+                const int ia = i0 + (lane >> 4), ib = ia + 4;
+                e = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[ia ^ sc], brow[(size_t)ia * B], e, 0, 0, 0);
+                e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[ib ^ sc], brow[(size_t)ib * B], e1, 0, 0, 0);
+            }
+            if (i0 < Li4_0) {
+                const int ia = i0 + (lane >> 4);
+                e = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[ia ^ sc], brow[(size_t)ia * B], e, 0, 0, 0);
+            }
+            e += e1;
+        }
+        if (g1.Lp > 0) {
+            const float* brow = NI + g1.offI + j0 + (lane & 15);
+            fvec4 o1 = {0, 0, 0, 0};
+            int i0 = 0;
+            for (; i0 + 8 <= Li4_1; i0 += 8) {
+                const int ia = i0 + (lane >> 4), ib = ia + 4;
+                o = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[(Li4_0 + ia) ^ sc], brow[(size_t)ia * B], o, 0, 0, 0);
+                o1 = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[(Li4_0 + ib) ^ sc], brow[(size_t)ib * B], o1, 0, 0, 0);
+            }
+            if (i0 < Li4_1) {
+                const int ia = i0 + (lane >> 4);
+                o = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[(Li4_0 + ia) ^ sc], brow[(size_t)ia * B], o, 0, 0, 0);
+            }
+            o += o1;
+        }
+        const int j = j0 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = cbase + c0 + Mma<float>::row(lane, r);
+            if (col >= ncols) continue;
+            const int f = col / cpf, rem = col - f * cpf;
+            const int row = (rem >> 1) ? N - m : m;
+            const float s = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
+            const float yn = (e[r] + o[r]) * s, ys = (e[r] - o[r]) * s;
+            float* xr = reinterpret_cast<float*>(X + ((size_t)f * N + row) * N) + (rem & 1);
+            xr[2 * j] = yn;
+            xr[2 * (N - 1 - j)] = ys;
+            if (fmt == 1 && m > 0) {
+                float* xm = reinterpret_cast<float*>(X + ((size_t)f * N + (N - m)) * N) + (rem & 1);
+                const float cs = (rem & 1) ? -1.f : 1.f;
+                xm[2 * j] = cs * yn;
+                xm[2 * (N - 1 - j)] = cs * ys;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 static int lg2(int n) { return __builtin_ctz(n); }
@@ -653,6 +872,87 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     {
         Span sp(tm, ST_PHI_INV, s);
         k_fphi_inv<R, V><<<dim3(N / rpb, F), kThreads, lds, s>>>(X, re, im, lgN, tw);
+        S2_HIP(hipGetLastError());
+    }
+}
+
+bool mixed_supported(int bw) {
+    const int N = 2 * bw;
+    return bw >= 32 && bw <= 512 && !(N & (N - 1));
+}
+
+void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
+    const int B = c.bw;
+    g.bw = B;
+    g.prec = 2;
+    g.h_meta.assign(2 * B, GemmMeta{});
+    long of = 0, oi = 0;
+    for (int m = 0; m < B; ++m)
+        for (int p = 0; p < 2; ++p) {
+            GemmMeta& e = g.h_meta[m * 2 + p];
+            e.Lp = gm_lp(m, p, B);
+            e.Kf = B;
+            e.offF = of;
+            e.offI = oi;
+            of += (long)B * ((e.Lp + 15) & ~15);
+            oi += (long)((e.Lp + 3) & ~3) * B;
+        }
+    g.nf = of;
+    g.ni = oi;
+    S2_HIP(hipMalloc(&g.meta, sizeof(GemmMeta) * 2 * B));
+    S2_HIP(hipMalloc(&g.af, sizeof(float) * std::max(1L, of)));
+    S2_HIP(hipMalloc(&g.ai, sizeof(float) * std::max(1L, oi)));
+    S2_HIP(hipMemsetAsync(g.af, 0, sizeof(float) * std::max(1L, of), s));
+    S2_HIP(hipMemsetAsync(g.ai, 0, sizeof(float) * std::max(1L, oi), s));
+    S2_HIP(hipMemcpyAsync(g.meta, g.h_meta.data(), sizeof(GemmMeta) * 2 * B, hipMemcpyHostToDevice, s));
+    k_pack_naive<<<B, 256, 0, s>>>(c.w_naive, g.meta, (float*)g.af, (float*)g.ai, B);
+    S2_HIP(hipGetLastError());
+    S2_HIP(hipStreamSynchronize(s));
+}
+
+static int ngemm_ncb(int B, bool fwd) {
+    const int row = (fwd ? 2 : 1) * B * (int)sizeof(float);
+    return std::min(64, 65536 / row) & ~15;
+}
+
+void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
+                   double* cim, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
+    constexpr int V = 16;
+    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int rpb = V * kThreads / N;
+    float2* X = (float2*)Xv;
+    {
+        Span sp(tm, ST_PHI_FWD, s);
+        k_fphi_fwd<float, V><<<dim3(N / rpb, F), kThreads, sizeof(float2) * (size_t)rpb * (N + 1), s>>>(
+            re, im, X, lgN, c.tw32);
+        S2_HIP(hipGetLastError());
+    }
+    {
+        const int ncb = ngemm_ncb(B, true);
+        Span sp(tm, ST_LEG_FWD, s);
+        k_ngemm_fwd<<<dim3(B, (4 * F + ncb - 1) / ncb), kThreads, sizeof(float) * 2 * B * ncb, s>>>(
+            X, (const float*)g.af, g.meta, cre, cim, lg2(B), F, ncb, (int)fmt);
+        S2_HIP(hipGetLastError());
+    }
+}
+
+void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
+                   double* im, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
+    constexpr int V = 16;
+    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int rpb = V * kThreads / N;
+    float2* X = (float2*)Xv;
+    {
+        const int ncb = ngemm_ncb(B, false);
+        Span sp(tm, ST_LEG_INV, s);
+        k_ngemm_inv<<<dim3(B, (4 * F + ncb - 1) / ncb), kThreads, sizeof(float) * B * ncb, s>>>(
+            cre, cim, (const float*)g.ai, g.meta, X, lg2(B), F, ncb, (int)fmt);
+        S2_HIP(hipGetLastError());
+    }
+    {
+        Span sp(tm, ST_PHI_INV, s);
+        k_fphi_inv<float, V><<<dim3(N / rpb, F), kThreads, sizeof(float2) * (size_t)rpb * (N + 1), s>>>(
+            X, re, im, lgN, c.tw32);
         S2_HIP(hipGetLastError());
     }
 }

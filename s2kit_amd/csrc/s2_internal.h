@@ -71,7 +71,7 @@ struct GemmMeta {
 };
 
 struct GemmTables {
-    int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32
+    int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32 (seminaive); 2: FP32 naive (mixed form)
     void* af = nullptr;
     void* ai = nullptr;
     GemmMeta* meta = nullptr;
@@ -163,5 +163,13 @@ void forward_fast(const DevConsts& c, const GemmTables& g, const double* re, con
                   double* cim, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm);
 void inverse_fast(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
                   double* im, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm);
+
+// ---- mixed form (s2_fast.hip): FP32 ring FFTs + naive DLT on FP32 MFMA ----
+bool mixed_supported(int bw);
+void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s);
+void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
+                   double* cim, int F, Fmt fmt, void* X, hipStream_t s, StageTimer* tm);
+void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
+                   double* im, int F, Fmt fmt, void* X, hipStream_t s, StageTimer* tm);
 
 }  // namespace s2

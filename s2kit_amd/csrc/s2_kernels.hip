@@ -23,6 +23,7 @@
 
 #include "s2_complex.h"
 #include "s2_internal.h"
+#include "s2_recurrence.h"
 
 namespace s2 {
 
@@ -457,27 +458,6 @@ __global__ void k_trans_mult(const double* dr, const double* di,
 // ---------------------------------------------------------------------------
 // tables and weights (FP64)
 // ---------------------------------------------------------------------------
-
-// L2-normalised seed P_m^m(cos th) (pmm.c:21-34) and recurrence coefficients
-// (l2_norms.c:16-38), in the reference's operation order.
-__device__ double pmm_coeff(int m) {
-    double c = sqrt(m + 0.5);
-    for (int i = 0; i < m; ++i) c *= sqrt((m - (i / 2.0)) / ((double)m - i));
-    if (m) c *= pow(2.0, -m / 2.0);
-    if (m & 1) c = -c;
-    return c;
-}
-__device__ __forceinline__ double l2_an(int m, int l) {
-    return sqrt(((2.0 * l + 3.0) / (2.0 * l + 1.0)) * ((l - m + 1.0) / (l + m + 1.0))) *
-           ((2.0 * l + 1.0) / (l - m + 1.0));
-}
-__device__ __forceinline__ double l2_cn(int m, int l) {
-    if (l == 0) return 0.0;
-    return -1.0 *
-           sqrt(((2.0 * l + 3.0) / (2.0 * l - 1.0)) * ((l - m + 1.0) / (l + m + 1.0)) *
-                (((double)l - m) / ((double)l + m))) *
-           ((l + m) / (l - m + 1.0));
-}
 
 // cosT[k*B + i] = cos(pi (2i+1) k / (2B)), argument reduced mod 2 pi exactly.
 __global__ void k_cos_table(double* __restrict__ cosT, int B) {
