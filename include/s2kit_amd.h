@@ -40,7 +40,7 @@ typedef enum {
     S2AMD_FP32 = 1,
     /* mixed: FP32 ring FFTs and the naive Legendre transform (naive.c) as an
      * FP32-MFMA GEMM over m on tables from the FP64 recurrence; FP64 in/out.
-     * Error ~1e-7 relative.  bw a power of two in [32, 512]. */
+     * Error ~1e-7 relative.  bw a power of two in [64, 512]. */
     S2AMD_MIXED = 2
 } s2amd_precision;
 

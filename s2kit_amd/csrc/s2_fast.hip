@@ -150,6 +150,33 @@ __device__ void fft_block(T* a, int lgN, int S, const T* __restrict__ tw) {
 
 __device__ __forceinline__ int frow_of(int q, int N, int cut) { return q < cut ? q : N - 2 * cut + 1 + q; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8), and
+// each XCD has its own L2.  Renumber blocks so that every XCD works through a
+// contiguous range of logical blocks (x fastest): neighbours that share cache
+// lines (transposed 64-byte row segments) or a table tile (same order m) then
+// meet in the same L2.  A bijection on [0, gridDim.x * gridDim.y).
+__device__ __forceinline__ void xcd_block(int& bx, int& by) {
+    const int nx = gridDim.x, total = nx * gridDim.y;
+    const int id = blockIdx.y * nx + blockIdx.x;
+    const int q = total >> 3, r = total & 7, k = id & 7, local = id >> 3;
+    const int lid = (k < r) ? k * (q + 1) + local : r * (q + 1) + (k - r) * q + local;
+    by = lid / nx;
+    bx = lid - by * nx;
+}
+
+// GEMM grids (column blocks x orders, orders a multiple of 8): XCD k takes
+// the orders m = k, k+8, k+16, ... (the work per order falls with m, so this
+// balances the XCDs) and runs all column blocks of one order back to back,
+// so that order's table tile stays in its L2.
+__device__ __forceinline__ void xcd_block_m(int& bx, int& m) {
+    const int nx = gridDim.x;
+    const int id = blockIdx.y * nx + blockIdx.x;
+    const int k = id & 7, local = id >> 3;
+    const int slot = local / nx;
+    bx = local - slot * nx;
+    m = k + 8 * slot;
+}
+
 // position of cosine index k in the parity-split planes of the Cb buffer
 __device__ __forceinline__ int cpos(int k, int half) { return (k & 1) ? half + (k >> 1) : (k >> 1); }
 
@@ -166,7 +193,9 @@ __global__ __launch_bounds__(kThreads) void k_fphi_fwd(const double* __restrict_
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
-    const int f = blockIdx.y, j0 = blockIdx.x * rpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int j0 = bx * rpb;
     const size_t base = ((size_t)f * N + j0) * N;
     for (int idx = threadIdx.x; idx < E; idx += kThreads)
         a[(idx >> lgN) * S + (idx & (N - 1))] = mk<T>((R)re[base + idx], (R)im[base + idx]);
@@ -189,7 +218,9 @@ __global__ __launch_bounds__(kThreads) void k_fdct_fwd(const typename Cplx<R>::t
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
-    const int f = blockIdx.y, q0 = blockIdx.x * rpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int q0 = bx * rpb;
     for (int idx = threadIdx.x; idx < E; idx += kThreads) {
         const int rl = idx >> lgN, j = idx & (N - 1);
         const int q = q0 + rl;
@@ -231,7 +262,9 @@ __global__ __launch_bounds__(kThreads) void k_fdct_inv(const R* __restrict__ Cb,
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
-    const int f = blockIdx.y, q0 = blockIdx.x * rpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int q0 = bx * rpb;
     // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]),  X[N] = 0, X[k >= B] = 0
     for (int idx = threadIdx.x; idx < E; idx += kThreads) {
         const int rl = idx >> lgN, k = idx & (N - 1);
@@ -277,7 +310,9 @@ __global__ __launch_bounds__(kThreads) void k_fphi_inv(const typename Cplx<R>::t
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1, B = N >> 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
-    const int f = blockIdx.y, j0 = blockIdx.x * rpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int j0 = bx * rpb;
     for (int idx = threadIdx.x; idx < E; idx += kThreads) {
         const int r = idx >> lgrpb, jl = idx & (rpb - 1);
         a[jl * S + r] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + jl];
@@ -373,7 +408,7 @@ __host__ __device__ constexpr int lds_pad() {
     return sizeof(R) == 4 ? 4 : 2;
 }
 
-// Forward: D'[c][i] = sum_t data[c][t] * A(m,p)[i][t].  grid (cut, ceil(cols/ncb)).
+// Forward: D'[c][i] = sum_t data[c][t] * A(m,p)[i][t].  grid (ceil(cols/ncb), bw).
 // LDS: Bd[c][k2], k2 = parity-split cosine position (p*bw/2 + t).
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb, const R* __restrict__ AF,
@@ -382,10 +417,11 @@ __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
     extern __shared__ __align__(16) unsigned char smem[];
     R* Bd = reinterpret_cast<R*>(smem);
     const int B = 1 << lgB, N = 2 * B, half = B >> 1;
-    const int m = blockIdx.x;
+    int bx, m;
+    xcd_block_m(bx, m);
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
-    const int cbase = blockIdx.y * ncb;
+    const int cbase = bx * ncb;
     if (cbase >= ncols) return;
     const int rs = B + lds_pad<R>();
     for (int idx = threadIdx.x; idx < ncb * B; idx += kThreads) {
@@ -443,7 +479,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
     }
 }
 
-// Inverse: D'[c][t] = sum_i coef[c][i] * AI(m,p)[i][t].  grid (cut, ceil(cols/ncb)).
+// Inverse: D'[c][t] = sum_i coef[c][i] * AI(m,p)[i][t].  grid (ceil(cols/ncb), bw).
 // LDS: Cd[c][q], q = i (p = 0) or roundup4(Lp0) + i (p = 1); q < bw always.
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
@@ -452,10 +488,11 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
     extern __shared__ __align__(16) unsigned char smem[];
     R* Cd = reinterpret_cast<R*>(smem);
     const int B = 1 << lgB, N = 2 * B, half = B >> 1;
-    const int m = blockIdx.x;
+    int bx, m;
+    xcd_block_m(bx, m);
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
-    const int cbase = blockIdx.y * ncb;
+    const int cbase = bx * ncb;
     if (cbase >= ncols) return;
     const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
     const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
@@ -521,13 +558,16 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
 // P_l^m(theta_{N-1-j}) = (-1)^(l-m) P_l^m(theta_j) and the quadrature weights
 // are symmetric, so each order folds the ring pair (j, N-1-j): degrees with
 // l-m even see x_j + x_{N-1-j}, odd ones x_j - x_{N-1-j}, over j < bw.
-//   NF(m,p) [j][i]  P_{m+p+2i}^m(theta_j) w_j sqrt(2pi)/N  (j < bw, i < roundup16(Lp))
+//   NF(m,p) [j][i]  P_{m+p+2i}^m(theta_j) w_j sqrt(2pi)/N  (j < bw, i < roundup64(Lp))
 //   NI(m,p) [i][j]  P_{m+p+2i}^m(theta_j)                  (i < roundup4(Lp), j < bw)
 // Both come from the reference's FP64 recurrence (pml.c:41-79), rounded once.
+// Columns of both are interleaved in groups of 64: entry 16q + x of a group
+// sits at 4x + q, so one float4 per lane feeds four 16-wide MFMA tiles.
+__host__ __device__ inline int ilv64(int i) { return (i & ~63) + 4 * (i & 15) + ((i >> 4) & 3); }
 
 // XOR swizzle of an LDS row index: 16 consecutive columns reading the same
-// k land in 16 different 4-bank groups (bw is a multiple of 32 here).
-__device__ __forceinline__ int swz(int c, int B) { return ((c & 15) << 2) & (B - 1); }
+// k land in 16 different 4-bank groups (bw is a multiple of 64 here).
+__device__ __forceinline__ int swz(int c) { return (c & 15) << 2; }
 
 __global__ void k_pack_naive(const double* __restrict__ w, const GemmMeta* __restrict__ meta, float* __restrict__ NF,
                              float* __restrict__ NI, int B) {
@@ -535,7 +575,7 @@ __global__ void k_pack_naive(const double* __restrict__ w, const GemmMeta* __res
     const int N = 2 * B;
     const double cm = (m == 0) ? 0.0 : pmm_coeff(m);
     const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
-    const int L16_0 = (g0.Lp + 15) & ~15, L16_1 = (g1.Lp + 15) & ~15;
+    const int L64_0 = (g0.Lp + 63) & ~63, L64_1 = (g1.Lp + 63) & ~63;
     for (int j = threadIdx.x; j < B; j += blockDim.x) {
         const double th = (2.0 * j + 1.0) * M_PI / (2.0 * N);
         const double x = cos((2.0 * j + 1.0) * M_PI / (2.0 * N));
@@ -545,8 +585,8 @@ __global__ void k_pack_naive(const double* __restrict__ w, const GemmMeta* __res
         for (int l = m; l < B; ++l) {
             const int p = (l - m) & 1, i = (l - m) >> 1;
             const GemmMeta& g = p ? g1 : g0;
-            NF[g.offF + (long)j * (p ? L16_1 : L16_0) + i] = (float)(cur * wj);
-            NI[g.offI + (long)i * B + j] = (float)cur;
+            NF[g.offF + (long)j * (p ? L64_1 : L64_0) + ilv64(i)] = (float)(cur * wj);
+            NI[g.offI + (long)i * B + ilv64(j)] = (float)cur;
             if (l + 1 < B) {
                 const double nx = l2_an(m, l) * (cur * x) + l2_cn(m, l) * prev;
                 prev = cur;
@@ -557,14 +597,15 @@ __global__ void k_pack_naive(const double* __restrict__ w, const GemmMeta* __res
 }
 
 // Forward: coefficient[c][i] = sum_j fold_p(x)[c][j] * NF(m,p)[j][i].
-// grid (bw, ceil(cols/ncb)); LDS Sd[c][p][j] (swizzled j), 2*bw floats per column.
+// grid (bw, ceil(cols/ncb)); LDS Sd[c][p][j] (swizzled j), 2*bw floats per
+// column.  A wave owns 16 columns x 64 degrees (four MFMA tiles).
 __global__ __launch_bounds__(kThreads) void k_ngemm_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
                                                         const GemmMeta* __restrict__ meta, double* __restrict__ cre,
                                                         double* __restrict__ cim, int lgB, int F, int ncb, int fmt) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* Sd = reinterpret_cast<float*>(smem);
     const int B = 1 << lgB, N = 2 * B;
-    const int m = blockIdx.x;
+    const int m = blockIdx.x;  // orders fastest: concurrent blocks stream the same fields' rows
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
     const int cbase = blockIdx.y * ncb;
@@ -581,38 +622,56 @@ __global__ __launch_bounds__(kThreads) void k_ngemm_fwd(const float2* __restrict
             a = row[j];
             b = row[N - 1 - j];
         }
-        const int s0 = swz(c, B), s1 = swz(c + 1, B);
+        const int j0 = j ^ swz(c), j1 = j ^ swz(c + 1);
         float* d0 = Sd + (size_t)c * 2 * B;
         float* d1 = d0 + 2 * B;
-        d0[j ^ s0] = a.x + b.x;
-        d1[j ^ s1] = a.y + b.y;
-        d0[B + (j ^ s0)] = a.x - b.x;
-        d1[B + (j ^ s1)] = a.y - b.y;
+        d0[j0] = a.x + b.x;
+        d1[j1] = a.y + b.y;
+        d0[B + j0] = a.x - b.x;
+        d1[B + j1] = a.y - b.y;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nct = ncb >> 4;
     const size_t B2 = (size_t)B * B;
     const double sg = (m & 1) ? -1.0 : 1.0;
-    for (int p = 0; p < 2; ++p) {
-        const GemmMeta g = meta[m * 2 + p];
-        if (g.Lp == 0) continue;
-        const int L16 = (g.Lp + 15) & ~15;
-        const int ntiles = (L16 >> 4) * nct;
-        for (int item = wave; item < ntiles; item += kThreads / 64) {
-            const int it = item / nct, ct = item - it * nct;
-            const int i0 = it * 16, c0 = ct * 16;
-            const int c = c0 + (lane & 15);
-            const float* arow = Sd + (size_t)c * 2 * B + p * B;
-            const int sc = swz(c, B);
-            const float* brow = NF + g.offF + i0 + (lane & 15);
-            fvec4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-            for (int j0 = 0; j0 < B; j0 += 8) {
-                const int ja = j0 + (lane >> 4), jb = ja + 4;
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[ja ^ sc], brow[(size_t)ja * L16], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[jb ^ sc], brow[(size_t)jb * L16], acc1, 0, 0, 0);
-            }
-            const int i = i0 + (lane & 15);
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int ng0 = (g0.Lp + 63) >> 6, ng1 = (g1.Lp + 63) >> 6;
+    for (int item = wave; item < (ng0 + ng1) * nct; item += kThreads / 64) {
+        const int gi = item / nct, ct = item - gi * nct;
+        const int p = gi >= ng0;
+        const GemmMeta g = p ? g1 : g0;
+        const int ig = p ? gi - ng0 : gi;
+        const int L64 = ((p ? g1.Lp : g0.Lp) + 63) & ~63;
+        const int c0 = ct * 16, c = c0 + (lane & 15);
+        const float* arow = Sd + (size_t)c * 2 * B + p * B;
+        const int sc = swz(c);
+        const float* brow = NF + g.offF + ig * 64 + 4 * (lane & 15);
+        fvec4 a0[4] = {}, a1[4] = {};
+        // table fragments are fetched one step ahead (the last fetch repeats)
+        const int kq = lane >> 4;
+        float4 ta = *reinterpret_cast<const float4*>(brow + (size_t)kq * L64);
+        float4 tb = *reinterpret_cast<const float4*>(brow + (size_t)(kq + 4) * L64);
+        for (int j0 = 0; j0 < B; j0 += 8) {
+            const int ja = j0 + kq, jb = ja + 4;
+            const int jn = min(j0 + 8, B - 8) + kq;
+            const float4 na = *reinterpret_cast<const float4*>(brow + (size_t)jn * L64);
+            const float4 nb = *reinterpret_cast<const float4*>(brow + (size_t)(jn + 4) * L64);
+            const float xa = arow[ja ^ sc], xb = arow[jb ^ sc];
+            a0[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.x, a0[0], 0, 0, 0);
+            a0[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.y, a0[1], 0, 0, 0);
+            a0[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.z, a0[2], 0, 0, 0);
+            a0[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.w, a0[3], 0, 0, 0);
+            a1[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.x, a1[0], 0, 0, 0);
+            a1[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.y, a1[1], 0, 0, 0);
+            a1[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.z, a1[2], 0, 0, 0);
+            a1[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.w, a1[3], 0, 0, 0);
+            ta = na;
+            tb = nb;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = ig * 64 + 16 * q + (lane & 15);
             if (i >= g.Lp) continue;
             const int l = m + p + 2 * i;
 #pragma unroll
@@ -622,7 +681,7 @@ __global__ __launch_bounds__(kThreads) void k_ngemm_fwd(const float2* __restrict
                 const int f = col / cpf, rem = col - f * cpf;
                 const int comp = rem & 1;
                 double* out = comp ? cim : cre;
-                const double v = (double)acc0[r] + (double)acc1[r];
+                const double v = (double)a0[q][r] + (double)a1[q][r];
                 if ((rem >> 1) == 0) {
                     out[f * B2 + index_of_coeff(m, l, B)] = v;
                     if (fmt == 1 && m > 0) out[f * B2 + index_of_coeff(-m, l, B)] = comp ? -sg * v : sg * v;
@@ -637,6 +696,46 @@ __global__ __launch_bounds__(kThreads) void k_ngemm_fwd(const float2* __restrict
 // Inverse: E/O[c][j] = sum_i coef_p[c][i] * NI(m,p)[i][j]; ring j gets E+O,
 // ring N-1-j gets E-O; x 1/sqrt(2pi), (-1) on odd negative rows, REAL mirror.
 // grid (bw, ceil(cols/ncb)); LDS Cd[c][q] (swizzled), q = i or roundup4(Lp0)+i.
+// A wave owns 16 columns x 64 rings (four MFMA tiles per parity).
+// acol: this lane's LDS column; rows qoff + i (swizzled with sc) for i < K4.
+__device__ __forceinline__ void ngemm_inv_k(const float* __restrict__ acol, int qoff, int sc,
+                                            const float* __restrict__ brow, int B, int K4, fvec4* acc) {
+    fvec4 a1[4] = {};
+    int i0 = 0;
+    const int kq = (threadIdx.x & 63) >> 4;
+    // table fragments one step ahead (clamped: the last fetch repeats)
+    // rows are clamped to the last step (K4 - 4 + kq): the tail step, when
+    // there is one, finds its fragment in ta; other clamped fetches go unused
+    float4 ta = *reinterpret_cast<const float4*>(brow + (size_t)kq * B);
+    float4 tb = *reinterpret_cast<const float4*>(brow + (size_t)(min(4, K4 - 4) + kq) * B);
+    for (; i0 + 8 <= K4; i0 += 8) {
+        const int ia = i0 + kq, ib = ia + 4;
+        const float4 na = *reinterpret_cast<const float4*>(brow + (size_t)(min(i0 + 8, K4 - 4) + kq) * B);
+        const float4 nb = *reinterpret_cast<const float4*>(brow + (size_t)(min(i0 + 12, K4 - 4) + kq) * B);
+        const float xa = acol[(qoff + ia) ^ sc], xb = acol[(qoff + ib) ^ sc];
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.x, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.y, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.z, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.w, acc[3], 0, 0, 0);
+        a1[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.x, a1[0], 0, 0, 0);
+        a1[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.y, a1[1], 0, 0, 0);
+        a1[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.z, a1[2], 0, 0, 0);
+        a1[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(xb, tb.w, a1[3], 0, 0, 0);
+        ta = na;
+        tb = nb;
+    }
+    if (i0 < K4) {  // one step of 4 left; ta holds it
+        const int ia = i0 + kq;
+        const float xa = acol[(qoff + ia) ^ sc];
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.x, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.y, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.z, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, ta.w, acc[3], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += a1[q];
+}
+
 __global__ __launch_bounds__(kThreads) void k_ngemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
                                                         const float* __restrict__ NI,
                                                         const GemmMeta* __restrict__ meta, float2* __restrict__ X,
@@ -644,7 +743,7 @@ __global__ __launch_bounds__(kThreads) void k_ngemm_inv(const double* __restrict
     extern __shared__ __align__(16) unsigned char smem[];
     float* Cd = reinterpret_cast<float*>(smem);
     const int B = 1 << lgB, N = 2 * B;
-    const int m = blockIdx.x;
+    const int m = blockIdx.x;  // orders fastest: concurrent blocks stream the same fields' rows
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
     const int cbase = blockIdx.y * ncb;
@@ -663,71 +762,45 @@ __global__ __launch_bounds__(kThreads) void k_ngemm_inv(const double* __restrict
             const double* src = (rem & 1) ? cim : cre;
             v = (float)src[f * B2 + index_of_coeff((rem >> 1) ? -m : m, m + li, B)];
         }
-        Cd[(size_t)c * B + (q ^ swz(c, B))] = v;
+        Cd[(size_t)c * B + (q ^ swz(c))] = v;
     }
     for (int idx = threadIdx.x; idx < ncb * 8; idx += kThreads) {
         const int c = idx >> 3, k = idx & 7;
         const int p = k >> 2, i = (p ? g1.Lp : g0.Lp) + (k & 3);
-        if (i < (p ? Li4_1 : Li4_0)) Cd[(size_t)c * B + (((p ? Li4_0 : 0) + i) ^ swz(c, B))] = 0.f;
+        if (i < (p ? Li4_1 : Li4_0)) Cd[(size_t)c * B + (((p ? Li4_0 : 0) + i) ^ swz(c))] = 0.f;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nct = ncb >> 4, ntj = B >> 4;
+    const int nct = ncb >> 4, ngj = B >> 6;
     const float inv_s2pi = 0.39894228040143267794f;
-    for (int item = wave; item < ntj * nct; item += kThreads / 64) {
-        const int jt = item / nct, ct = item - jt * nct;
-        const int j0 = jt * 16, c0 = ct * 16;
-        const int c = c0 + (lane & 15);
-        const int sc = swz(c, B);
+    for (int item = wave; item < ngj * nct; item += kThreads / 64) {
+        const int jg = item / nct, ct = item - jg * nct;
+        const int c0 = ct * 16, c = c0 + (lane & 15);
+        const int sc = swz(c);
         const float* acol = Cd + (size_t)c * B;
-        fvec4 e = {0, 0, 0, 0}, o = {0, 0, 0, 0};
-        {
-            const float* brow = NI + g0.offI + j0 + (lane & 15);
-            fvec4 e1 = {0, 0, 0, 0};
-            int i0 = 0;
-            for (; i0 + 8 <= Li4_0; i0 += 8) {
-                const int ia = i0 + (lane >> 4), ib = ia + 4;
-                e = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[ia ^ sc], brow[(size_t)ia * B], e, 0, 0, 0);
-                e1 = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[ib ^ sc], brow[(size_t)ib * B], e1, 0, 0, 0);
-            }
-            if (i0 < Li4_0) {
-                const int ia = i0 + (lane >> 4);
-                e = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[ia ^ sc], brow[(size_t)ia * B], e, 0, 0, 0);
-            }
-            e += e1;
-        }
-        if (g1.Lp > 0) {
-            const float* brow = NI + g1.offI + j0 + (lane & 15);
-            fvec4 o1 = {0, 0, 0, 0};
-            int i0 = 0;
-            for (; i0 + 8 <= Li4_1; i0 += 8) {
-                const int ia = i0 + (lane >> 4), ib = ia + 4;
-                o = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[(Li4_0 + ia) ^ sc], brow[(size_t)ia * B], o, 0, 0, 0);
-                o1 = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[(Li4_0 + ib) ^ sc], brow[(size_t)ib * B], o1, 0, 0, 0);
-            }
-            if (i0 < Li4_1) {
-                const int ia = i0 + (lane >> 4);
-                o = __builtin_amdgcn_mfma_f32_16x16x4f32(acol[(Li4_0 + ia) ^ sc], brow[(size_t)ia * B], o, 0, 0, 0);
-            }
-            o += o1;
-        }
-        const int j = j0 + (lane & 15);
+        fvec4 e[4] = {}, o[4] = {};
+        ngemm_inv_k(acol, 0, sc, NI + g0.offI + jg * 64 + 4 * (lane & 15), B, Li4_0, e);
+        if (g1.Lp > 0) ngemm_inv_k(acol, Li4_0, sc, NI + g1.offI + jg * 64 + 4 * (lane & 15), B, Li4_1, o);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int col = cbase + c0 + Mma<float>::row(lane, r);
-            if (col >= ncols) continue;
-            const int f = col / cpf, rem = col - f * cpf;
-            const int row = (rem >> 1) ? N - m : m;
-            const float s = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
-            const float yn = (e[r] + o[r]) * s, ys = (e[r] - o[r]) * s;
-            float* xr = reinterpret_cast<float*>(X + ((size_t)f * N + row) * N) + (rem & 1);
-            xr[2 * j] = yn;
-            xr[2 * (N - 1 - j)] = ys;
-            if (fmt == 1 && m > 0) {
-                float* xm = reinterpret_cast<float*>(X + ((size_t)f * N + (N - m)) * N) + (rem & 1);
-                const float cs = (rem & 1) ? -1.f : 1.f;
-                xm[2 * j] = cs * yn;
-                xm[2 * (N - 1 - j)] = cs * ys;
+        for (int q = 0; q < 4; ++q) {
+            const int j = jg * 64 + 16 * q + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {  // rows r, r+1: the re, im columns of one (field, sign)
+                const int col = cbase + c0 + Mma<float>::row(lane, r);
+                if (col >= ncols) continue;
+                const int f = col / cpf, rem = col - f * cpf;
+                const int row = (rem >> 1) ? N - m : m;
+                const float s = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
+                const float2 yn = make_float2((e[q][r] + o[q][r]) * s, (e[q][r + 1] + o[q][r + 1]) * s);
+                const float2 ys = make_float2((e[q][r] - o[q][r]) * s, (e[q][r + 1] - o[q][r + 1]) * s);
+                float2* xr = X + ((size_t)f * N + row) * N;
+                xr[j] = yn;
+                xr[N - 1 - j] = ys;
+                if (fmt == 1 && m > 0) {
+                    float2* xm = X + ((size_t)f * N + (N - m)) * N;
+                    xm[j] = make_float2(yn.x, -yn.y);
+                    xm[N - 1 - j] = make_float2(ys.x, -ys.y);
+                }
             }
         }
     }
@@ -835,7 +908,7 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
         const int ncb = gemm_ncb(B, g.prec);
         const int cols = 4 * F;
         Span sp(tm, ST_LEG_FWD, s);
-        k_gemm_fwd<R><<<dim3(B, (cols + ncb - 1) / ncb), kThreads, gemm_lds(B, g.prec), s>>>(
+        k_gemm_fwd<R><<<dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s>>>(
             Cb, (const R*)g.af, g.meta, cre, cim, lg2(B), F, ncb, (int)fmt);
         S2_HIP(hipGetLastError());
     }
@@ -858,7 +931,7 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
         const int ncb = gemm_ncb(B, g.prec);
         const int cols = 4 * F;
         Span sp(tm, ST_LEG_INV, s);
-        k_gemm_inv<R><<<dim3(B, (cols + ncb - 1) / ncb), kThreads, gemm_lds(B, g.prec), s>>>(
+        k_gemm_inv<R><<<dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s>>>(
             cre, cim, (const R*)g.ai, g.meta, Cb, lg2(B), F, ncb, (int)fmt);
         S2_HIP(hipGetLastError());
     }
@@ -878,7 +951,7 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
 
 bool mixed_supported(int bw) {
     const int N = 2 * bw;
-    return bw >= 32 && bw <= 512 && !(N & (N - 1));
+    return bw >= 64 && bw <= 512 && !(N & (N - 1));
 }
 
 void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
@@ -894,7 +967,7 @@ void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
             e.Kf = B;
             e.offF = of;
             e.offI = oi;
-            of += (long)B * ((e.Lp + 15) & ~15);
+            of += (long)B * ((e.Lp + 63) & ~63);
             oi += (long)((e.Lp + 3) & ~3) * B;
         }
     g.nf = of;

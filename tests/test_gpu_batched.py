@@ -35,7 +35,7 @@ def to_dev(torch, dev, arrs):
 @pytest.mark.parametrize("bw,cutoff,fmt,prec", [(8, 8, 0, 0), (17, 17, 0, 0), (32, 32, 0, 0), (64, 64, 0, 0),
                                                 (64, 16, 0, 0), (64, 64, 1, 0), (128, 128, 0, 0), (512, 512, 0, 0),
                                                 (32, 32, 0, 1), (64, 64, 0, 1), (128, 128, 1, 1), (256, 256, 0, 1),
-                                                (32, 32, 0, 2), (64, 64, 0, 2), (128, 128, 1, 2), (256, 256, 0, 2),
+                                                (64, 64, 0, 2), (128, 128, 1, 2), (256, 256, 0, 2),
                                                 (256, 256, 1, 2)])
 def test_plan_matches_oracle(env, O, bw, cutoff, fmt, prec):
     S, torch, dev = env
