@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
     ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 1 GiB device copy (reference for rocprofv3 byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -118,6 +120,13 @@ def main():
     dev = torch.device("cuda:0")
     n = 2 * bw
     prec = {"fp64": S.FP64, "fp32": S.FP32, "mixed": S.MIXED}[args.precision]
+
+    if args.copy_calibration:  # reads 1 GiB and writes 1 GiB, nothing else
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---- workload: band-limited grids from power-law coefficients ---------
     c0r, c0i = synth.power_law_coeffs(bw, F, 1234, dev)

@@ -98,7 +98,7 @@ __device__ __forceinline__ void dftR(T* x) {
 // One Stockham pass of radix R over the E = V*kThreads values of the block
 // (E/N transforms of length N, row stride S).  Sub-transform size Ns = 2^lgNs.
 template <typename T, int SIGN, int V, int R, int LGR>
-__device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T* __restrict__ tw) {
+__device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T* tw) {
     constexpr int PER = V / R;
     const int lgNR = lgN - LGR;
     const int NR = 1 << lgNR;
@@ -119,11 +119,13 @@ __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T
         const int f = b >> lgNR, j = b & (NR - 1);
         const int k = j & (Ns - 1);
         const int step = k << (lgNR - lgNs);  // k * N / (R * Ns)
+        if (lgNs > 0) {  // the first pass has k = 0: unit twiddles
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-            T w = tw[r * step];
-            if (SIGN > 0) w.y = -w.y;
-            x[u * R + r] = cmul(x[u * R + r], w);
+            for (int r = 1; r < R; ++r) {
+                T w = tw[r * step];
+                if (SIGN > 0) w.y = -w.y;
+                x[u * R + r] = cmul(x[u * R + r], w);
+            }
         }
         dftR<R, SIGN>(x + u * R);
         T* dst = a + f * S + ((j - k) << LGR) + k;
@@ -135,7 +137,7 @@ __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T
 
 // In-place FFTs of all E/N rows of the block's LDS tile (natural order in and out).
 template <typename T, int SIGN, int V>
-__device__ void fft_block(T* a, int lgN, int S, const T* __restrict__ tw) {
+__device__ __forceinline__ void fft_block(T* a, int lgN, int S, const T* tw) {
     int lgNs = 0, rem = lgN;
     while (rem >= 3) {
         fft_pass<T, SIGN, V, 8, 3>(a, lgN, S, lgNs, tw);
@@ -180,6 +182,15 @@ __device__ __forceinline__ void xcd_block_m(int& bx, int& m) {
 // position of cosine index k in the parity-split planes of the Cb buffer
 __device__ __forceinline__ int cpos(int k, int half) { return (k & 1) ? half + (k >> 1) : (k >> 1); }
 
+// Copy the N twiddles exp(-2 pi i t/N) into LDS (they are read by every
+// butterfly of every pass).
+template <typename T>
+__device__ __forceinline__ T* stage_twiddles(unsigned char* smem, const T* __restrict__ tw, int N) {
+    T* t = reinterpret_cast<T*>(smem);
+    for (int i = threadIdx.x; i < N; i += kThreads) t[i] = tw[i];
+    return t;
+}
+
 // ---------------------------------------------------------------------------
 // FFT-type kernels.  grid.y = field, block = kThreads, LDS = rows*(N+1) values
 // ---------------------------------------------------------------------------
@@ -189,21 +200,42 @@ __global__ __launch_bounds__(kThreads) void k_fphi_fwd(const double* __restrict_
                                                        const typename Cplx<R>::type* __restrict__ tw) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    T* a = reinterpret_cast<T*>(smem);
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1;
+    const T* twl = stage_twiddles(smem, tw, N);
+    T* a = reinterpret_cast<T*>(smem) + N;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
     xcd_block(bx, f);
     const int j0 = bx * rpb;
     const size_t base = ((size_t)f * N + j0) * N;
-    for (int idx = threadIdx.x; idx < E; idx += kThreads)
-        a[(idx >> lgN) * S + (idx & (N - 1))] = mk<T>((R)re[base + idx], (R)im[base + idx]);
+    {
+        double vr[V], vi[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            vr[u] = re[base + threadIdx.x + u * kThreads];
+            vi[u] = im[base + threadIdx.x + u * kThreads];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            a[(idx >> lgN) * S + (idx & (N - 1))] = mk<T>((R)vr[u], (R)vi[u]);
+        }
+    }
     __syncthreads();
-    fft_block<T, -1, V>(a, lgN, S, tw);
-    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
-        const int r = idx >> lgrpb, jl = idx & (rpb - 1);
-        X[((size_t)f * N + r) * N + j0 + jl] = a[jl * S + r];
+    fft_block<T, -1, V>(a, lgN, S, twl);
+    {
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            v[u] = a[(idx & (rpb - 1)) * S + (idx >> lgrpb)];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            X[((size_t)f * N + (idx >> lgrpb)) * N + j0 + (idx & (rpb - 1))] = v[u];
+        }
     }
 }
 
@@ -214,7 +246,8 @@ __global__ __launch_bounds__(kThreads) void k_fdct_fwd(const typename Cplx<R>::t
                                                        const typename Cplx<R>::type* __restrict__ ph) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    T* a = reinterpret_cast<T*>(smem);
+    const T* twl = stage_twiddles(smem, tw, 1 << lgN);
+    T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
@@ -234,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void k_fdct_fwd(const typename Cplx<R>::t
         a[rl * S + p] = x;
     }
     __syncthreads();
-    fft_block<T, -1, V>(a, lgN, S, tw);
+    fft_block<T, -1, V>(a, lgN, S, twl);
     for (int idx = threadIdx.x; idx < rpb * B; idx += kThreads) {
         const int rl = idx >> (lgN - 1), k = idx & (B - 1);
         const int q = q0 + rl;
@@ -258,7 +291,8 @@ __global__ __launch_bounds__(kThreads) void k_fdct_inv(const R* __restrict__ Cb,
                                                        const typename Cplx<R>::type* __restrict__ ph) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    T* a = reinterpret_cast<T*>(smem);
+    const T* twl = stage_twiddles(smem, tw, 1 << lgN);
+    T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
@@ -282,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void k_fdct_inv(const R* __restrict__ Cb,
         a[rl * S + k] = v;
     }
     __syncthreads();
-    fft_block<T, +1, V>(a, lgN, S, tw);
+    fft_block<T, +1, V>(a, lgN, S, twl);
     const R inv_s2pi = (R)0.39894228040143267794;
     for (int idx = threadIdx.x; idx < E; idx += kThreads) {
         const int rl = idx >> lgN, j = idx & (N - 1);
@@ -306,24 +340,43 @@ __global__ __launch_bounds__(kThreads) void k_fphi_inv(const typename Cplx<R>::t
                                                        const typename Cplx<R>::type* __restrict__ tw) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    T* a = reinterpret_cast<T*>(smem);
     constexpr int E = V * kThreads;
     const int N = 1 << lgN, S = N + 1, B = N >> 1;
+    const T* twl = stage_twiddles(smem, tw, N);
+    T* a = reinterpret_cast<T*>(smem) + N;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
     xcd_block(bx, f);
     const int j0 = bx * rpb;
-    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
-        const int r = idx >> lgrpb, jl = idx & (rpb - 1);
-        a[jl * S + r] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + jl];
+    {
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            const int r = idx >> lgrpb;
+            v[u] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + (idx & (rpb - 1))];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            a[(idx & (rpb - 1)) * S + (idx >> lgrpb)] = v[u];
+        }
     }
     __syncthreads();
-    fft_block<T, +1, V>(a, lgN, S, tw);
+    fft_block<T, +1, V>(a, lgN, S, twl);
     const size_t base = ((size_t)f * N + j0) * N;
-    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
-        const T v = a[(idx >> lgN) * S + (idx & (N - 1))];
-        re[base + idx] = (double)v.x;
-        im[base + idx] = (double)v.y;
+    {
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            v[u] = a[(idx >> lgN) * S + (idx & (N - 1))];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            re[base + threadIdx.x + u * kThreads] = (double)v[u].x;
+            im[base + threadIdx.x + u * kThreads] = (double)v[u].y;
+        }
     }
 }
 
@@ -693,6 +746,430 @@ __global__ __launch_bounds__(kThreads) void k_ngemm_fwd(const float2* __restrict
     }
 }
 
+// Forward, streamed form: a block owns TM = 128 columns (32 fields x +-m x
+// re/im) of one order and all its degrees, so each order's table is read
+// 4F/128 times instead of once per 32 columns.  The folded data goes through
+// LDS in double-buffered chunks of KC = 32 rings: the next chunk's global
+// loads are in flight while the current chunk feeds the MFMAs.  8 waves, each
+// owning 16 columns x every 64-degree group of both parities (G groups per
+// parity at most).  LDS chunk layout [p][c][k], k swizzled by ((c>>1)&7)<<2 so
+// that a wave's 16 columns x 4 rings hit 64 distinct banks.
+constexpr int kNgThreads = 512;
+constexpr int kNgTM = 128;
+constexpr int kNgKC = 32;
+
+__device__ __forceinline__ int swz32(int c) { return ((c >> 1) & 7) << 2; }
+
+template <int G>
+__global__ __launch_bounds__(kNgThreads) void k_ngemm_fwd2(const float2* __restrict__ X, const float* __restrict__ NF,
+                                                           const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                                           double* __restrict__ cim, int lgB, int F, int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sm = reinterpret_cast<float*>(smem);  // [2 buffers][2 parities][TM][KC]
+    constexpr int TM = kNgTM, KC = kNgKC, BUF = 2 * TM * KC;
+    const int B = 1 << lgB, N = 2 * B;
+    const int m = blockIdx.x;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    const int cbase = blockIdx.y * TM;
+    if (cbase >= ncols) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int ng[2] = {(g0.Lp + 63) >> 6, (g1.Lp + 63) >> 6};
+    const int L64[2] = {ng[0] * 64, ng[1] * 64};
+
+    // fill role of this thread: (field, sign) pairs fp and fp+32 (columns
+    // 2fp, 2fp+1 and 2fp+64, 2fp+65), rings fk and fk+16 of each chunk
+    const int fk = tid & 15, fp = tid >> 4;  // 32 x 16 threads
+    const float2* frow[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int col = cbase + 2 * (fp + 32 * e);
+        frow[e] = nullptr;
+        if (col < ncols) {
+            const int f = col / cpf, rem = col - f * cpf;
+            frow[e] = X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N;
+        }
+    }
+    float2 ra[2][2], rb[2][2];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int j = kc + fk + 16 * h;
+                ra[e][h] = frow[e] ? frow[e][j] : make_float2(0.f, 0.f);
+                rb[e][h] = frow[e] ? frow[e][N - 1 - j] : make_float2(0.f, 0.f);
+            }
+    };
+    auto stash = [&](float* buf) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int c = 2 * (fp + 32 * e);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = fk + 16 * h;
+                const int k0 = k ^ swz32(c), k1 = k ^ swz32(c + 1);
+                buf[(0 * TM + c) * KC + k0] = ra[e][h].x + rb[e][h].x;
+                buf[(0 * TM + c + 1) * KC + k1] = ra[e][h].y + rb[e][h].y;
+                buf[(1 * TM + c) * KC + k0] = ra[e][h].x - rb[e][h].x;
+                buf[(1 * TM + c + 1) * KC + k1] = ra[e][h].y - rb[e][h].y;
+            }
+        }
+    };
+
+    const int c = wave * 16 + (lane & 15);  // this lane's column in MFMA operand A
+    const int sc = swz32(c), kq = lane >> 4;
+    fvec4 acc[2][G][4] = {};
+    fetch(0);
+    stash(sm);
+    __syncthreads();
+    for (int kc = 0; kc < B; kc += KC) {
+        const float* cur = sm + ((kc / KC) & 1) * BUF;
+        const bool more = kc + KC < B;
+        if (more) fetch(kc + KC);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const GemmMeta g = p ? g1 : g0;
+            const float* arow = cur + (p * TM + c) * KC;
+            const float* brow = NF + g.offF + 4 * (lane & 15);
+#pragma unroll
+            for (int kk = 0; kk < KC; kk += 4) {
+                const float a = arow[(kk + kq) ^ sc];
+                const size_t jrow = (size_t)(kc + kk + kq) * L64[p];
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    if (q < ng[p]) {
+                        const float4 t = *reinterpret_cast<const float4*>(brow + jrow + q * 64);
+                        acc[p][q][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.x, acc[p][q][0], 0, 0, 0);
+                        acc[p][q][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.y, acc[p][q][1], 0, 0, 0);
+                        acc[p][q][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.z, acc[p][q][2], 0, 0, 0);
+                        acc[p][q][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.w, acc[p][q][3], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (more) stash(sm + (((kc / KC) + 1) & 1) * BUF);
+        __syncthreads();
+    }
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const int Lp = p ? g1.Lp : g0.Lp;
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = q * 64 + 16 * u + (lane & 15);
+                if (q >= ng[p] || i >= Lp) continue;
+                const int l = m + p + 2 * i;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = cbase + wave * 16 + Mma<float>::row(lane, r);
+                    if (col >= ncols) continue;
+                    const int f = col / cpf, rem = col - f * cpf;
+                    const int comp = rem & 1;
+                    double* out = comp ? cim : cre;
+                    const double v = (double)acc[p][q][u][r];
+                    if ((rem >> 1) == 0) {
+                        out[f * B2 + index_of_coeff(m, l, B)] = v;
+                        if (fmt == 1 && m > 0) out[f * B2 + index_of_coeff(-m, l, B)] = comp ? -sg * v : sg * v;
+                    } else {
+                        out[f * B2 + index_of_coeff(-m, l, B)] = sg * v;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Forward, pipelined form.  A block owns TM = 128 columns of one order m and
+// all its degrees.  Both operands go through LDS in double-buffered chunks of
+// KC = 16 rings: the folded data A[p][c][k] and the table rows T[k][p][G*64].
+// The next chunk's global loads are issued before the current chunk's MFMAs
+// and stored after them, so the MFMA loop only waits on LDS.  G (64-degree
+// groups per parity) is a template constant: orders are launched in buckets
+// of equal G, so the loop has no branches (a parity with one group fewer
+// reads a zero-filled group).  8 waves, each 16 columns x 2 x G x 4 tiles.
+constexpr int kF3Threads = 512;
+constexpr int kF3TM = 128;
+constexpr int kF3KC = 16;
+
+__device__ __forceinline__ int swz16(int c) { return ((c >> 2) & 3) << 2; }
+
+template <int G>
+__global__ __launch_bounds__(kF3Threads) void k_ngemm_fwd3(const float2* __restrict__ X, const float* __restrict__ NF,
+                                                           const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                                           double* __restrict__ cim, int lgB, int F, int fmt, int m0) {
+    constexpr int TM = kF3TM, KC = kF3KC, GW = G * 64;
+    constexpr int ABUF = 2 * TM * KC;  // floats of folded data per buffer
+    constexpr int TBUF = KC * 2 * GW;  // floats of table per buffer
+    constexpr int RW = 2 * GW / 4;     // float4 per table chunk row (both parities)
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sm = reinterpret_cast<float*>(smem);
+    const int B = 1 << lgB, N = 2 * B;
+    const int m = m0 + blockIdx.x;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    const int cbase = blockIdx.y * TM;
+    if (cbase >= ncols) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const GemmMeta gm[2] = {meta[m * 2], meta[m * 2 + 1]};
+    const int L64[2] = {(gm[0].Lp + 63) & ~63, (gm[1].Lp + 63) & ~63};
+
+    // data fill: pair ap (columns 2ap, 2ap+1), rings k0 and k0+8 of a chunk
+    const int ap = tid >> 3, k0 = tid & 7;
+    const float2* xrow = nullptr;
+    {
+        const int col = cbase + 2 * ap;
+        if (col < ncols) {
+            const int f = col / cpf, rem = col - f * cpf;
+            xrow = X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N;
+        }
+    }
+    float2 xa[2], xb[2];
+    float4 tv[G];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int j = kc + k0 + 8 * h;
+            xa[h] = xrow ? xrow[j] : make_float2(0.f, 0.f);
+            xb[h] = xrow ? xrow[N - 1 - j] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int e = tid + kF3Threads * u;
+            const int k = e / RW, rest = e - k * RW;
+            const int p = rest / (GW / 4), i4 = rest - p * (GW / 4);
+            tv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (4 * i4 < L64[p])
+                tv[u] = *reinterpret_cast<const float4*>(NF + gm[p].offF + (size_t)(kc + k) * L64[p] + 4 * i4);
+        }
+    };
+    auto stash = [&](float* buf) {
+        const int c = 2 * ap;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = k0 + 8 * h;
+            const int ka = k ^ swz16(c), kb = k ^ swz16(c + 1);
+            buf[(0 * TM + c) * KC + ka] = xa[h].x + xb[h].x;
+            buf[(0 * TM + c + 1) * KC + kb] = xa[h].y + xb[h].y;
+            buf[(1 * TM + c) * KC + ka] = xa[h].x - xb[h].x;
+            buf[(1 * TM + c + 1) * KC + kb] = xa[h].y - xb[h].y;
+        }
+        float* tb = buf + ABUF;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int e = tid + kF3Threads * u;
+            const int k = e / RW, rest = e - k * RW;
+            const int p = rest / (GW / 4), i4 = rest - p * (GW / 4);
+            *reinterpret_cast<float4*>(tb + (k * 2 + p) * GW + 4 * i4) = tv[u];
+        }
+    };
+
+    const int c = wave * 16 + (lane & 15);
+    const int sc = swz16(c), kq = lane >> 4;
+    fvec4 acc[2][G][4] = {};
+    fetch(0);
+    stash(sm);
+    __syncthreads();
+    for (int kc = 0, bi = 0; kc < B; kc += KC, bi ^= 1) {
+        const float* cur = sm + bi * (ABUF + TBUF);
+        const bool more = kc + KC < B;
+        if (more) fetch(kc + KC);
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const float a = cur[(p * TM + c) * KC + ((kk + kq) ^ sc)];
+                const float* trow = cur + ABUF + ((kk + kq) * 2 + p) * GW + 4 * (lane & 15);
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    const float4 t = *reinterpret_cast<const float4*>(trow + q * 64);
+                    acc[p][q][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.x, acc[p][q][0], 0, 0, 0);
+                    acc[p][q][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.y, acc[p][q][1], 0, 0, 0);
+                    acc[p][q][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.z, acc[p][q][2], 0, 0, 0);
+                    acc[p][q][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.w, acc[p][q][3], 0, 0, 0);
+                }
+            }
+        }
+        if (more) stash(sm + (bi ^ 1) * (ABUF + TBUF));
+        __syncthreads();
+    }
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = q * 64 + 16 * u + (lane & 15);
+                if (i >= gm[p].Lp) continue;
+                const int l = m + p + 2 * i;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = cbase + wave * 16 + Mma<float>::row(lane, r);
+                    if (col >= ncols) continue;
+                    const int f = col / cpf, rem = col - f * cpf;
+                    const int comp = rem & 1;
+                    double* out = comp ? cim : cre;
+                    const double v = (double)acc[p][q][u][r];
+                    if ((rem >> 1) == 0) {
+                        out[f * B2 + index_of_coeff(m, l, B)] = v;
+                        if (fmt == 1 && m > 0) out[f * B2 + index_of_coeff(-m, l, B)] = comp ? -sg * v : sg * v;
+                    } else {
+                        out[f * B2 + index_of_coeff(-m, l, B)] = sg * v;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Inverse, pipelined form.  A block owns TM = 128 columns of one order and a
+// range of GJ*64 rings (blockIdx.z selects the range).  Chunks of 32
+// consecutive degrees (16 per parity) stream through double-buffered LDS:
+// coefficients A[c][p][i] and table rows T[p][i][GJ*64].  8 waves, each 16
+// columns x GJ x 4 ring tiles x (even, odd) accumulators.
+constexpr int kI3KC = 16;  // degrees per parity per chunk
+
+template <int GJ>
+__global__ __launch_bounds__(kF3Threads) void k_ngemm_inv3(const double* __restrict__ cre,
+                                                           const double* __restrict__ cim,
+                                                           const float* __restrict__ NI,
+                                                           const GemmMeta* __restrict__ meta,
+                                                           float2* __restrict__ X, int lgB, int F, int fmt) {
+    constexpr int TM = kF3TM, KC = kI3KC, JW = GJ * 64;
+    constexpr int ABUF = TM * 2 * KC;  // floats of coefficients per buffer
+    constexpr int TBUF = 2 * KC * JW;  // floats of table per buffer
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sm = reinterpret_cast<float*>(smem);
+    const int B = 1 << lgB, N = 2 * B;
+    const int m = blockIdx.x;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    const int cbase = blockIdx.y * TM;
+    if (cbase >= ncols) return;
+    const int jbase = blockIdx.z * JW;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const GemmMeta gm[2] = {meta[m * 2], meta[m * 2 + 1]};
+    const int Li4[2] = {(gm[0].Lp + 3) & ~3, (gm[1].Lp + 3) & ~3};
+    const int L = B - m;
+    const size_t B2 = (size_t)B * B;
+
+    // coefficient fill: pair ap (columns 2ap re, 2ap+1 im), degrees l0+d, d = dk + 8h
+    const int ap = tid >> 3, dk = tid & 7;
+    const double* cr = nullptr;
+    const double* ci = nullptr;
+    {
+        const int col = cbase + 2 * ap;
+        if (col < ncols) {
+            const int f = col / cpf, rem = col - f * cpf;
+            const size_t base = f * B2 + index_of_coeff((rem >> 1) ? -m : m, m, B);
+            cr = cre + base;
+            ci = cim + base;
+        }
+    }
+    double vr[4], vi[4];
+    float4 tv[GJ];
+    constexpr int RW = JW / 4;  // float4 per table row
+    auto fetch = [&](int d0) {  // degrees m + d0 .. m + d0 + 31
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int d = d0 + dk + 8 * h;
+            const bool ok = cr && d < L;
+            vr[h] = ok ? cr[d] : 0.0;
+            vi[h] = ok ? ci[d] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < GJ; ++u) {
+            const int e = tid + kF3Threads * u;  // < 2 * KC * RW
+            const int row = e / RW, j4 = e - row * RW;
+            const int p = row / KC, i = d0 / 2 + (row - p * KC);
+            tv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < Li4[p])
+                tv[u] = *reinterpret_cast<const float4*>(NI + gm[p].offI + (size_t)i * B + jbase + 4 * j4);
+        }
+    };
+    auto stash = [&](float* buf) {
+        const int c = 2 * ap;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int d = dk + 8 * h;  // within the chunk: parity d&1, index d>>1
+            const int p = d & 1, i = d >> 1;
+            buf[(c * 2 + p) * KC + (i ^ swz16(c))] = (float)vr[h];
+            buf[((c + 1) * 2 + p) * KC + (i ^ swz16(c + 1))] = (float)vi[h];
+        }
+        float* tb = buf + ABUF;
+#pragma unroll
+        for (int u = 0; u < GJ; ++u) {
+            const int e = tid + kF3Threads * u;
+            const int row = e / RW, j4 = e - row * RW;
+            *reinterpret_cast<float4*>(tb + row * JW + 4 * j4) = tv[u];
+        }
+    };
+
+    const int c = wave * 16 + (lane & 15);
+    const int sc = swz16(c), kq = lane >> 4;
+    fvec4 acc[2][GJ][4] = {};
+    fetch(0);
+    stash(sm);
+    __syncthreads();
+    for (int d0 = 0, bi = 0; d0 < L; d0 += 2 * KC, bi ^= 1) {
+        const float* cur = sm + bi * (ABUF + TBUF);
+        const bool more = d0 + 2 * KC < L;
+        if (more) fetch(d0 + 2 * KC);
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 4) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const float a = cur[(c * 2 + p) * KC + ((kk + kq) ^ sc)];
+                const float* trow = cur + ABUF + (p * KC + kk + kq) * JW + 4 * (lane & 15);
+#pragma unroll
+                for (int q = 0; q < GJ; ++q) {
+                    const float4 t = *reinterpret_cast<const float4*>(trow + q * 64);
+                    acc[p][q][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.x, acc[p][q][0], 0, 0, 0);
+                    acc[p][q][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.y, acc[p][q][1], 0, 0, 0);
+                    acc[p][q][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.z, acc[p][q][2], 0, 0, 0);
+                    acc[p][q][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, t.w, acc[p][q][3], 0, 0, 0);
+                }
+            }
+        }
+        if (more) stash(sm + (bi ^ 1) * (ABUF + TBUF));
+        __syncthreads();
+    }
+    const float inv_s2pi = 0.39894228040143267794f;
+#pragma unroll
+    for (int q = 0; q < GJ; ++q) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = jbase + q * 64 + 16 * u + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {  // rows r, r+1: re, im of one (field, sign)
+                const int col = cbase + wave * 16 + Mma<float>::row(lane, r);
+                if (col >= ncols) continue;
+                const int f = col / cpf, rem = col - f * cpf;
+                const int row = (rem >> 1) ? N - m : m;
+                const float s = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
+                const fvec4& e = acc[0][q][u];
+                const fvec4& o = acc[1][q][u];
+                const float2 yn = make_float2((e[r] + o[r]) * s, (e[r + 1] + o[r + 1]) * s);
+                const float2 ys = make_float2((e[r] - o[r]) * s, (e[r + 1] - o[r + 1]) * s);
+                float2* xr = X + ((size_t)f * N + row) * N;
+                xr[j] = yn;
+                xr[N - 1 - j] = ys;
+                if (fmt == 1 && m > 0) {
+                    float2* xm = X + ((size_t)f * N + (N - m)) * N;
+                    xm[j] = make_float2(yn.x, -yn.y);
+                    xm[N - 1 - j] = make_float2(ys.x, -ys.y);
+                }
+            }
+        }
+    }
+}
+
 // Inverse: E/O[c][j] = sum_i coef_p[c][i] * NI(m,p)[i][j]; ring j gets E+O,
 // ring N-1-j gets E-O; x 1/sqrt(2pi), (-1) on odd negative rows, REAL mirror.
 // grid (bw, ceil(cols/ncb)); LDS Cd[c][q] (swizzled), q = i or roundup4(Lp0)+i.
@@ -880,6 +1357,15 @@ bool fast_supported(int bw, int cutoff, int prec) {
     return gemm_ncb(bw, prec) >= 16;  // GEMM LDS tile of at least 16 columns
 }
 
+// Launch with `lds` bytes of dynamic LDS; above 64 KiB the kernel must opt
+// in (gfx950 has 160 KiB per workgroup).
+template <typename K, typename... A>
+static void launch_lds(K kernel, dim3 grid, int threads, size_t lds, hipStream_t s, A... args) {
+    if (lds > 65536) S2_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    kernel<<<grid, threads, lds, s>>>(args...);
+    S2_HIP(hipGetLastError());
+}
+
 template <typename R>
 static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
                      double* cim, int F, Fmt fmt, void* Xv, void* Cbv, hipStream_t s, StageTimer* tm) {
@@ -887,7 +1373,7 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
     constexpr int V = sizeof(R) == 8 ? 8 : 16;
     const int B = c.bw, N = c.N, lgN = lg2(N);
     const int E = V * kThreads, rpb = E / N;
-    const size_t lds = sizeof(T) * (size_t)rpb * (N + 1);
+    const size_t lds = sizeof(T) * ((size_t)rpb * (N + 1) + N);
     T* X = (T*)Xv;
     R* Cb = (R*)Cbv;
     const T* tw = (const T*)(sizeof(R) == 8 ? (const void*)c.tw : (const void*)c.tw32);
@@ -895,14 +1381,13 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
     const R* ws = (const R*)(sizeof(R) == 8 ? (const void*)c.w_semi : (const void*)c.w_semi32);
     {
         Span sp(tm, ST_PHI_FWD, s);
-        k_fphi_fwd<R, V><<<dim3(N / rpb, F), kThreads, lds, s>>>(re, im, X, lgN, tw);
-        S2_HIP(hipGetLastError());
+        launch_lds(k_fphi_fwd<R, V>, dim3(N / rpb, F), kThreads, lds, s, re, im, X, lgN, tw);
     }
     const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
     {
         Span sp(tm, ST_DCT_FWD, s);
-        k_fdct_fwd<R, V><<<dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s>>>(X, Cb, ws, lgN, B, nrows, tw, ph);
-        S2_HIP(hipGetLastError());
+        launch_lds(k_fdct_fwd<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const T*)X, Cb, ws, lgN, B,
+                   nrows, tw, ph);
     }
     {
         const int ncb = gemm_ncb(B, g.prec);
@@ -921,7 +1406,7 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     constexpr int V = sizeof(R) == 8 ? 8 : 16;
     const int B = c.bw, N = c.N, lgN = lg2(N);
     const int E = V * kThreads, rpb = E / N;
-    const size_t lds = sizeof(T) * (size_t)rpb * (N + 1);
+    const size_t lds = sizeof(T) * ((size_t)rpb * (N + 1) + N);
     T* X = (T*)Xv;
     R* Cb = (R*)Cbv;
     const T* tw = (const T*)(sizeof(R) == 8 ? (const void*)c.tw : (const void*)c.tw32);
@@ -938,14 +1423,12 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
     {
         Span sp(tm, ST_DCT_INV, s);
-        k_fdct_inv<R, V><<<dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s>>>(Cb, X, sv, lgN, B, nrows, (int)fmt,
-                                                                                 tw, ph);
-        S2_HIP(hipGetLastError());
+        launch_lds(k_fdct_inv<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const R*)Cb, X, sv, lgN, B,
+                   nrows, (int)fmt, tw, ph);
     }
     {
         Span sp(tm, ST_PHI_INV, s);
-        k_fphi_inv<R, V><<<dim3(N / rpb, F), kThreads, lds, s>>>(X, re, im, lgN, tw);
-        S2_HIP(hipGetLastError());
+        launch_lds(k_fphi_inv<R, V>, dim3(N / rpb, F), kThreads, lds, s, (const T*)X, re, im, lgN, tw);
     }
 }
 
@@ -996,16 +1479,27 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
     float2* X = (float2*)Xv;
     {
         Span sp(tm, ST_PHI_FWD, s);
-        k_fphi_fwd<float, V><<<dim3(N / rpb, F), kThreads, sizeof(float2) * (size_t)rpb * (N + 1), s>>>(
+        k_fphi_fwd<float, V><<<dim3(N / rpb, F), kThreads, sizeof(float2) * ((size_t)rpb * (N + 1) + N), s>>>(
             re, im, X, lgN, c.tw32);
         S2_HIP(hipGetLastError());
     }
     {
-        const int ncb = ngemm_ncb(B, true);
         Span sp(tm, ST_LEG_FWD, s);
-        k_ngemm_fwd<<<dim3(B, (4 * F + ncb - 1) / ncb), kThreads, sizeof(float) * 2 * B * ncb, s>>>(
-            X, (const float*)g.af, g.meta, cre, cim, lg2(B), F, ncb, (int)fmt);
-        S2_HIP(hipGetLastError());
+        // orders in buckets of equal 64-degree group count g: m in [B - 128g, B - 128(g-1))
+        const int gmax = (B / 2 + 63) / 64;
+        for (int gq = 1; gq <= gmax; ++gq) {
+            const int lo = std::max(0, B - 128 * gq), hi = std::min(B, B - 128 * (gq - 1));
+            if (lo >= hi) continue;
+            const dim3 grid(hi - lo, (4 * F + kF3TM - 1) / kF3TM);
+            const size_t lds = sizeof(float) * 2 * (2 * kF3TM * kF3KC + kF3KC * 2 * 64 * gq);
+            const float* nf = (const float*)g.af;
+            switch (gq) {
+                case 1: launch_lds(k_ngemm_fwd3<1>, grid, kF3Threads, lds, s, X, nf, g.meta, cre, cim, lg2(B), F, (int)fmt, lo); break;
+                case 2: launch_lds(k_ngemm_fwd3<2>, grid, kF3Threads, lds, s, X, nf, g.meta, cre, cim, lg2(B), F, (int)fmt, lo); break;
+                case 3: launch_lds(k_ngemm_fwd3<3>, grid, kF3Threads, lds, s, X, nf, g.meta, cre, cim, lg2(B), F, (int)fmt, lo); break;
+                default: launch_lds(k_ngemm_fwd3<4>, grid, kF3Threads, lds, s, X, nf, g.meta, cre, cim, lg2(B), F, (int)fmt, lo); break;
+            }
+        }
     }
 }
 
@@ -1016,15 +1510,20 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
     const int rpb = V * kThreads / N;
     float2* X = (float2*)Xv;
     {
-        const int ncb = ngemm_ncb(B, false);
         Span sp(tm, ST_LEG_INV, s);
-        k_ngemm_inv<<<dim3(B, (4 * F + ncb - 1) / ncb), kThreads, sizeof(float) * B * ncb, s>>>(
-            cre, cim, (const float*)g.ai, g.meta, X, lg2(B), F, ncb, (int)fmt);
-        S2_HIP(hipGetLastError());
+        const int gj = std::min(4, B / 64);
+        const dim3 grid(B, (4 * F + kF3TM - 1) / kF3TM, B / (64 * gj));
+        const size_t lds = sizeof(float) * 2 * (kF3TM * 2 * kI3KC + 2 * kI3KC * 64 * gj);
+        const float* ni = (const float*)g.ai;
+        switch (gj) {
+            case 1: launch_lds(k_ngemm_inv3<1>, grid, kF3Threads, lds, s, cre, cim, ni, g.meta, X, lg2(B), F, (int)fmt); break;
+            case 2: launch_lds(k_ngemm_inv3<2>, grid, kF3Threads, lds, s, cre, cim, ni, g.meta, X, lg2(B), F, (int)fmt); break;
+            default: launch_lds(k_ngemm_inv3<4>, grid, kF3Threads, lds, s, cre, cim, ni, g.meta, X, lg2(B), F, (int)fmt); break;
+        }
     }
     {
         Span sp(tm, ST_PHI_INV, s);
-        k_fphi_inv<float, V><<<dim3(N / rpb, F), kThreads, sizeof(float2) * (size_t)rpb * (N + 1), s>>>(
+        k_fphi_inv<float, V><<<dim3(N / rpb, F), kThreads, sizeof(float2) * ((size_t)rpb * (N + 1) + N), s>>>(
             X, re, im, lgN, c.tw32);
         S2_HIP(hipGetLastError());
     }
