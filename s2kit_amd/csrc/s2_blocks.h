@@ -1,0 +1,407 @@
+// Device building blocks shared by the batched FP64/FP32 path (s2_fast.hip)
+// and the mixed form (s2_mixed.hip): register DFTs, the LDS Stockham FFT,
+// XCD-aware block numbering, the four ring-FFT / DCT kernels, MFMA wrappers
+// and a launcher for kernels with more than 64 KiB of dynamic LDS.
+//
+//   k_fphi_fwd   ring FFT along phi, order-major store          (FST_semi_memo.c:81-90)
+//   k_fdct_fwd   weights + DCT-II per order row                 (seminaive.c:161-176)
+//   k_fdct_inv   DCT-III, x sin, row signs, 1/sqrt(2pi)         (seminaive.c:107-114; FST_semi_memo.c:301-348)
+//   k_fphi_inv   inverse ring FFT                               (FST_semi_memo.c:350)
+//
+// FFTs: in-place Stockham passes of radix 8 (then 4 or 2) with the values of
+// each butterfly in registers and one padded LDS buffer (row stride N+1);
+// the N twiddles are staged in LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "s2_complex.h"
+#include "s2_internal.h"
+
+namespace s2 {
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// small DFTs in registers (SIGN -1 forward, +1 inverse)
+// ---------------------------------------------------------------------------
+template <int SIGN, typename T>
+__device__ __forceinline__ T mul_si(T v) {  // v * (SIGN * i)
+    using R = typename Real<T>::type;
+    return mk<T>(-(R)SIGN * v.y, (R)SIGN * v.x);
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft2(T* x) {
+    const T a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft4v(T& x0, T& x1, T& x2, T& x3) {
+    const T t0 = cadd(x0, x2), t1 = csub(x0, x2), t2 = cadd(x1, x3), t3 = csub(x1, x3);
+    const T u3 = mul_si<SIGN>(t3);
+    x0 = cadd(t0, t2);
+    x2 = csub(t0, t2);
+    x1 = cadd(t1, u3);
+    x3 = csub(t1, u3);
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft4(T* x) {
+    dft4v<SIGN>(x[0], x[1], x[2], x[3]);
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft8(T* x) {
+    using R = typename Real<T>::type;
+    const R c = (R)0.70710678118654752440;
+    const R s = (R)SIGN;
+    T e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    T o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4v<SIGN>(e0, e1, e2, e3);
+    dft4v<SIGN>(o0, o1, o2, o3);
+    o1 = mk<T>(c * (o1.x - s * o1.y), c * (o1.y + s * o1.x));   // * exp(SIGN i pi/4)
+    o2 = mul_si<SIGN>(o2);                                      // * exp(SIGN i pi/2)
+    o3 = mk<T>(-c * (o3.x + s * o3.y), c * (s * o3.x - o3.y));  // * exp(SIGN 3 i pi/4)
+    x[0] = cadd(e0, o0);
+    x[4] = csub(e0, o0);
+    x[1] = cadd(e1, o1);
+    x[5] = csub(e1, o1);
+    x[2] = cadd(e2, o2);
+    x[6] = csub(e2, o2);
+    x[3] = cadd(e3, o3);
+    x[7] = csub(e3, o3);
+}
+
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dftR(T* x) {
+    if constexpr (R == 8)
+        dft8<SIGN>(x);
+    else if constexpr (R == 4)
+        dft4<SIGN>(x);
+    else
+        dft2<SIGN>(x);
+}
+
+// One Stockham pass of radix R over the E = V*kThreads values of the block
+// (E/N transforms of length N, row stride S).  Sub-transform size Ns = 2^lgNs.
+template <typename T, int SIGN, int V, int R, int LGR>
+__device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T* tw) {
+    constexpr int PER = V / R;
+    const int lgNR = lgN - LGR;
+    const int NR = 1 << lgNR;
+    const int Ns = 1 << lgNs;
+    T x[V];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int b = threadIdx.x + u * kThreads;
+        const int f = b >> lgNR, j = b & (NR - 1);
+        const T* src = a + f * S + j;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[u * R + r] = src[r * NR];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int b = threadIdx.x + u * kThreads;
+        const int f = b >> lgNR, j = b & (NR - 1);
+        const int k = j & (Ns - 1);
+        const int step = k << (lgNR - lgNs);  // k * N / (R * Ns)
+        if (lgNs > 0) {  // the first pass has k = 0: unit twiddles
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                T w = tw[r * step];
+                if (SIGN > 0) w.y = -w.y;
+                x[u * R + r] = cmul(x[u * R + r], w);
+            }
+        }
+        dftR<R, SIGN>(x + u * R);
+        T* dst = a + f * S + ((j - k) << LGR) + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[r * Ns] = x[u * R + r];
+    }
+    __syncthreads();
+}
+
+// In-place FFTs of all E/N rows of the block's LDS tile (natural order in and out).
+template <typename T, int SIGN, int V>
+__device__ __forceinline__ void fft_block(T* a, int lgN, int S, const T* tw) {
+    int lgNs = 0, rem = lgN;
+    while (rem >= 3) {
+        fft_pass<T, SIGN, V, 8, 3>(a, lgN, S, lgNs, tw);
+        lgNs += 3;
+        rem -= 3;
+    }
+    if (rem == 2)
+        fft_pass<T, SIGN, V, 4, 2>(a, lgN, S, lgNs, tw);
+    else if (rem == 1)
+        fft_pass<T, SIGN, V, 2, 1>(a, lgN, S, lgNs, tw);
+}
+
+__device__ __forceinline__ int frow_of(int q, int N, int cut) { return q < cut ? q : N - 2 * cut + 1 + q; }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8), and
+// each XCD has its own L2.  Renumber blocks so that every XCD works through a
+// contiguous range of logical blocks (x fastest): neighbours that share cache
+// lines (transposed 64-byte row segments) or a table tile (same order m) then
+// meet in the same L2.  A bijection on [0, gridDim.x * gridDim.y).
+__device__ __forceinline__ void xcd_block(int& bx, int& by) {
+    const int nx = gridDim.x, total = nx * gridDim.y;
+    const int id = blockIdx.y * nx + blockIdx.x;
+    const int q = total >> 3, r = total & 7, k = id & 7, local = id >> 3;
+    const int lid = (k < r) ? k * (q + 1) + local : r * (q + 1) + (k - r) * q + local;
+    by = lid / nx;
+    bx = lid - by * nx;
+}
+
+// GEMM grids (column blocks x orders, orders a multiple of 8): XCD k takes
+// the orders m = k, k+8, k+16, ... (the work per order falls with m, so this
+// balances the XCDs) and runs all column blocks of one order back to back,
+// so that order's table tile stays in its L2.
+__device__ __forceinline__ void xcd_block_m(int& bx, int& m) {
+    const int nx = gridDim.x;
+    const int id = blockIdx.y * nx + blockIdx.x;
+    const int k = id & 7, local = id >> 3;
+    const int slot = local / nx;
+    bx = local - slot * nx;
+    m = k + 8 * slot;
+}
+
+// position of cosine index k in the parity-split planes of the Cb buffer
+__device__ __forceinline__ int cpos(int k, int half) { return (k & 1) ? half + (k >> 1) : (k >> 1); }
+
+// Copy the N twiddles exp(-2 pi i t/N) into LDS (they are read by every
+// butterfly of every pass).
+template <typename T>
+__device__ __forceinline__ T* stage_twiddles(unsigned char* smem, const T* __restrict__ tw, int N) {
+    T* t = reinterpret_cast<T*>(smem);
+    for (int i = threadIdx.x; i < N; i += kThreads) t[i] = tw[i];
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// FFT-type kernels.  grid.y = field, block = kThreads, LDS = rows*(N+1) values
+// ---------------------------------------------------------------------------
+template <typename R, int V>
+__global__ __launch_bounds__(kThreads) void k_fphi_fwd(const double* __restrict__ re, const double* __restrict__ im,
+                                                       typename Cplx<R>::type* __restrict__ X, int lgN,
+                                                       const typename Cplx<R>::type* __restrict__ tw) {
+    using T = typename Cplx<R>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int E = V * kThreads;
+    const int N = 1 << lgN, S = N + 1;
+    const T* twl = stage_twiddles(smem, tw, N);
+    T* a = reinterpret_cast<T*>(smem) + N;
+    const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int j0 = bx * rpb;
+    const size_t base = ((size_t)f * N + j0) * N;
+    {
+        double vr[V], vi[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            vr[u] = re[base + threadIdx.x + u * kThreads];
+            vi[u] = im[base + threadIdx.x + u * kThreads];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            a[(idx >> lgN) * S + (idx & (N - 1))] = mk<T>((R)vr[u], (R)vi[u]);
+        }
+    }
+    __syncthreads();
+    fft_block<T, -1, V>(a, lgN, S, twl);
+    {
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            v[u] = a[(idx & (rpb - 1)) * S + (idx >> lgrpb)];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            X[((size_t)f * N + (idx >> lgrpb)) * N + j0 + (idx & (rpb - 1))] = v[u];
+        }
+    }
+}
+
+template <typename R, int V>
+__global__ __launch_bounds__(kThreads) void k_fdct_fwd(const typename Cplx<R>::type* __restrict__ X, R* __restrict__ Cb,
+                                                       const R* __restrict__ w_semi, int lgN, int cut, int nrows,
+                                                       const typename Cplx<R>::type* __restrict__ tw,
+                                                       const typename Cplx<R>::type* __restrict__ ph) {
+    using T = typename Cplx<R>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const T* twl = stage_twiddles(smem, tw, 1 << lgN);
+    T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
+    constexpr int E = V * kThreads;
+    const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
+    const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int q0 = bx * rpb;
+    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
+        const int rl = idx >> lgN, j = idx & (N - 1);
+        const int q = q0 + rl;
+        const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+        T x = mk<T>(0, 0);
+        if (q < nrows) {
+            const int r = frow_of(q, N, cut);
+            const int am = r < B ? r : N - r;
+            x = cscale(X[((size_t)f * N + r) * N + j], w_semi[(am & 1) * N + j]);
+        }
+        a[rl * S + p] = x;
+    }
+    __syncthreads();
+    fft_block<T, -1, V>(a, lgN, S, twl);
+    for (int idx = threadIdx.x; idx < rpb * B; idx += kThreads) {
+        const int rl = idx >> (lgN - 1), k = idx & (B - 1);
+        const int q = q0 + rl;
+        if (q >= nrows) continue;
+        const int r = frow_of(q, N, cut);
+        const T* v = a + rl * S;
+        const T e = ph[k];
+        T c = cadd(cmul(e, v[k]), cmul(cconj(e), v[(N - k) & (N - 1)]));
+        if (k == 0) c = cscale(c, (R)0.70710678118654752440);
+        R* row = Cb + ((size_t)f * N + r) * 2 * B;
+        const int pos = cpos(k, half);
+        row[pos] = c.x;
+        row[B + pos] = c.y;
+    }
+}
+
+template <typename R, int V>
+__global__ __launch_bounds__(kThreads) void k_fdct_inv(const R* __restrict__ Cb, typename Cplx<R>::type* __restrict__ X,
+                                                       const R* __restrict__ sinv, int lgN, int cut, int nrows,
+                                                       int fmt, const typename Cplx<R>::type* __restrict__ tw,
+                                                       const typename Cplx<R>::type* __restrict__ ph) {
+    using T = typename Cplx<R>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const T* twl = stage_twiddles(smem, tw, 1 << lgN);
+    T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
+    constexpr int E = V * kThreads;
+    const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
+    const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int q0 = bx * rpb;
+    // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]),  X[N] = 0, X[k >= B] = 0
+    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
+        const int rl = idx >> lgN, k = idx & (N - 1);
+        const int q = q0 + rl;
+        T v = mk<T>(0, 0);
+        if (q < nrows) {
+            const int r = frow_of(q, N, cut);
+            const R* row = Cb + ((size_t)f * N + r) * 2 * B;
+            T xk = mk<T>(0, 0), xn = mk<T>(0, 0);
+            if (k < B) xk = mk<T>(row[cpos(k, half)], row[B + cpos(k, half)]);
+            const int kn = N - k;
+            if (k > 0 && kn < B) xn = mk<T>(row[cpos(kn, half)], row[B + cpos(kn, half)]);
+            v = cmul(cconj(ph[k]), mk<T>(xk.x + xn.y, xk.y - xn.x));
+        }
+        a[rl * S + k] = v;
+    }
+    __syncthreads();
+    fft_block<T, +1, V>(a, lgN, S, twl);
+    const R inv_s2pi = (R)0.39894228040143267794;
+    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
+        const int rl = idx >> lgN, j = idx & (N - 1);
+        const int q = q0 + rl;
+        if (q >= nrows) continue;
+        const int r = frow_of(q, N, cut);
+        const int am = r < B ? r : N - r;
+        const int n = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+        R sc = inv_s2pi;
+        if (am & 1) sc *= sinv[j];
+        if (r > B && (r & 1)) sc = -sc;
+        const T y = cscale(a[rl * S + n], sc);
+        X[((size_t)f * N + r) * N + j] = y;
+        if (fmt == 1 && r > 0) X[((size_t)f * N + (N - r)) * N + j] = cconj(y);
+    }
+}
+
+template <typename R, int V>
+__global__ __launch_bounds__(kThreads) void k_fphi_inv(const typename Cplx<R>::type* __restrict__ X,
+                                                       double* __restrict__ re, double* __restrict__ im, int lgN,
+                                                       const typename Cplx<R>::type* __restrict__ tw) {
+    using T = typename Cplx<R>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int E = V * kThreads;
+    const int N = 1 << lgN, S = N + 1, B = N >> 1;
+    const T* twl = stage_twiddles(smem, tw, N);
+    T* a = reinterpret_cast<T*>(smem) + N;
+    const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
+    int bx, f;
+    xcd_block(bx, f);
+    const int j0 = bx * rpb;
+    {
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            const int r = idx >> lgrpb;
+            v[u] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + (idx & (rpb - 1))];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            a[(idx & (rpb - 1)) * S + (idx >> lgrpb)] = v[u];
+        }
+    }
+    __syncthreads();
+    fft_block<T, +1, V>(a, lgN, S, twl);
+    const size_t base = ((size_t)f * N + j0) * N;
+    {
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * kThreads;
+            v[u] = a[(idx >> lgN) * S + (idx & (N - 1))];
+        }
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            re[base + threadIdx.x + u * kThreads] = (double)v[u].x;
+            im[base + threadIdx.x + u * kThreads] = (double)v[u].y;
+        }
+    }
+}
+
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+template <typename R>
+struct Mma;
+template <>
+struct Mma<double> {
+    using acc = dvec4;
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row of the r-th accumulator of `lane` in the 16x16 result
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mma<float> {
+    using acc = fvec4;
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+inline int lg2(int n) { return __builtin_ctz(n); }
+
+// Launch with `lds` bytes of dynamic LDS; above 64 KiB the kernel must opt
+// in (gfx950 has 160 KiB per workgroup).
+template <typename K, typename... A>
+inline void launch_lds(K kernel, dim3 grid, int threads, size_t lds, hipStream_t s, A... args) {
+    if (lds > 65536) S2_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    kernel<<<grid, threads, lds, s>>>(args...);
+    S2_HIP(hipGetLastError());
+}
+
+}  // namespace s2

@@ -62,12 +62,14 @@ struct DevConsts {
 };
 
 // Zero-padded, parity-split Legendre matrices for the MFMA GEMMs of the fast
-// path (s2_fast.hip).  Entry m*2+p describes order m, degrees l = m+p+2i.
+// path (s2_fast.hip) and the mixed form (s2_mixed.hip).  Entry m*2+p describes order m, degrees l = m+p+2i.
 struct GemmMeta {
     int Lp;     // number of degrees in the class
-    int Kf;     // cosine terms of its last (longest) row
-    long offF;  // offset of AF(m,p) [roundup4(Kf)][roundup16(Lp)]
-    long offI;  // offset of AI(m,p) [roundup4(Lp)][roundup16(bw/2)]
+    int Kf;     // seminaive: cosine terms of its last (longest) row
+    long offF;  // seminaive: AF(m,p) [roundup4(Kf)][roundup16(Lp)]; mixed: NF(m), both parities
+    long offI;  // seminaive: AI(m,p) [roundup4(Lp)][roundup16(bw/2)]; mixed: NI(m,p)
+    int n0;     // mixed: 16-degree tiles per parity, ceil(Lp(m,0)/16)
+    int ki;     // mixed: rows of NI(m,p), Lp padded to whole 16-degree chunks
 };
 
 struct GemmTables {
@@ -164,7 +166,7 @@ void forward_fast(const DevConsts& c, const GemmTables& g, const double* re, con
 void inverse_fast(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
                   double* im, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm);
 
-// ---- mixed form (s2_fast.hip): FP32 ring FFTs + naive DLT on FP32 MFMA ----
+// ---- mixed form (s2_mixed.hip): FP32 ring FFTs + naive DLT on FP32 MFMA ----
 bool mixed_supported(int bw);
 void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s);
 void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,

@@ -54,6 +54,9 @@ __host__ __device__ inline int transpose_row_size(int row, int m, int bw) {
 // First degree read by row i of the transposed table, as an offset from m.
 __host__ __device__ inline int transpose_first(int i, int m) { return (i > m) ? (i - m) + (m & 1) : (i & 1); }
 
+// Degrees l = m+p+2i (i = 0, 1, ...) of order m below bw with l-m of parity p.
+__host__ __device__ inline int gm_lp(int m, int p, int bw) { return (bw - m - p > 0) ? (bw - m - p + 1) / 2 : 0; }
+
 __host__ __device__ inline int index_of_coeff(int m, int l, int bw) {
     int L = bw - 1;
     if (m >= 0) return m * (L + 1) - (m * (m - 1)) / 2 + (l - m);

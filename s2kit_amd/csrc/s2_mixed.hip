@@ -1,0 +1,519 @@
+// Mixed form: FP32 ring FFTs and the naive Legendre transform (naive.c:34-95)
+// as FP32 MFMA GEMMs over the orders; FP64 inputs and outputs, tables from
+// the FP64 three-term recurrence (pml.c:41-79) rounded once to FP32.
+//
+//   forward   k_fphi_fwd<float> (s2_blocks.h) -> k_mix_fwd
+//   inverse   k_mix_inv -> k_fphi_inv<float>
+//
+// Folding.  P_l^m(theta_{N-1-j}) = (-1)^(l-m) P_l^m(theta_j) and the
+// quadrature weights are symmetric, so degrees with l-m even see
+// x_j + x_{N-1-j} and odd ones x_j - x_{N-1-j}, over the bw rings j < bw.
+//
+// GEMMs.  The data columns are (field, +m or -m row, re or im); per order m
+//   forward  C[col][l]  = sum_{j<bw} fold_p(X[col])[j] * w_j P_l^m(theta_j)
+//   inverse  E/O[col][j] = sum_i coef[col][m+p+2i] * P_{m+p+2i}^m(theta_j)
+// with ring j <- E+O and ring N-1-j <- E-O.  One launch covers every order:
+// a block owns kMixCols columns of one order; workgroups are numbered so that
+// each XCD takes orders m = 8q + k and bw-1-(8q+k) alternately, a cheap
+// order (few degrees, bound by its data stream) next to an expensive one.
+//
+// Operands.  A (data or coefficients) goes from global memory straight into
+// registers in the MFMA operand layout; the K index is permuted so that each
+// lane reads S consecutive rings (degrees) per chunk.  B (the table) streams
+// through two LDS buffers: the next chunk's loads are issued before the
+// current chunk's MFMAs and stored after them, one barrier per chunk.
+#include <algorithm>
+#include <cmath>
+
+#include "s2_blocks.h"
+#include "s2_recurrence.h"
+
+namespace s2 {
+
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+// Waves per block, 16 data columns each.
+#ifndef S2_MIX_WAVES
+#define S2_MIX_WAVES 4
+#endif
+constexpr int kMixThreads = 64 * S2_MIX_WAVES;
+constexpr int kMixCols = 16 * S2_MIX_WAVES;
+
+// Tables.
+//   NF(m) [j < bw][W], W = 32 n0(m): tiles t < n0 hold degrees l = m+2(16t+x),
+//         tiles n0 + t hold l = m+1+2(16t+x), as w_j P_l^m(theta_j)
+//         (w_j: even DH weight times sqrt(2pi)/N).  Tiles are interleaved in
+//         fours -- entry x of tile 4g+v sits at 64g + 4x + v, a last pair at
+//         64G + 2x + v -- so one float4 (float2) per lane feeds four (two)
+//         16x16 MFMA tiles.  Entries past the last degree are zero.
+//   NI(m,p) [i < ki(m)][j < bw]: P_{m+p+2i}^m(theta_j), rings interleaved in
+//         groups of 64 the same way; rows past the last degree are zero.
+__host__ __device__ inline int mix_n0(int m, int B) { return (gm_lp(m, 0, B) + 15) / 16; }
+__host__ __device__ inline int mix_ki(int m, int B) { return 16 * ((B - m + 31) / 32); }
+__host__ __device__ inline int mix_pos(int t, int x, int nt) {
+    const int g4 = nt >> 2;
+    return (t < 4 * g4) ? 64 * (t >> 2) + 4 * x + (t & 3) : 64 * g4 + 2 * x + (t & 1);
+}
+__host__ __device__ inline int ilv64(int j) { return (j & ~63) + 4 * (j & 15) + ((j >> 4) & 3); }
+
+// Rings per forward chunk: at least 16 MFMAs per wave between barriers.
+__host__ __device__ constexpr int mix_kc(int n0) { return n0 <= 2 ? 32 : 16; }
+constexpr int kMixInvKC = 16;  // degrees per parity per inverse chunk
+
+__global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restrict__ meta, float* __restrict__ NF,
+                           float* __restrict__ NI, int B) {
+    const int m = blockIdx.x;
+    const int N = 2 * B;
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int nt = 2 * g0.n0, W = 16 * nt;
+    const double cm = (m == 0) ? 0.0 : pmm_coeff(m);
+    for (int j = threadIdx.x; j < B; j += blockDim.x) {
+        const double th = (2.0 * j + 1.0) * M_PI / (2.0 * N);
+        const double x = cos(th);
+        const double wj = w[j];
+        double cur = (m == 0) ? 0.70710678118654752440 : cm * pow(sin(th), (double)m);
+        double prev = 0.0;
+        for (int l = m; l < B; ++l) {
+            const int p = (l - m) & 1, i = (l - m) >> 1;
+            NF[g0.offF + (long)j * W + mix_pos(p * g0.n0 + (i >> 4), i & 15, nt)] = (float)(cur * wj);
+            NI[(p ? g1.offI : g0.offI) + (long)i * B + ilv64(j)] = (float)cur;
+            if (l + 1 < B) {
+                const double nx = l2_an(m, l) * (cur * x) + l2_cn(m, l) * prev;
+                prev = cur;
+                cur = nx;
+            }
+        }
+    }
+}
+
+// Workgroup id -> (order m, column block cb) over nb column blocks per order.
+// Ids go round-robin over the 8 XCDs (id mod 8); XCD k runs, in turn, the
+// column blocks of orders 8q+k and bw-1-(8q+k) interleaved.  bw % 16 == 0.
+__device__ __forceinline__ void mix_block(int B, int nb, int& m, int& cb) {
+    const int id = blockIdx.x, k = id & 7, local = id >> 3;
+    const int q = local / (2 * nb), r = local - q * 2 * nb;
+    cb = r >> 1;
+    const int ml = 8 * q + k;
+    m = (r & 1) ? B - 1 - ml : ml;
+}
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+template <int N0>
+__device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const float* __restrict__ NF,
+                                             const GemmMeta& g0, const GemmMeta& g1, double* __restrict__ cre,
+                                             double* __restrict__ cim, int m, int cb, int B, int F, int fmt,
+                                             float* sm) {
+    constexpr int NT = 2 * N0, G4 = NT / 4, R2 = NT % 4;  // R2: 0, or 2 tiles in a last pair
+    constexpr int W = 16 * NT;
+    constexpr int KC = mix_kc(N0), S = KC / 4;
+    constexpr int NV = KC * W / 4;  // float4 per table chunk
+    constexpr int PT = (NV + kMixThreads - 1) / kMixThreads;
+    const int N = 2 * B;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    if (cb * kMixCols >= ncols) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x16 = lane & 15, kq = lane >> 4;
+
+    // operand A: this lane's column, rings kc + kq*S + s (s < S) of a chunk
+    const int col = cb * kMixCols + wave * 16 + x16;
+    const bool imcol = col & 1;
+    const float2* xr;
+    {
+        const int cc = col < ncols ? col : 0;  // past the end: read column 0, store nothing
+        const int f = cc / cpf, rem = cc - f * cpf;
+        xr = X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N;
+    }
+    const fvec4* tsrc = reinterpret_cast<const fvec4*>(NF + g0.offF);
+
+    fvec4 xa[S / 2], xb[S / 2];  // rings j0 .. j0+S-1 and N-j0-S .. N-j0-1 (pairs of complex values)
+    fvec4 tv[PT];
+    auto fetch = [&](int ch) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < PT; ++u) {  // a partial last round re-reads the chunk's last float4
+            const int e = (NV % kMixThreads == 0) ? tid + u * kMixThreads : min(tid + u * kMixThreads, NV - 1);
+            tv[u] = tsrc[(size_t)ch * NV + e];
+        }
+        const int j0 = ch * KC + kq * S;
+        const fvec4* pa = reinterpret_cast<const fvec4*>(xr + j0);
+        const fvec4* pb = reinterpret_cast<const fvec4*>(xr + N - j0 - S);
+#pragma unroll
+        for (int h = 0; h < S / 2; ++h) {
+            xa[h] = pa[h];
+            xb[h] = pb[h];
+        }
+    };
+    auto stash = [&](float* tb) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < PT; ++u)
+            if (NV % kMixThreads == 0 || tid + u * kMixThreads < NV)
+                reinterpret_cast<fvec4*>(tb)[tid + u * kMixThreads] = tv[u];
+    };
+    float a0[S], a1[S];  // folded operands: x_j + x_{N-1-j}, x_j - x_{N-1-j}
+    auto fold = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const fvec4 va = xa[s >> 1];
+            const fvec4 vb = xb[(S - 1 - s) >> 1];  // ring N-1-j0-s is entry S-1-s of the mirror run
+            const float u = (s & 1) ? (imcol ? va.w : va.z) : (imcol ? va.y : va.x);
+            const int sb = S - 1 - s;
+            const float v = (sb & 1) ? (imcol ? vb.w : vb.z) : (imcol ? vb.y : vb.x);
+            a0[s] = u + v;
+            a1[s] = u - v;
+        }
+    };
+
+    fvec4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = fvec4{0.f, 0.f, 0.f, 0.f};
+    constexpr int TB = KC * W;  // floats per table buffer
+    fetch(0);
+    stash(sm);
+    __syncthreads();
+    const int nch = B / KC;
+    for (int ch = 0; ch < nch; ++ch) {
+        const float* tb = sm + (ch & 1) * TB + kq * S * W;  // this lane's table rows: ring kq*S + s
+        fold();
+        const bool more = ch + 1 < nch;
+        if (more) fetch(ch + 1);
+        fvec4 tq[2][G4 > 0 ? G4 : 1];
+        fvec2 tp[2];
+        auto lds = [&](int s, int b) __attribute__((always_inline)) {
+            const float* r = tb + s * W;
+#pragma unroll
+            for (int g = 0; g < G4; ++g) tq[b][g] = *reinterpret_cast<const fvec4*>(r + 64 * g + 4 * x16);
+            if (R2) tp[b] = *reinterpret_cast<const fvec2*>(r + 64 * G4 + 2 * x16);
+        };
+        lds(0, 0);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int b = s & 1;
+            if (s + 1 < S) lds(s + 1, b ^ 1);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const float a = t < N0 ? a0[s] : a1[s];
+                float bv;
+                if (t < 4 * G4) {
+                    bv = tq[b][t >> 2][t & 3];
+                } else {
+                    bv = tp[b][t & 1];
+                }
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[t], 0, 0, 0);
+            }
+        }
+        if (more) stash(sm + ((ch + 1) & 1) * TB);
+        __syncthreads();
+    }
+
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int p = t < N0 ? 0 : 1;
+        const int i = 16 * (t - p * N0) + x16;
+        if (i >= (p ? g1.Lp : g0.Lp)) continue;
+        const int l = m + p + 2 * i;
+        const int ip = index_of_coeff(m, l, B), in = index_of_coeff(-m, l, B);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = cb * kMixCols + wave * 16 + 4 * kq + r;
+            if (c >= ncols) continue;
+            const int f = c / cpf, rem = c - f * cpf;
+            const int comp = rem & 1;
+            double* out = (comp ? cim : cre) + f * B2;
+            const double v = (double)acc[t][r];
+            if ((rem >> 1) == 0) {
+                out[ip] = v;
+                if (fmt == 1 && m > 0) out[in] = comp ? -sg * v : sg * v;
+            } else {
+                out[in] = sg * v;
+            }
+        }
+    }
+}
+
+template <int MAXN0>
+__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_mix_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
+                                                        const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                                        double* __restrict__ cim, int lgB, int F, int fmt, int nb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sm = reinterpret_cast<float*>(smem);
+    const int B = 1 << lgB;
+    int m, cb;
+    mix_block(B, nb, m, cb);
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+#define S2_MIX_FWD(n) \
+    case n:           \
+        if (n <= MAXN0) mix_fwd_body<(n <= MAXN0 ? n : 1)>(X, NF, g0, g1, cre, cim, m, cb, B, F, fmt, sm); \
+        break;
+    switch (g0.n0) {
+        S2_MIX_FWD(1)
+        S2_MIX_FWD(2)
+        S2_MIX_FWD(3)
+        S2_MIX_FWD(4)
+        S2_MIX_FWD(5)
+        S2_MIX_FWD(6)
+        S2_MIX_FWD(7)
+        S2_MIX_FWD(8)
+        S2_MIX_FWD(9)
+        S2_MIX_FWD(10)
+        S2_MIX_FWD(11)
+        S2_MIX_FWD(12)
+        S2_MIX_FWD(13)
+        S2_MIX_FWD(14)
+        S2_MIX_FWD(15)
+        S2_MIX_FWD(16)
+        default: break;
+    }
+#undef S2_MIX_FWD
+}
+
+// ---------------------------------------------------------------------------
+// inverse
+// ---------------------------------------------------------------------------
+// A block owns kMixCols columns of one order and JW = 64*GJ of the bw folded rings
+// (jb selects which); chunks of KC degrees per parity.
+template <int GJ>
+__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_mix_inv(const double* __restrict__ cre, const double* __restrict__ cim,
+                                                        const float* __restrict__ NI, const GemmMeta* __restrict__ meta,
+                                                        float2* __restrict__ X, int lgB, int F, int fmt, int nb) {
+    constexpr int JW = 64 * GJ;
+    constexpr int KC = kMixInvKC, S = KC / 4;
+    constexpr int NV = 2 * KC * JW / 4;  // float4 per table chunk (both parities)
+    constexpr int PT = NV / kMixThreads;
+    static_assert(NV % kMixThreads == 0, "table chunk must split evenly over the block");
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sm = reinterpret_cast<float*>(smem);
+    const int B = 1 << lgB, N = 2 * B;
+    const int nz = B / JW;
+    int m, cz;
+    mix_block(B, nb * nz, m, cz);
+    const int cb = cz / nz, jbase = (cz - cb * nz) * JW;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    if (cb * kMixCols >= ncols) return;
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int L = B - m;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x16 = lane & 15, kq = lane >> 4;
+    const size_t B2 = (size_t)B * B;
+
+    // operand A: coefficients of this lane's column, degree offsets
+    // d = 2(ch*KC + kq*S) + e, e < 2S (parity e & 1, index s = e >> 1)
+    const int col = cb * kMixCols + wave * 16 + x16;
+    const double* cp;
+    {
+        const int cc = col < ncols ? col : 0;
+        const int f = cc / cpf, rem = cc - f * cpf;
+        cp = ((rem & 1) ? cim : cre) + f * B2 + index_of_coeff((rem >> 1) ? -m : m, m, B);
+    }
+    // table: float4 e = tid + u*kMixThreads of a chunk is parity p, row rw, quad c4
+    const float* tsrc[PT];
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+        const int e = tid + u * kMixThreads;
+        const int p = e / (KC * JW / 4), rest = e - p * (KC * JW / 4);
+        const int rw = rest / (JW / 4), c4 = rest - rw * (JW / 4);
+        tsrc[u] = NI + (p ? g1.offI : g0.offI) + (size_t)rw * B + jbase + 4 * c4;
+    }
+    double cv[2 * S];
+    fvec4 tv[PT];
+    auto fetch = [&](int ch) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < PT; ++u) tv[u] = *reinterpret_cast<const fvec4*>(tsrc[u] + (size_t)ch * KC * B);
+        const int d0 = 2 * (ch * KC + kq * S);
+#pragma unroll
+        for (int e = 0; e < 2 * S; ++e) cv[e] = cp[min(d0 + e, L - 1)];
+    };
+    auto stash = [&](float* tb) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < PT; ++u) reinterpret_cast<fvec4*>(tb)[tid + u * kMixThreads] = tv[u];
+    };
+    float a[2][S];
+    auto convert = [&](int ch) __attribute__((always_inline)) {
+        const int d0 = 2 * (ch * KC + kq * S);
+#pragma unroll
+        for (int e = 0; e < 2 * S; ++e) a[e & 1][e >> 1] = (d0 + e < L) ? (float)cv[e] : 0.f;
+    };
+
+    fvec4 acc[2][4 * GJ];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int t = 0; t < 4 * GJ; ++t) acc[p][t] = fvec4{0.f, 0.f, 0.f, 0.f};
+    constexpr int TB = 2 * KC * JW;  // floats per table buffer
+    fetch(0);
+    stash(sm);
+    __syncthreads();
+    const int nch = g0.ki / KC;
+    for (int ch = 0; ch < nch; ++ch) {
+        const float* tb = sm + (ch & 1) * TB;
+        convert(ch);
+        const bool more = ch + 1 < nch;
+        if (more) fetch(ch + 1);
+        fvec4 tq[2][2][GJ];
+        auto lds = [&](int s, int b) __attribute__((always_inline)) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const float* r = tb + (p * KC + kq * S + s) * JW;
+#pragma unroll
+                for (int g = 0; g < GJ; ++g) tq[b][p][g] = *reinterpret_cast<const fvec4*>(r + 64 * g + 4 * x16);
+            }
+        };
+        lds(0, 0);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int b = s & 1;
+            if (s + 1 < S) lds(s + 1, b ^ 1);
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int g = 0; g < GJ; ++g) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[p][4 * g + v] =
+                            __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][s], tq[b][p][g][v], acc[p][4 * g + v], 0, 0, 0);
+                }
+        }
+        if (more) stash(sm + ((ch + 1) & 1) * TB);
+        __syncthreads();
+    }
+
+    const float inv_s2pi = 0.39894228040143267794f;
+#pragma unroll
+    for (int t = 0; t < 4 * GJ; ++t) {
+        const int j = jbase + 16 * t + x16;
+#pragma unroll
+        for (int r = 0; r < 4; r += 2) {  // rows r, r+1: re, im of one (field, sign)
+            const int c = cb * kMixCols + wave * 16 + 4 * kq + r;
+            if (c >= ncols) continue;
+            const int f = c / cpf, rem = c - f * cpf;
+            const int row = (rem >> 1) ? N - m : m;
+            const float sc = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
+            const fvec4& e = acc[0][t];
+            const fvec4& o = acc[1][t];
+            const float2 yn = make_float2((e[r] + o[r]) * sc, (e[r + 1] + o[r + 1]) * sc);
+            const float2 ys = make_float2((e[r] - o[r]) * sc, (e[r + 1] - o[r + 1]) * sc);
+            float2* xr = X + ((size_t)f * N + row) * N;
+            xr[j] = yn;
+            xr[N - 1 - j] = ys;
+            if (fmt == 1 && m > 0) {
+                float2* xm = X + ((size_t)f * N + (N - m)) * N;
+                xm[j] = make_float2(yn.x, -yn.y);
+                xm[N - 1 - j] = make_float2(ys.x, -ys.y);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+bool mixed_supported(int bw) {
+    const int N = 2 * bw;
+    return bw >= 64 && bw <= 512 && !(N & (N - 1));
+}
+
+void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
+    const int B = c.bw;
+    g.bw = B;
+    g.prec = 2;
+    g.h_meta.assign(2 * B, GemmMeta{});
+    long of = 0, oi = 0;
+    for (int m = 0; m < B; ++m) {
+        const int n0 = mix_n0(m, B), ki = mix_ki(m, B);
+        for (int p = 0; p < 2; ++p) {
+            GemmMeta& e = g.h_meta[m * 2 + p];
+            e.Lp = gm_lp(m, p, B);
+            e.Kf = B;
+            e.n0 = n0;
+            e.ki = ki;
+            e.offF = of;
+            e.offI = oi;
+            oi += (long)ki * B;
+        }
+        of += (long)B * 32 * n0;
+    }
+    g.nf = of;
+    g.ni = oi;
+    S2_HIP(hipMalloc(&g.meta, sizeof(GemmMeta) * 2 * B));
+    S2_HIP(hipMalloc(&g.af, sizeof(float) * std::max(1L, of)));
+    S2_HIP(hipMalloc(&g.ai, sizeof(float) * std::max(1L, oi)));
+    S2_HIP(hipMemsetAsync(g.af, 0, sizeof(float) * std::max(1L, of), s));
+    S2_HIP(hipMemsetAsync(g.ai, 0, sizeof(float) * std::max(1L, oi), s));
+    S2_HIP(hipMemcpyAsync(g.meta, g.h_meta.data(), sizeof(GemmMeta) * 2 * B, hipMemcpyHostToDevice, s));
+    k_mix_pack<<<B, 256, 0, s>>>(c.w_naive, g.meta, (float*)g.af, (float*)g.ai, B);
+    S2_HIP(hipGetLastError());
+    S2_HIP(hipStreamSynchronize(s));
+}
+
+// LDS of k_mix_fwd: two table chunks of the widest order.
+static size_t mix_fwd_lds(int B) {
+    size_t mx = 0;
+    for (int n0 = 1; n0 <= mix_n0(0, B); ++n0) mx = std::max(mx, sizeof(float) * 2 * mix_kc(n0) * 32 * n0);
+    return mx;
+}
+
+void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
+                   double* cim, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
+    constexpr int V = 16;
+    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int rpb = V * kThreads / N;
+    float2* X = (float2*)Xv;
+    {
+        Span sp(tm, ST_PHI_FWD, s);
+        launch_lds(k_fphi_fwd<float, V>, dim3(N / rpb, F), kThreads, sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
+                   re, im, X, lgN, (const float2*)c.tw32);
+    }
+    {
+        Span sp(tm, ST_LEG_FWD, s);
+        const int nb = (4 * F + kMixCols - 1) / kMixCols;
+        const dim3 grid(B * nb);
+        const size_t lds = mix_fwd_lds(B);
+        const float* nf = (const float*)g.af;
+        const int n0max = mix_n0(0, B);
+        if (n0max <= 2)
+            launch_lds(k_mix_fwd<2>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre, cim,
+                       lg2(B), F, (int)fmt, nb);
+        else if (n0max <= 4)
+            launch_lds(k_mix_fwd<4>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre, cim,
+                       lg2(B), F, (int)fmt, nb);
+        else if (n0max <= 8)
+            launch_lds(k_mix_fwd<8>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre, cim,
+                       lg2(B), F, (int)fmt, nb);
+        else
+            launch_lds(k_mix_fwd<16>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre,
+                       cim, lg2(B), F, (int)fmt, nb);
+    }
+}
+
+void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
+                   double* im, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
+    constexpr int V = 16;
+    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int rpb = V * kThreads / N;
+    float2* X = (float2*)Xv;
+    {
+        Span sp(tm, ST_LEG_INV, s);
+        const int gj = std::min(4, B / 64);
+        const int nb = (4 * F + kMixCols - 1) / kMixCols;
+        const dim3 grid(B * nb * (B / (64 * gj)));
+        const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * 64 * gj);
+        const float* ni = (const float*)g.ai;
+        const GemmMeta* meta = g.meta;
+        switch (gj) {
+            case 1: launch_lds(k_mix_inv<1>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
+            case 2: launch_lds(k_mix_inv<2>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
+            default: launch_lds(k_mix_inv<4>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
+        }
+    }
+    {
+        Span sp(tm, ST_PHI_INV, s);
+        launch_lds(k_fphi_inv<float, V>, dim3(N / rpb, F), kThreads, sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
+                   (const float2*)X, re, im, lgN, (const float2*)c.tw32);
+    }
+}
+
+}  // namespace s2
