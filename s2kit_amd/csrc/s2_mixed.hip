@@ -36,6 +36,15 @@ typedef float fvec2 __attribute__((ext_vector_type(2)));
 #ifndef S2_MIX_WAVES
 #define S2_MIX_WAVES 4
 #endif
+#ifndef S2_MIX_WPE_FWD
+#define S2_MIX_WPE_FWD 2
+#endif
+#ifndef S2_MIX_WPE_INV
+#define S2_MIX_WPE_INV 2
+#endif
+#ifndef S2_MIX_GJMAX
+#define S2_MIX_GJMAX 4
+#endif
 constexpr int kMixThreads = 64 * S2_MIX_WAVES;
 constexpr int kMixCols = 16 * S2_MIX_WAVES;
 
@@ -57,7 +66,13 @@ __host__ __device__ inline int mix_pos(int t, int x, int nt) {
 __host__ __device__ inline int ilv64(int j) { return (j & ~63) + 4 * (j & 15) + ((j >> 4) & 3); }
 
 // Rings per forward chunk: at least 16 MFMAs per wave between barriers.
-__host__ __device__ constexpr int mix_kc(int n0) { return n0 <= 2 ? 32 : 16; }
+#ifndef S2_MIX_KC_SMALL
+#define S2_MIX_KC_SMALL 32
+#endif
+#ifndef S2_MIX_KC_BIG
+#define S2_MIX_KC_BIG 16
+#endif
+__host__ __device__ constexpr int mix_kc(int n0) { return n0 <= 2 ? S2_MIX_KC_SMALL : S2_MIX_KC_BIG; }
 constexpr int kMixInvKC = 16;  // degrees per parity per inverse chunk
 
 __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restrict__ meta, float* __restrict__ NF,
@@ -178,28 +193,18 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
         fold();
         const bool more = ch + 1 < nch;
         if (more) fetch(ch + 1);
-        fvec4 tq[2][G4 > 0 ? G4 : 1];
-        fvec2 tp[2];
-        auto lds = [&](int s, int b) __attribute__((always_inline)) {
-            const float* r = tb + s * W;
-#pragma unroll
-            for (int g = 0; g < G4; ++g) tq[b][g] = *reinterpret_cast<const fvec4*>(r + 64 * g + 4 * x16);
-            if (R2) tp[b] = *reinterpret_cast<const fvec2*>(r + 64 * G4 + 2 * x16);
-        };
-        lds(0, 0);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const int b = s & 1;
-            if (s + 1 < S) lds(s + 1, b ^ 1);
+            const float* r = tb + s * W;
+            fvec4 tq[G4 > 0 ? G4 : 1];
+            fvec2 tp;
+#pragma unroll
+            for (int g = 0; g < G4; ++g) tq[g] = *reinterpret_cast<const fvec4*>(r + 64 * g + 4 * x16);
+            if (R2) tp = *reinterpret_cast<const fvec2*>(r + 64 * G4 + 2 * x16);
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const float a = t < N0 ? a0[s] : a1[s];
-                float bv;
-                if (t < 4 * G4) {
-                    bv = tq[b][t >> 2][t & 3];
-                } else {
-                    bv = tp[b][t & 1];
-                }
+                const float bv = (t < 4 * G4) ? tq[t >> 2][t & 3] : tp[t & 1];
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[t], 0, 0, 0);
             }
         }
@@ -207,35 +212,49 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
         __syncthreads();
     }
 
+    // Epilogue.  A column's coefficients of order m are one contiguous run
+    // of L = bw-m values (IndexOfHarmonicCoeff), so the tiles go through LDS
+    // (32 columns at a time, degree offset l-m = 2i+p along a row) and every
+    // run is written with consecutive lanes on consecutive addresses.
+    static_assert(kMixCols % 32 == 0 && KC >= 16, "staging needs 32 rows of W floats");
     const size_t B2 = (size_t)B * B;
     const double sg = (m & 1) ? -1.0 : 1.0;
+    const int L = B - m;
+    const int bp = index_of_coeff(m, m, B), bn = index_of_coeff(-m, m, B);
+    for (int h = 0; h < kMixCols / 32; ++h) {
+        if ((wave >> 1) == h) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int p = t < N0 ? 0 : 1;
-        const int i = 16 * (t - p * N0) + x16;
-        if (i >= (p ? g1.Lp : g0.Lp)) continue;
-        const int l = m + p + 2 * i;
-        const int ip = index_of_coeff(m, l, B), in = index_of_coeff(-m, l, B);
+            for (int t = 0; t < NT; ++t) {
+                const int p = t < N0 ? 0 : 1;
+                const int d = 2 * (16 * (t - p * N0) + x16) + p;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int c = cb * kMixCols + wave * 16 + 4 * kq + r;
+                for (int r = 0; r < 4; ++r) sm[((wave & 1) * 16 + 4 * kq + r) * W + d] = acc[t][r];
+            }
+        }
+        __syncthreads();
+        for (int cl = wave; cl < 32; cl += kMixThreads / 64) {
+            const int c = cb * kMixCols + 32 * h + cl;
             if (c >= ncols) continue;
             const int f = c / cpf, rem = c - f * cpf;
             const int comp = rem & 1;
             double* out = (comp ? cim : cre) + f * B2;
-            const double v = (double)acc[t][r];
-            if ((rem >> 1) == 0) {
-                out[ip] = v;
-                if (fmt == 1 && m > 0) out[in] = comp ? -sg * v : sg * v;
-            } else {
-                out[in] = sg * v;
+            const float* row = sm + cl * W;
+            for (int d = lane; d < L; d += 64) {
+                const double v = (double)row[d];
+                if ((rem >> 1) == 0) {
+                    out[bp + d] = v;
+                    if (fmt == 1 && m > 0) out[bn + d] = comp ? -sg * v : sg * v;
+                } else {
+                    out[bn + d] = sg * v;
+                }
             }
         }
+        __syncthreads();
     }
 }
 
 template <int MAXN0>
-__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_mix_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
+__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_MIX_WPE_FWD))) void k_mix_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
                                                         const GemmMeta* __restrict__ meta, double* __restrict__ cre,
                                                         double* __restrict__ cim, int lgB, int F, int fmt, int nb) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -276,7 +295,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2))
 // A block owns kMixCols columns of one order and JW = 64*GJ of the bw folded rings
 // (jb selects which); chunks of KC degrees per parity.
 template <int GJ>
-__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_mix_inv(const double* __restrict__ cre, const double* __restrict__ cim,
+__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_MIX_WPE_INV))) void k_mix_inv(const double* __restrict__ cre, const double* __restrict__ cim,
                                                         const float* __restrict__ NI, const GemmMeta* __restrict__ meta,
                                                         float2* __restrict__ X, int lgB, int F, int fmt, int nb) {
     constexpr int JW = 64 * GJ;
@@ -381,28 +400,47 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(2))
         __syncthreads();
     }
 
+    // Epilogue: this lane holds rings jbase + 16t + x16 (t < 4GJ) of columns
+    // 4kq + r; rows r, r+1 are the re, im parts of one (field, sign) row.
+    // Addresses are set up once per row pair, stores then use fixed offsets.
     const float inv_s2pi = 0.39894228040143267794f;
+    const bool mirror = fmt == 1 && m > 0;
+    float2* plo[2];
+    float2* phi[2];
+    float2* qlo[2];
+    float2* qhi[2];
+    float scl[2];
+    bool ok[2];
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+        const int c = cb * kMixCols + wave * 16 + 4 * kq + 2 * rr;
+        ok[rr] = c < ncols;
+        const int cc = ok[rr] ? c : 0;
+        const int f = cc / cpf, rem = cc - f * cpf;
+        const int row = (rem >> 1) ? N - m : m;
+        scl[rr] = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
+        float2* xr = X + ((size_t)f * N + row) * N;
+        float2* xm = X + ((size_t)f * N + (N - m)) * N;
+        plo[rr] = xr + jbase + x16;
+        phi[rr] = xr + (N - 1 - jbase - x16);
+        qlo[rr] = xm + jbase + x16;
+        qhi[rr] = xm + (N - 1 - jbase - x16);
+    }
 #pragma unroll
     for (int t = 0; t < 4 * GJ; ++t) {
-        const int j = jbase + 16 * t + x16;
+        const fvec4 e = acc[0][t], o = acc[1][t];
 #pragma unroll
-        for (int r = 0; r < 4; r += 2) {  // rows r, r+1: re, im of one (field, sign)
-            const int c = cb * kMixCols + wave * 16 + 4 * kq + r;
-            if (c >= ncols) continue;
-            const int f = c / cpf, rem = c - f * cpf;
-            const int row = (rem >> 1) ? N - m : m;
-            const float sc = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
-            const fvec4& e = acc[0][t];
-            const fvec4& o = acc[1][t];
+        for (int rr = 0; rr < 2; ++rr) {
+            if (!ok[rr]) continue;
+            const int r = 2 * rr;
+            const float sc = scl[rr];
             const float2 yn = make_float2((e[r] + o[r]) * sc, (e[r + 1] + o[r + 1]) * sc);
             const float2 ys = make_float2((e[r] - o[r]) * sc, (e[r + 1] - o[r + 1]) * sc);
-            float2* xr = X + ((size_t)f * N + row) * N;
-            xr[j] = yn;
-            xr[N - 1 - j] = ys;
-            if (fmt == 1 && m > 0) {
-                float2* xm = X + ((size_t)f * N + (N - m)) * N;
-                xm[j] = make_float2(yn.x, -yn.y);
-                xm[N - 1 - j] = make_float2(ys.x, -ys.y);
+            plo[rr][16 * t] = yn;
+            phi[rr][-16 * t] = ys;
+            if (mirror) {
+                qlo[rr][16 * t] = make_float2(yn.x, -yn.y);
+                qhi[rr][-16 * t] = make_float2(ys.x, -ys.y);
             }
         }
     }
@@ -497,7 +535,7 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
     float2* X = (float2*)Xv;
     {
         Span sp(tm, ST_LEG_INV, s);
-        const int gj = std::min(4, B / 64);
+        const int gj = std::min(S2_MIX_GJMAX, B / 64);
         const int nb = (4 * F + kMixCols - 1) / kMixCols;
         const dim3 grid(B * nb * (B / (64 * gj)));
         const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * 64 * gj);
