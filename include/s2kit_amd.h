@@ -40,7 +40,7 @@ typedef enum {
     S2AMD_FP32 = 1,
     /* mixed: FP32 ring FFTs and the naive Legendre transform (naive.c) as an
      * FP32-MFMA GEMM over m on tables from the FP64 recurrence; FP64 in/out.
-     * Error ~1e-7 relative.  bw a power of two in [64, 512]. */
+     * Error ~4e-7 relative.  bw a power of two in [64, 512]. */
     S2AMD_MIXED = 2
 } s2amd_precision;
 
@@ -50,6 +50,7 @@ typedef struct s2amd_plan s2amd_plan;
  * seminaive for orders < cutoff (cutoff = bw: all orders).  Generates the
  * forward and transposed tables on the current device. */
 int s2amd_plan_create(s2amd_plan** plan, int bw, int cutoff, int max_fields, int precision);
+/* Waits for the device (work queued on any stream may use the plan), then frees. */
 int s2amd_plan_destroy(s2amd_plan* plan);
 /* Device bytes held by the plan (tables + scratch). */
 size_t s2amd_plan_bytes(const s2amd_plan* plan);

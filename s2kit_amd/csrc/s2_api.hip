@@ -768,6 +768,9 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
 extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
     return run_plan("s2amd_plan_destroy", [&] {
         if (!p) return;
+        // work queued on the caller's streams may still read the plan's buffers
+        S2_HIP(hipSetDevice(p->device));
+        S2_HIP(hipDeviceSynchronize());
         consts_destroy(p->c);
         tables_destroy(p->t);
         gemm_tables_destroy(p->g);

@@ -494,10 +494,45 @@ static size_t mix_fwd_lds(int B) {
     return mx;
 }
 
+static void mix_fwd_gemm(const DevConsts& c, const GemmTables& g, const float2* X, double* cre, double* cim, int F,
+                         Fmt fmt, hipStream_t s) {
+    const int B = c.bw;
+    const int nb = (4 * F + kMixCols - 1) / kMixCols;
+    const dim3 grid(B * nb);
+    const size_t lds = mix_fwd_lds(B);
+    const float* nf = (const float*)g.af;
+    const GemmMeta* meta = g.meta;
+    const int n0max = mix_n0(0, B);
+    if (n0max <= 2)
+        launch_lds(k_mix_fwd<2>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+    else if (n0max <= 4)
+        launch_lds(k_mix_fwd<4>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+    else if (n0max <= 8)
+        launch_lds(k_mix_fwd<8>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+    else
+        launch_lds(k_mix_fwd<16>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+}
+
+static void mix_inv_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, float2* X,
+                         int F, Fmt fmt, hipStream_t s) {
+    const int B = c.bw;
+    const int gj = std::min(S2_MIX_GJMAX, B / 64);
+    const int nb = (4 * F + kMixCols - 1) / kMixCols;
+    const dim3 grid(B * nb * (B / (64 * gj)));
+    const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * 64 * gj);
+    const float* ni = (const float*)g.ai;
+    const GemmMeta* meta = g.meta;
+    switch (gj) {
+        case 1: launch_lds(k_mix_inv<1>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
+        case 2: launch_lds(k_mix_inv<2>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
+        default: launch_lds(k_mix_inv<4>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
+    }
+}
+
 void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
                    double* cim, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
     constexpr int V = 16;
-    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int N = c.N, lgN = lg2(N);
     const int rpb = V * kThreads / N;
     float2* X = (float2*)Xv;
     {
@@ -507,45 +542,19 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
     }
     {
         Span sp(tm, ST_LEG_FWD, s);
-        const int nb = (4 * F + kMixCols - 1) / kMixCols;
-        const dim3 grid(B * nb);
-        const size_t lds = mix_fwd_lds(B);
-        const float* nf = (const float*)g.af;
-        const int n0max = mix_n0(0, B);
-        if (n0max <= 2)
-            launch_lds(k_mix_fwd<2>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre, cim,
-                       lg2(B), F, (int)fmt, nb);
-        else if (n0max <= 4)
-            launch_lds(k_mix_fwd<4>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre, cim,
-                       lg2(B), F, (int)fmt, nb);
-        else if (n0max <= 8)
-            launch_lds(k_mix_fwd<8>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre, cim,
-                       lg2(B), F, (int)fmt, nb);
-        else
-            launch_lds(k_mix_fwd<16>, grid, kMixThreads, lds, s, (const float2*)X, nf, (const GemmMeta*)g.meta, cre,
-                       cim, lg2(B), F, (int)fmt, nb);
+        mix_fwd_gemm(c, g, X, cre, cim, F, fmt, s);
     }
 }
 
 void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
                    double* im, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
     constexpr int V = 16;
-    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int N = c.N, lgN = lg2(N);
     const int rpb = V * kThreads / N;
     float2* X = (float2*)Xv;
     {
         Span sp(tm, ST_LEG_INV, s);
-        const int gj = std::min(S2_MIX_GJMAX, B / 64);
-        const int nb = (4 * F + kMixCols - 1) / kMixCols;
-        const dim3 grid(B * nb * (B / (64 * gj)));
-        const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * 64 * gj);
-        const float* ni = (const float*)g.ai;
-        const GemmMeta* meta = g.meta;
-        switch (gj) {
-            case 1: launch_lds(k_mix_inv<1>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
-            case 2: launch_lds(k_mix_inv<2>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
-            default: launch_lds(k_mix_inv<4>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
-        }
+        mix_inv_gemm(c, g, cre, cim, X, F, fmt, s);
     }
     {
         Span sp(tm, ST_PHI_INV, s);
