@@ -52,6 +52,31 @@ def stage_bytes(stage, bw, nfields, elem, form):
     return per_field * nfields
 
 
+# kernel behind each timed stage, per form (names as rocprofv3 reports them)
+KERNELS = {
+    "mixed": {"phi_fwd": "k_fphi_fwd<float, 16>", "leg_fwd": "k_mix_fwd", "leg_inv": "k_mix_inv",
+              "phi_inv": "k_fphi_inv<float, 16>"},
+    "fp32": {"phi_fwd": "k_fphi_fwd<float, 16>", "dct_fwd": "k_fdct_fwd<float, 16>", "leg_fwd": "k_gemm_fwd<float>",
+             "leg_inv": "k_gemm_inv<float>", "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16>"},
+    "fp64": {"phi_fwd": "k_fphi_fwd<double, 8>", "dct_fwd": "k_fdct_fwd<double, 8>", "leg_fwd": "k_gemm_fwd<double>",
+             "leg_inv": "k_gemm_inv<double>", "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8>"},
+}
+
+
+def measured_traffic(form, bw, nfields, stage):
+    """HBM bytes per launch of `stage`'s kernel from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each
+    scaled by a plain copy's reading), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{form}_bw{bw}_f{nfields}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == stage:
+            return float(f[2])
+    return None
+
+
 def cpu_worker(args):
     """One CPU process: `pairs` oracle inverse+forward transforms at bw."""
     bw, pairs, seed = args
@@ -203,12 +228,14 @@ def main():
     achieved = stage_bytes(dom, bw, F, elem, args.precision) / avg_s
     roofline = {
         "bound": "hbm",
-        "kernel": dom,
+        "kernel": KERNELS[args.precision].get(dom, dom),
+        "stage": dom,
         "achieved": achieved / 1e9,
         "peak": HBM_PEAK / 1e9,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK,
-        "traffic": None,
+        "traffic": measured_traffic(args.precision, bw, F, dom),
+        "algorithmic_bytes": stage_bytes(dom, bw, F, elem, args.precision),
         "avg_launch_ms": dom_ms / dom_n,
         "stages_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
     }
