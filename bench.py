@@ -17,13 +17,18 @@ Also reported on the same JSON line:
   cpu_baseline  the CPU FP64 oracle (a port of the reference path) timed on
               the host, one process per core, on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+With --workload conv: an uneven-bandwidth batch of 4096 convolutions (one
+recorded HIP graph per form), mixed against FP64, errors per bandwidth.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sht|conv]
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -77,6 +82,153 @@ def measured_traffic(form, bw, nfields, stage):
     return None
 
 
+def table_doubles(bw):
+    """Doubles in one direction's table at cutoff = bw: sum over m of
+    TableSize(m, bw) (cospml.c:39-59)."""
+    tot = 0
+    for m in range(bw):
+        mm, ll = (m - 1, bw - 1) if m & 1 else (m, bw)
+        hl, hm = ll // 2, mm // 2
+        off = hl * (hl + 1) - hm * (hm + 1)
+        if ll & 1:
+            off += hl + 1
+        tot += off
+    return tot
+
+
+def single_field_bw512(dev, pairs):
+    """One power-law field at bw=512 (1024x1024 grid): the FP64 form called
+    per field streams both FP64 tables from HBM on every pair; the FP32 and
+    mixed forms are compared with it on the same field."""
+    import torch
+    import s2kit_amd as S
+    from s2kit_amd import synth
+    bw, n = 512, 1024
+    c0r, c0i = synth.power_law_coeffs(bw, 1, 99, dev)
+    gre = torch.empty((1, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    p64 = S.Plan(bw, 1, precision=S.FP64)
+    p64.inverse(c0r, c0i, gre, gim)
+    cr, ci = torch.empty_like(c0r), torch.empty_like(c0i)
+    for _ in range(2):
+        p64.forward(gre, gim, cr, ci)
+        p64.inverse(cr, ci, gre, gim)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(pairs):
+        p64.forward(gre, gim, cr, ci)
+        p64.inverse(cr, ci, gre, gim)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / pairs
+    tbytes = 2 * 8 * table_doubles(bw)  # forward + transposed table, FP64
+    out = {"bw": bw, "fp64_pair_ms": dt * 1e3, "table_bytes_per_pair": tbytes,
+           "table_stream_GBps": tbytes / dt / 1e9, "errors_vs_fp64": {}}
+    p64.inverse(c0r, c0i, gre, gim)
+    ref_r, ref_i = torch.empty_like(c0r), torch.empty_like(c0i)
+    p64.forward(gre, gim, ref_r, ref_i)
+    for name, prec in (("fp32", S.FP32), ("mixed", S.MIXED)):
+        p = S.Plan(bw, 1, precision=prec)
+        r, i = torch.empty_like(c0r), torch.empty_like(c0i)
+        g2r, g2i = torch.empty_like(gre), torch.empty_like(gim)
+        p.forward(gre, gim, r, i)
+        p.inverse(c0r, c0i, g2r, g2i)
+        torch.cuda.synchronize()
+        fe = float((torch.complex(r, i) - torch.complex(ref_r, ref_i)).abs().max() /
+                   torch.complex(ref_r, ref_i).abs().max())
+        ie = float((torch.complex(g2r, g2i) - torch.complex(gre, gim)).abs().max() / torch.complex(gre, gim).abs().max())
+        out["errors_vs_fp64"][name] = {"forward": fe, "inverse": ie}
+        p.close()
+    p64.close()
+    return out
+
+
+def conv_uneven(args):
+    """Uneven-bandwidth convolution batch: 4096 ConvOn2SphereSemiMemo requests
+    with bw drawn from {32, 64, 128, 256, 512} (p = .3, .3, .2, .15, .05),
+    each a uniform[-1, 1] coefficient signal and its own zonal Gaussian filter
+    h_l = exp(-l(l+1)/(2 sigma^2)), sigma ~ U[5, bw/2].  The whole batch is one
+    recorded HIP graph (s2amd_conv_batch_*); a step is one replay.  Mixed and
+    FP64 batches run on the same inputs; errors are per bandwidth, mixed
+    against FP64, normwise per request."""
+    import torch
+    import s2kit_amd as S
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(2024)
+    nreq = args.conv_requests
+    bws = rng.choice([32, 64, 128, 256, 512], size=nreq, p=[0.3, 0.3, 0.2, 0.15, 0.05])
+    data = {}  # bw -> dict of [count, N, N] tensors (requests of one bw back to back, in request order)
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    for bw in sorted(set(bws.tolist())):
+        cnt = int((bws == bw).sum())
+        n = 2 * bw
+        t = {k: torch.empty((cnt, n, n), dtype=torch.float64, device=dev)
+             for k in ("sre", "sim", "hre", "him", "o64r", "o64i", "omxr", "omxi")}
+        cr = torch.rand((cnt, bw * bw), generator=g, dtype=torch.float64, device=dev) * 2 - 1
+        ci = torch.rand((cnt, bw * bw), generator=g, dtype=torch.float64, device=dev) * 2 - 1
+        p = S.Plan(bw, cnt, precision=S.FP64)
+        p.inverse(cr, ci, t["sre"], t["sim"])
+        sig = torch.tensor(rng.uniform(5, bw / 2, cnt), dtype=torch.float64, device=dev)
+        l = torch.arange(bw, dtype=torch.float64, device=dev)
+        hc = torch.zeros((cnt, bw * bw), dtype=torch.float64, device=dev)
+        hc[:, :bw] = torch.exp(-l[None, :] * (l[None, :] + 1) / (2 * sig[:, None] ** 2))
+        p.inverse(hc, torch.zeros_like(hc), t["hre"], t["him"])
+        torch.cuda.synchronize()
+        p.close()
+        data[bw] = t
+    slot = {bw: 0 for bw in data}
+    req64, reqmx = [], []
+    for bw in bws.tolist():
+        j = slot[bw]
+        slot[bw] += 1
+        t = data[bw]
+        req64.append((bw, t["sre"][j], t["sim"][j], t["hre"][j], t["him"][j], t["o64r"][j], t["o64i"][j]))
+        reqmx.append((bw, t["sre"][j], t["sim"][j], t["hre"][j], t["him"][j], t["omxr"][j], t["omxi"][j]))
+    out = {}
+    for name, reqs, prec in (("fp64", req64, S.FP64), ("mixed", reqmx, S.MIXED)):
+        b = S.ConvBatch(reqs, precision=prec)
+        for _ in range(args.warmup):
+            b.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        out[name] = {"requests_per_s": nreq / dt, "ms_per_batch": dt * 1e3, "groups": b.groups()}
+        b.close()
+    errs = {}
+    for bw, t in data.items():
+        d = torch.complex(t["omxr"], t["omxi"]) - torch.complex(t["o64r"], t["o64i"])
+        ref = torch.complex(t["o64r"], t["o64i"]).abs().amax(dim=(1, 2))
+        errs[str(bw)] = {"requests": int(t["sre"].shape[0]),
+                         "max_rel_err_mixed_vs_fp64": float((d.abs().amax(dim=(1, 2)) / ref).max())}
+    line = {
+        "metric": "uneven-bandwidth conv requests/s (bw in {32..512}, 4096 requests): mixed vs FP64",
+        "value": out["mixed"]["requests_per_s"],
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": out["mixed"]["ms_per_batch"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{nreq} ConvOn2SphereSemiMemo requests, bw from {{32,64,128,256,512}} "
+                               "p={.3,.3,.2,.15,.05}, uniform signals, zonal Gaussian filters, one HIP graph",
+                   "requests": nreq, "parallelism": "single GPU"},
+        "fp64_requests_per_s": out["fp64"]["requests_per_s"],
+        "speedup_mixed_vs_fp64": out["mixed"]["requests_per_s"] / out["fp64"]["requests_per_s"],
+        "groups": {"mixed": out["mixed"]["groups"], "fp64": out["fp64"]["groups"]},
+        "errors_per_bw": errs,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_worker(args):
     """One CPU process: `pairs` oracle inverse+forward transforms at bw."""
     bw, pairs, seed = args
@@ -120,6 +272,11 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
     ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
+    ap.add_argument("--single-pairs", type=int, default=10,
+                    help="FP64 pairs timed on one bw=512 field (table streaming); 0: skip")
+    ap.add_argument("--workload", choices=["sht", "conv"], default="sht",
+                    help="sht: batched fwd+inv SHT (default); conv: uneven-bandwidth convolution batch")
+    ap.add_argument("--conv-requests", type=int, default=4096, help="requests in the conv workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device copy (reference for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -128,6 +285,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(fields would shard across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "conv":
+        conv_uneven(args)
+        return
 
     bw, F = args.bw, args.fields
     cpu = None
@@ -220,6 +381,9 @@ def main():
         plain = pf / (time.perf_counter() - t1)
         p1.close()
 
+    # ---- one field at bw=512: table streaming and per-form error -------------
+    single = single_field_bw512(dev, args.single_pairs) if args.single_pairs > 0 else None
+
     # ---- roofline of the dominant kernel ----------------------------------
     elem = 16 if prec == S.FP64 else 8  # bytes per complex intermediate value
     dom = max(stages, key=lambda k: stages[k][0])
@@ -259,6 +423,7 @@ def main():
         "errors": errors,
         "fp64_batch1_pairs_per_s": plain,
         "speedup_vs_fp64_batch1": (value / plain) if plain else None,
+        "single_field_bw512": single,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

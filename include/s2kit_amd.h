@@ -71,6 +71,38 @@ int s2amd_zonal(s2amd_plan* plan, const double* re, const double* im, double* zr
 int s2amd_conv(s2amd_plan* plan, const double* sre, const double* sim, const double* hre, const double* him,
                double* ore, double* oim, int nfields, void* stream);
 
+/* ---- convolution batches of uneven bandwidth --------------------------------
+ * Many ConvOn2SphereSemiMemo requests, each at its own bandwidth, on device
+ * grids.  _create groups the requests by bandwidth (one plan per distinct
+ * bandwidth, max_fields = the group's size), and records the whole batch --
+ * gathering each group's grids into contiguous staging buffers (skipped for
+ * an array whose requests already lie back to back in request order), the
+ * group's forward / zonal / TransMult / inverse pipeline, scattering the
+ * outputs -- as one HIP graph; _run replays it on a stream.  Requests are
+ * copied at creation: their buffers must stay allocated while the batch is
+ * used; their contents are read at every _run.  precision: S2AMD_FP64 or
+ * S2AMD_MIXED; a group whose bandwidth the mixed form does not support (not a
+ * power of two in [64, 512]) runs in FP64.  No counterpart in the reference,
+ * which convolves one pair of host grids per call (FST_semi_memo.c:439-518). */
+typedef struct {
+    int bw;
+    const double* sre; /* signal grid [2bw][2bw], re and im */
+    const double* sim;
+    const double* hre; /* filter grid */
+    const double* him;
+    double* ore; /* output grid */
+    double* oim;
+} s2amd_conv_request;
+
+typedef struct s2amd_conv_batch s2amd_conv_batch;
+int s2amd_conv_batch_create(s2amd_conv_batch** batch, const s2amd_conv_request* req, int n, int precision,
+                            void* stream);
+int s2amd_conv_batch_run(s2amd_conv_batch* batch, void* stream);
+/* Groups in ascending bandwidth: their bandwidths, request counts and the
+ * precision each runs in (arrays of at least *ngroups entries, or NULL). */
+int s2amd_conv_batch_info(const s2amd_conv_batch* batch, int* ngroups, int* bws, int* counts, int* precisions);
+int s2amd_conv_batch_destroy(s2amd_conv_batch* batch);
+
 /* Per-stage kernel timing: when enabled, every launch of the plan is
  * bracketed by HIP events recorded on the call's stream.  stage_times waits
  * for the recorded events, returns the summed milliseconds and launch counts

@@ -65,6 +65,11 @@ SIGNATURES = {
     "s2amd_inverse": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "s2amd_zonal": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "s2amd_conv": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "s2amd_conv_batch_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _i, _vp]),
+    "s2amd_conv_batch_run": (_i, [_vp, _vp]),
+    "s2amd_conv_batch_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                   ctypes.POINTER(_i)]),
+    "s2amd_conv_batch_destroy": (_i, [_vp]),
     "s2amd_plan_timing": (_i, [_vp, _i]),
     "s2amd_plan_stage_times": (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long)]),
     "s2amd_stage_name": (ctypes.c_char_p, [_i]),
@@ -292,3 +297,53 @@ class Plan:
         n = sre.shape[0] if nfields is None else nfields
         self._check(lib().s2amd_conv(self.h, _ptr(sre), _ptr(sim), _ptr(hre), _ptr(him), _ptr(ore), _ptr(oim), n,
                                      self._stream(stream)))
+
+
+class ConvRequest(ctypes.Structure):
+    """s2amd_conv_request: one ConvOn2SphereSemiMemo on device grids [2bw, 2bw]."""
+    _fields_ = [("bw", ctypes.c_int), ("sre", ctypes.c_void_p), ("sim", ctypes.c_void_p),
+                ("hre", ctypes.c_void_p), ("him", ctypes.c_void_p), ("ore", ctypes.c_void_p),
+                ("oim", ctypes.c_void_p)]
+
+
+class ConvBatch:
+    """Convolutions of uneven bandwidth recorded once as a HIP graph.
+
+    ``requests`` is a list of (bw, sre, sim, hre, him, ore, oim) with torch
+    float64 CUDA tensors of shape [2bw, 2bw] (or any tensors holding 4bw^2
+    values).  The tensors must stay alive while the batch is used; their
+    contents are read at every run()."""
+
+    def __init__(self, requests, precision=MIXED):
+        arr = (ConvRequest * len(requests))()
+        for k, (bw, *ts) in enumerate(requests):
+            arr[k] = ConvRequest(bw, *[t.data_ptr() for t in ts])
+        h = ctypes.c_void_p()
+        rc = lib().s2amd_conv_batch_create(ctypes.byref(h), ctypes.cast(arr, ctypes.c_void_p), len(requests),
+                                           precision, None)
+        if rc != 0:
+            raise S2Error(last_error())
+        self.h = h
+        self._keep = [r[1:] for r in requests]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().s2amd_conv_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def run(self, stream=None):
+        rc = lib().s2amd_conv_batch_run(self.h, Plan._stream(stream))
+        if rc != 0:
+            raise S2Error(last_error())
+
+    def groups(self):
+        """[(bw, count, precision)] in ascending bandwidth."""
+        n = ctypes.c_int()
+        lib().s2amd_conv_batch_info(self.h, ctypes.byref(n), None, None, None)
+        bws, cnt, prec = (ctypes.c_int * n.value)(), (ctypes.c_int * n.value)(), (ctypes.c_int * n.value)()
+        rc = lib().s2amd_conv_batch_info(self.h, ctypes.byref(n), bws, cnt, prec)
+        if rc != 0:
+            raise S2Error(last_error())
+        return [(bws[i], cnt[i], prec[i]) for i in range(n.value)]

@@ -303,7 +303,7 @@ static void plan_inverse(Plan* p, const double* cre, const double* cim, double* 
         inverse_fp64(p->c, p->t, cre, cim, re, im, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
 }
 
-static int plan_status(const char* fn, const std::exception_ptr& ep) {
+int api_status(const char* fn, const std::exception_ptr& ep) {
     try {
         std::rethrow_exception(ep);
     } catch (const ArgError& e) {
@@ -327,7 +327,7 @@ static int run_plan(const char* fn, F&& body) {
         body();
         return S2AMD_OK;
     } catch (...) {
-        return plan_status(fn, std::current_exception());
+        return api_status(fn, std::current_exception());
     }
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,19 @@ struct HipError {
 struct ArgError {
     std::string msg;
 };
+
+// Map the exception escaping an API body to an S2AMD_* code, recording the
+// message for s2amd_last_error().
+int api_status(const char* fn, const std::exception_ptr& ep);
+template <typename F>
+int run_api(const char* fn, F&& body) {
+    try {
+        body();
+        return 0;
+    } catch (...) {
+        return api_status(fn, std::current_exception());
+    }
+}
 
 // ---- device tables of one (bw, cutoff) -------------------------------------
 // Order m's forward table starts at fwd + off[m], its transposed table at

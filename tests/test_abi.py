@@ -105,3 +105,27 @@ def test_plan_rejects_bad_arguments(S):
     assert L.s2amd_forward(None, None, None, None, None, 1, 0, None) == -1
     assert "plan is NULL" in S.last_error()
     assert L.s2amd_plan_destroy(None) == 0
+
+
+def test_conv_batch_rejects_bad_arguments_before_touching_the_device(S):
+    L = S.lib()
+    h = ctypes.c_void_p()
+    req = (S.ConvRequest * 1)()
+    req[0] = S.ConvRequest(8, 256, 256, 256, 256, 256, 256)  # aligned dummy addresses, never dereferenced
+    p = ctypes.cast(req, ctypes.c_void_p)
+    assert L.s2amd_conv_batch_create(ctypes.byref(h), p, 0, 0, None) == -1
+    assert "at least one request" in S.last_error()
+    assert L.s2amd_conv_batch_create(ctypes.byref(h), p, 1, 1, None) == -1
+    assert "precision" in S.last_error()
+    req[0].bw = 2000
+    assert L.s2amd_conv_batch_create(ctypes.byref(h), p, 1, 0, None) == -1
+    assert "bw must be in" in S.last_error()
+    req[0].bw = 8
+    req[0].ore = None
+    assert L.s2amd_conv_batch_create(ctypes.byref(h), p, 1, 0, None) == -1
+    assert "NULL" in S.last_error()
+    req[0].ore = 260
+    assert L.s2amd_conv_batch_create(ctypes.byref(h), p, 1, 0, None) == -1
+    assert "aligned" in S.last_error()
+    assert L.s2amd_conv_batch_run(None, None) == -1
+    assert L.s2amd_conv_batch_destroy(None) == 0
