@@ -85,9 +85,10 @@ __device__ __forceinline__ void dftR(T* x) {
         dft2<SIGN>(x);
 }
 
-// One Stockham pass of radix R over the E = V*kThreads values of the block
-// (E/N transforms of length N, row stride S).  Sub-transform size Ns = 2^lgNs.
-template <typename T, int SIGN, int V, int R, int LGR>
+// One Stockham pass of radix R over the E = V*NTH values of the block of NTH
+// threads (E/N transforms of length N, row stride S).  Sub-transform size
+// Ns = 2^lgNs.
+template <typename T, int SIGN, int V, int R, int LGR, int NTH>
 __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T* tw) {
     constexpr int PER = V / R;
     const int lgNR = lgN - LGR;
@@ -96,7 +97,7 @@ __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T
     T x[V];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int b = threadIdx.x + u * kThreads;
+        const int b = threadIdx.x + u * NTH;
         const int f = b >> lgNR, j = b & (NR - 1);
         const T* src = a + f * S + j;
 #pragma unroll
@@ -105,7 +106,7 @@ __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int b = threadIdx.x + u * kThreads;
+        const int b = threadIdx.x + u * NTH;
         const int f = b >> lgNR, j = b & (NR - 1);
         const int k = j & (Ns - 1);
         const int step = k << (lgNR - lgNs);  // k * N / (R * Ns)
@@ -126,18 +127,18 @@ __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T
 }
 
 // In-place FFTs of all E/N rows of the block's LDS tile (natural order in and out).
-template <typename T, int SIGN, int V>
+template <typename T, int SIGN, int V, int NTH = kThreads>
 __device__ __forceinline__ void fft_block(T* a, int lgN, int S, const T* tw) {
     int lgNs = 0, rem = lgN;
     while (rem >= 3) {
-        fft_pass<T, SIGN, V, 8, 3>(a, lgN, S, lgNs, tw);
+        fft_pass<T, SIGN, V, 8, 3, NTH>(a, lgN, S, lgNs, tw);
         lgNs += 3;
         rem -= 3;
     }
     if (rem == 2)
-        fft_pass<T, SIGN, V, 4, 2>(a, lgN, S, lgNs, tw);
+        fft_pass<T, SIGN, V, 4, 2, NTH>(a, lgN, S, lgNs, tw);
     else if (rem == 1)
-        fft_pass<T, SIGN, V, 2, 1>(a, lgN, S, lgNs, tw);
+        fft_pass<T, SIGN, V, 2, 1, NTH>(a, lgN, S, lgNs, tw);
 }
 
 __device__ __forceinline__ int frow_of(int q, int N, int cut) { return q < cut ? q : N - 2 * cut + 1 + q; }
@@ -174,25 +175,26 @@ __device__ __forceinline__ int cpos(int k, int half) { return (k & 1) ? half + (
 
 // Copy the N twiddles exp(-2 pi i t/N) into LDS (they are read by every
 // butterfly of every pass).
-template <typename T>
+template <typename T, int NTH = kThreads>
 __device__ __forceinline__ T* stage_twiddles(unsigned char* smem, const T* __restrict__ tw, int N) {
     T* t = reinterpret_cast<T*>(smem);
-    for (int i = threadIdx.x; i < N; i += kThreads) t[i] = tw[i];
+    for (int i = threadIdx.x; i < N; i += NTH) t[i] = tw[i];
     return t;
 }
 
 // ---------------------------------------------------------------------------
-// FFT-type kernels.  grid.y = field, block = kThreads, LDS = rows*(N+1) values
+// FFT-type kernels.  grid.y = field, block = NTH (default kThreads) threads,
+// LDS = N twiddles + rows*(N+1) values
 // ---------------------------------------------------------------------------
-template <typename R, int V>
-__global__ __launch_bounds__(kThreads) void k_fphi_fwd(const double* __restrict__ re, const double* __restrict__ im,
+template <typename R, int V, int NTH = kThreads>
+__global__ __launch_bounds__(NTH) void k_fphi_fwd(const double* __restrict__ re, const double* __restrict__ im,
                                                        typename Cplx<R>::type* __restrict__ X, int lgN,
                                                        const typename Cplx<R>::type* __restrict__ tw) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int E = V * kThreads;
+    constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1;
-    const T* twl = stage_twiddles(smem, tw, N);
+    const T* twl = stage_twiddles<T, NTH>(smem, tw, N);
     T* a = reinterpret_cast<T*>(smem) + N;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
@@ -203,27 +205,27 @@ __global__ __launch_bounds__(kThreads) void k_fphi_fwd(const double* __restrict_
         double vr[V], vi[V];
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            vr[u] = re[base + threadIdx.x + u * kThreads];
-            vi[u] = im[base + threadIdx.x + u * kThreads];
+            vr[u] = re[base + threadIdx.x + u * NTH];
+            vi[u] = im[base + threadIdx.x + u * NTH];
         }
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * kThreads;
+            const int idx = threadIdx.x + u * NTH;
             a[(idx >> lgN) * S + (idx & (N - 1))] = mk<T>((R)vr[u], (R)vi[u]);
         }
     }
     __syncthreads();
-    fft_block<T, -1, V>(a, lgN, S, twl);
+    fft_block<T, -1, V, NTH>(a, lgN, S, twl);
     {
         T v[V];
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * kThreads;
+            const int idx = threadIdx.x + u * NTH;
             v[u] = a[(idx & (rpb - 1)) * S + (idx >> lgrpb)];
         }
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * kThreads;
+            const int idx = threadIdx.x + u * NTH;
             X[((size_t)f * N + (idx >> lgrpb)) * N + j0 + (idx & (rpb - 1))] = v[u];
         }
     }
@@ -324,15 +326,15 @@ __global__ __launch_bounds__(kThreads) void k_fdct_inv(const R* __restrict__ Cb,
     }
 }
 
-template <typename R, int V>
-__global__ __launch_bounds__(kThreads) void k_fphi_inv(const typename Cplx<R>::type* __restrict__ X,
+template <typename R, int V, int NTH = kThreads>
+__global__ __launch_bounds__(NTH) void k_fphi_inv(const typename Cplx<R>::type* __restrict__ X,
                                                        double* __restrict__ re, double* __restrict__ im, int lgN,
                                                        const typename Cplx<R>::type* __restrict__ tw) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int E = V * kThreads;
+    constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1, B = N >> 1;
-    const T* twl = stage_twiddles(smem, tw, N);
+    const T* twl = stage_twiddles<T, NTH>(smem, tw, N);
     T* a = reinterpret_cast<T*>(smem) + N;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
@@ -342,30 +344,30 @@ __global__ __launch_bounds__(kThreads) void k_fphi_inv(const typename Cplx<R>::t
         T v[V];
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * kThreads;
+            const int idx = threadIdx.x + u * NTH;
             const int r = idx >> lgrpb;
             v[u] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + (idx & (rpb - 1))];
         }
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * kThreads;
+            const int idx = threadIdx.x + u * NTH;
             a[(idx & (rpb - 1)) * S + (idx >> lgrpb)] = v[u];
         }
     }
     __syncthreads();
-    fft_block<T, +1, V>(a, lgN, S, twl);
+    fft_block<T, +1, V, NTH>(a, lgN, S, twl);
     const size_t base = ((size_t)f * N + j0) * N;
     {
         T v[V];
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * kThreads;
+            const int idx = threadIdx.x + u * NTH;
             v[u] = a[(idx >> lgN) * S + (idx & (N - 1))];
         }
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            re[base + threadIdx.x + u * kThreads] = (double)v[u].x;
-            im[base + threadIdx.x + u * kThreads] = (double)v[u].y;
+            re[base + threadIdx.x + u * NTH] = (double)v[u].x;
+            im[base + threadIdx.x + u * NTH] = (double)v[u].y;
         }
     }
 }

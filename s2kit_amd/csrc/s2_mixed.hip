@@ -45,6 +45,12 @@ typedef float fvec2 __attribute__((ext_vector_type(2)));
 #ifndef S2_MIX_GJMAX
 #define S2_MIX_GJMAX 4
 #endif
+#ifndef S2_MIX_FFT_THREADS
+#define S2_MIX_FFT_THREADS 512
+#endif
+// Ring-FFT blocks of the mixed form: 16 values per thread, so a block holds
+// 16 rings at bw=256 and stores full 128-byte row segments.
+constexpr int kMixFftThreads = S2_MIX_FFT_THREADS;
 constexpr int kMixThreads = 64 * S2_MIX_WAVES;
 constexpr int kMixCols = 16 * S2_MIX_WAVES;
 
@@ -533,11 +539,12 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
                    double* cim, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
     constexpr int V = 16;
     const int N = c.N, lgN = lg2(N);
-    const int rpb = V * kThreads / N;
+    const int rpb = V * kMixFftThreads / N;
     float2* X = (float2*)Xv;
     {
         Span sp(tm, ST_PHI_FWD, s);
-        launch_lds(k_fphi_fwd<float, V>, dim3(N / rpb, F), kThreads, sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
+        launch_lds(k_fphi_fwd<float, V, kMixFftThreads>, dim3(N / rpb, F), kMixFftThreads,
+                   sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
                    re, im, X, lgN, (const float2*)c.tw32);
     }
     {
@@ -550,7 +557,7 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
                    double* im, int F, Fmt fmt, void* Xv, hipStream_t s, StageTimer* tm) {
     constexpr int V = 16;
     const int N = c.N, lgN = lg2(N);
-    const int rpb = V * kThreads / N;
+    const int rpb = V * kMixFftThreads / N;
     float2* X = (float2*)Xv;
     {
         Span sp(tm, ST_LEG_INV, s);
@@ -558,7 +565,8 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
     }
     {
         Span sp(tm, ST_PHI_INV, s);
-        launch_lds(k_fphi_inv<float, V>, dim3(N / rpb, F), kThreads, sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
+        launch_lds(k_fphi_inv<float, V, kMixFftThreads>, dim3(N / rpb, F), kMixFftThreads,
+                   sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
                    (const float2*)X, re, im, lgN, (const float2*)c.tw32);
     }
 }
