@@ -313,7 +313,14 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
     const R* ws = (const R*)(sizeof(R) == 8 ? (const void*)c.w_semi : (const void*)c.w_semi32);
     {
         Span sp(tm, ST_PHI_FWD, s);
-        launch_lds(k_fphi_fwd<R, V>, dim3(N / rpb, F), kThreads, lds, s, re, im, X, lgN, tw);
+        // 512-thread blocks (twice the rings, full 128-byte row segments) when a block still holds whole rows
+        if ((long)N * N >= 512L * V) {
+            const int rp2 = V * 512 / N;
+            launch_lds(k_fphi_fwd<R, V, 512>, dim3(N / rp2, F), 512, sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, re,
+                       im, X, lgN, tw);
+        } else {
+            launch_lds(k_fphi_fwd<R, V>, dim3(N / rpb, F), kThreads, lds, s, re, im, X, lgN, tw);
+        }
     }
     const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
     {
@@ -360,7 +367,13 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     }
     {
         Span sp(tm, ST_PHI_INV, s);
-        launch_lds(k_fphi_inv<R, V>, dim3(N / rpb, F), kThreads, lds, s, (const T*)X, re, im, lgN, tw);
+        if ((long)N * N >= 512L * V) {
+            const int rp2 = V * 512 / N;
+            launch_lds(k_fphi_inv<R, V, 512>, dim3(N / rp2, F), 512, sizeof(T) * ((size_t)rp2 * (N + 1) + N), s,
+                       (const T*)X, re, im, lgN, tw);
+        } else {
+            launch_lds(k_fphi_inv<R, V>, dim3(N / rpb, F), kThreads, lds, s, (const T*)X, re, im, lgN, tw);
+        }
     }
 }
 
