@@ -107,15 +107,19 @@ __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restr
     }
 }
 
-// Workgroup id -> (order m, column block cb) over nb column blocks per order.
-// Ids go round-robin over the 8 XCDs (id mod 8); XCD k runs, in turn, the
-// column blocks of orders 8q+k and bw-1-(8q+k) interleaved.  bw % 16 == 0.
-__device__ __forceinline__ void mix_block(int B, int nb, int& m, int& cb) {
+// Workgroup id -> an order pair (m, bw-1-m) and a column block cb of nb per
+// order.  A workgroup runs both orders of its pair, so every workgroup has
+// about the same work (bw-m + m+1 degrees): blocks of very different lengths
+// would leave the dispatcher's round-robin placement unbalanced.  Ids go
+// round-robin over the 8 XCDs (id mod 8); XCD k takes the pairs m = 8q+k, all
+// column blocks of one pair back to back, so the pair's table tiles stay in
+// that XCD's L2.  bw % 16 == 0.
+__device__ __forceinline__ void mix_pair(int B, int nb, int& m, int& cb) {
     const int id = blockIdx.x, k = id & 7, local = id >> 3;
-    const int q = local / (2 * nb), r = local - q * 2 * nb;
-    cb = r >> 1;
-    const int ml = 8 * q + k;
-    m = (r & 1) ? B - 1 - ml : ml;
+    const int q = local / nb;
+    cb = local - q * nb;
+    m = 8 * q + k;  // < bw/2; its partner is bw-1-m
+    (void)B;
 }
 
 // ---------------------------------------------------------------------------
@@ -266,9 +270,11 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
     extern __shared__ __align__(16) unsigned char smem[];
     float* sm = reinterpret_cast<float*>(smem);
     const int B = 1 << lgB;
-    int m, cb;
-    mix_block(B, nb, m, cb);
-    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    int mp, cb;
+    mix_pair(B, nb, mp, cb);
+    for (int h = 0; h < 2; ++h) {
+        const int m = h ? B - 1 - mp : mp;
+        const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
 #define S2_MIX_FWD(n) \
     case n:           \
         if (n <= MAXN0) mix_fwd_body<(n <= MAXN0 ? n : 1)>(X, NF, g0, g1, cre, cim, m, cb, B, F, fmt, sm); \
@@ -293,6 +299,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
         default: break;
     }
 #undef S2_MIX_FWD
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -301,20 +308,16 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // A block owns kMixCols columns of one order and JW = 64*GJ of the bw folded rings
 // (jb selects which); chunks of KC degrees per parity.
 template <int GJ>
-__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_MIX_WPE_INV))) void k_mix_inv(const double* __restrict__ cre, const double* __restrict__ cim,
-                                                        const float* __restrict__ NI, const GemmMeta* __restrict__ meta,
-                                                        float2* __restrict__ X, int lgB, int F, int fmt, int nb) {
+__device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, const double* __restrict__ cim,
+                                             const float* __restrict__ NI, const GemmMeta* __restrict__ meta,
+                                             float2* __restrict__ X, int B, int F, int fmt, int m, int cz, float* sm) {
     constexpr int JW = 64 * GJ;
     constexpr int KC = kMixInvKC, S = KC / 4;
     constexpr int NV = 2 * KC * JW / 4;  // float4 per table chunk (both parities)
     constexpr int PT = NV / kMixThreads;
     static_assert(NV % kMixThreads == 0, "table chunk must split evenly over the block");
-    extern __shared__ __align__(16) unsigned char smem[];
-    float* sm = reinterpret_cast<float*>(smem);
-    const int B = 1 << lgB, N = 2 * B;
+    const int N = 2 * B;
     const int nz = B / JW;
-    int m, cz;
-    mix_block(B, nb * nz, m, cz);
     const int cb = cz / nz, jbase = (cz - cb * nz) * JW;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
@@ -452,6 +455,20 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
     }
 }
 
+template <int GJ>
+__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_MIX_WPE_INV))) void k_mix_inv(
+    const double* __restrict__ cre, const double* __restrict__ cim, const float* __restrict__ NI,
+    const GemmMeta* __restrict__ meta, float2* __restrict__ X, int lgB, int F, int fmt, int nb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sm = reinterpret_cast<float*>(smem);
+    const int B = 1 << lgB;
+    int mp, cz;
+    mix_pair(B, nb * (B / (64 * GJ)), mp, cz);
+    mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, mp, cz, sm);
+    __syncthreads();  // the partner order's prologue reuses the table buffers
+    mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, B - 1 - mp, cz, sm);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -504,7 +521,7 @@ static void mix_fwd_gemm(const DevConsts& c, const GemmTables& g, const float2* 
                          Fmt fmt, hipStream_t s) {
     const int B = c.bw;
     const int nb = (4 * F + kMixCols - 1) / kMixCols;
-    const dim3 grid(B * nb);
+    const dim3 grid(B / 2 * nb);  // order pairs x column blocks
     const size_t lds = mix_fwd_lds(B);
     const float* nf = (const float*)g.af;
     const GemmMeta* meta = g.meta;
@@ -524,7 +541,7 @@ static void mix_inv_gemm(const DevConsts& c, const GemmTables& g, const double* 
     const int B = c.bw;
     const int gj = std::min(S2_MIX_GJMAX, B / 64);
     const int nb = (4 * F + kMixCols - 1) / kMixCols;
-    const dim3 grid(B * nb * (B / (64 * gj)));
+    const dim3 grid(B / 2 * nb * (B / (64 * gj)));  // order pairs x column blocks x ring ranges
     const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * 64 * gj);
     const float* ni = (const float*)g.ai;
     const GemmMeta* meta = g.meta;
