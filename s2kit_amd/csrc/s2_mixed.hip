@@ -37,7 +37,7 @@ typedef float fvec2 __attribute__((ext_vector_type(2)));
 #define S2_MIX_WAVES 4
 #endif
 #ifndef S2_MIX_WPE_FWD
-#define S2_MIX_WPE_FWD 2
+#define S2_MIX_WPE_FWD 3
 #endif
 #ifndef S2_MIX_WPE_INV
 #define S2_MIX_WPE_INV 2
@@ -64,7 +64,13 @@ constexpr int kMixCols = 16 * S2_MIX_WAVES;
 //   NI(m,p) [i < ki(m)][j < bw]: P_{m+p+2i}^m(theta_j), rings interleaved in
 //         groups of 64 the same way; rows past the last degree are zero.
 __host__ __device__ inline int mix_n0(int m, int B) { return (gm_lp(m, 0, B) + 15) / 16; }
-__host__ __device__ inline int mix_ki(int m, int B) { return 16 * ((B - m + 31) / 32); }
+#ifndef S2_MIX_INV_KC
+#define S2_MIX_INV_KC 8
+#endif
+constexpr int kMixInvKC = S2_MIX_INV_KC;  // degrees per parity per inverse chunk
+__host__ __device__ inline int mix_ki(int m, int B) {
+    return kMixInvKC * ((B - m + 2 * kMixInvKC - 1) / (2 * kMixInvKC));
+}
 __host__ __device__ inline int mix_pos(int t, int x, int nt) {
     const int g4 = nt >> 2;
     return (t < 4 * g4) ? 64 * (t >> 2) + 4 * x + (t & 3) : 64 * g4 + 2 * x + (t & 1);
@@ -79,7 +85,6 @@ __host__ __device__ inline int ilv64(int j) { return (j & ~63) + 4 * (j & 15) + 
 #define S2_MIX_KC_BIG 16
 #endif
 __host__ __device__ constexpr int mix_kc(int n0) { return n0 <= 2 ? S2_MIX_KC_SMALL : S2_MIX_KC_BIG; }
-constexpr int kMixInvKC = 16;  // degrees per parity per inverse chunk
 
 __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restrict__ meta, float* __restrict__ NF,
                            float* __restrict__ NI, int B) {
