@@ -36,7 +36,8 @@ typedef enum {
      * Matches the reference to ~1e-13. */
     S2AMD_FP64 = 0,
     /* seminaive in FP32 (FFTs, DCTs, FP32-MFMA projections). Measured error
-     * 1e-6 .. 4e-5 relative: the cosine-space sums cancel. */
+     * 1e-6 .. 4e-5 relative: the cosine-space sums cancel.  bw a power of two
+     * in [32, 1024], cutoff = bw. */
     S2AMD_FP32 = 1,
     /* mixed: FP32 ring FFTs and the naive Legendre transform (naive.c) as an
      * FP32-MFMA GEMM over m on tables from the FP64 recurrence; FP64 in/out.

@@ -283,11 +283,20 @@ struct Plan {
     StageTimer timer;
 };
 
+// FP64 plans: fields per call from which the batched GEMM path beats the
+// per-field kernels (measured on MI355X: one table pass per call either way,
+// but the GEMM tiles only pay off once a few fields share them).
+static bool use_fast(const Plan* p, int F) {
+    if (!p->fast) return false;
+    if (p->precision != S2AMD_FP64) return true;
+    return F >= (p->bw >= 512 ? 6 : p->bw >= 256 ? 2 : 3);
+}
+
 static void plan_forward(Plan* p, const double* re, const double* im, double* cre, double* cim, int F, Fmt fmt,
                          hipStream_t s) {
     if (p->precision == S2AMD_MIXED)
         forward_mixed(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, s, &p->timer);
-    else if (p->fast)
+    else if (use_fast(p, F))
         forward_fast(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
     else
         forward_fp64(p->c, p->t, re, im, cre, cim, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
@@ -297,7 +306,7 @@ static void plan_inverse(Plan* p, const double* cre, const double* cim, double* 
                          hipStream_t s) {
     if (p->precision == S2AMD_MIXED)
         inverse_mixed(p->c, p->g, cre, cim, re, im, F, fmt, p->X.p, s, &p->timer);
-    else if (p->fast)
+    else if (use_fast(p, F))
         inverse_fast(p->c, p->g, cre, cim, re, im, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
     else
         inverse_fp64(p->c, p->t, cre, cim, re, im, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
@@ -725,7 +734,7 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
         if (precision == S2AMD_MIXED && !mixed_supported(bw))
             throw ArgError{"the mixed form needs a power-of-two grid with bw in [64, 512]"};
         if (precision == S2AMD_FP32 && !fast_supported(bw, cutoff, 1))
-            throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 512] and cutoff = bw"};
+            throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 1024] and cutoff = bw"};
         std::unique_ptr<s2amd_plan> p(new s2amd_plan());
         p->bw = bw;
         p->cutoff = cutoff;

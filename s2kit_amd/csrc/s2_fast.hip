@@ -280,10 +280,11 @@ size_t fast_cb_bytes(int bw, int prec) {
 }
 
 // Columns per GEMM block: the largest multiple of 16 (at most 64) whose
-// padded LDS tile fits in 64 KiB.
+// padded LDS tile fits in 64 KiB, or failing that in gfx950's 160 KiB.
 static int gemm_ncb(int B, int prec) {
     const int row = prec == 0 ? 8 * (B + lds_pad<double>()) : 4 * (B + lds_pad<float>());
-    return std::min(64, 65536 / row) & ~15;
+    const int small = std::min(64, 65536 / row) & ~15;
+    return small >= 16 ? small : (std::min(64, 163840 / row) & ~15);
 }
 
 static size_t gemm_lds(int B, int prec) {
@@ -332,9 +333,8 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
         const int ncb = gemm_ncb(B, g.prec);
         const int cols = 4 * F;
         Span sp(tm, ST_LEG_FWD, s);
-        k_gemm_fwd<R><<<dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s>>>(
-            Cb, (const R*)g.af, g.meta, cre, cim, lg2(B), F, ncb, (int)fmt);
-        S2_HIP(hipGetLastError());
+        launch_lds(k_gemm_fwd<R>, dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s, (const R*)Cb,
+                   (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, lg2(B), F, ncb, (int)fmt);
     }
 }
 
@@ -355,9 +355,8 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
         const int ncb = gemm_ncb(B, g.prec);
         const int cols = 4 * F;
         Span sp(tm, ST_LEG_INV, s);
-        k_gemm_inv<R><<<dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s>>>(
-            cre, cim, (const R*)g.ai, g.meta, Cb, lg2(B), F, ncb, (int)fmt);
-        S2_HIP(hipGetLastError());
+        launch_lds(k_gemm_inv<R>, dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s, cre, cim,
+                   (const R*)g.ai, (const GemmMeta*)g.meta, Cb, lg2(B), F, ncb, (int)fmt);
     }
     const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
     {

@@ -160,7 +160,8 @@ def test_full_size_round_trip_bw256(env, O, prec, tol):
     plan.close()
 
 
-@pytest.mark.parametrize("bw,prec,tol", [(512, 1, 1e-4), (512, 0, 1e-11), (512, 2, 1e-6)])
+@pytest.mark.parametrize("bw,prec,tol", [(512, 1, 1e-4), (512, 0, 1e-11), (512, 2, 1e-6), (1024, 0, 1e-11),
+                                         (1024, 1, 1e-3)])
 def test_large_bw_round_trip(env, bw, prec, tol):
     # Largest supported grids: inverse then forward on band-limited
     # power-law coefficients (size-independent identity; no oracle at this size).
