@@ -59,12 +59,12 @@ def stage_bytes(stage, bw, nfields, elem, form):
 
 # kernel behind each timed stage, per form (names as rocprofv3 reports them)
 KERNELS = {
-    "mixed": {"phi_fwd": "k_fphi_fwd<float, 16>", "leg_fwd": "k_mix_fwd", "leg_inv": "k_mix_inv",
-              "phi_inv": "k_fphi_inv<float, 16>"},
-    "fp32": {"phi_fwd": "k_fphi_fwd<float, 16>", "dct_fwd": "k_fdct_fwd<float, 16>", "leg_fwd": "k_gemm_fwd<float>",
-             "leg_inv": "k_gemm_inv<float>", "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16>"},
-    "fp64": {"phi_fwd": "k_fphi_fwd<double, 8>", "dct_fwd": "k_fdct_fwd<double, 8>", "leg_fwd": "k_gemm_fwd<double>",
-             "leg_inv": "k_gemm_inv<double>", "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8>"},
+    "mixed": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "leg_fwd": "k_mix_fwd", "leg_inv": "k_mix_inv",
+              "phi_inv": "k_fphi_inv<float, 16, 512>"},
+    "fp32": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "dct_fwd": "k_fdct_fwd<float, 16>", "leg_fwd": "k_gemm_fwd<float>",
+             "leg_inv": "k_gemm_inv<float>", "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16, 512>"},
+    "fp64": {"phi_fwd": "k_fphi_fwd<double, 8, 512>", "dct_fwd": "k_fdct_fwd<double, 8>", "leg_fwd": "k_gemm_fwd<double>",
+             "leg_inv": "k_gemm_inv<double>", "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8, 512>"},
 }
 
 
