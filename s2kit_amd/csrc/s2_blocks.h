@@ -231,96 +231,124 @@ __global__ __launch_bounds__(NTH) void k_fphi_fwd(const double* __restrict__ re,
     }
 }
 
-template <typename R, int V>
-__global__ __launch_bounds__(kThreads) void k_fdct_fwd(const typename Cplx<R>::type* __restrict__ X, R* __restrict__ Cb,
-                                                       const R* __restrict__ w_semi, int lgN, int cut, int nrows,
-                                                       const typename Cplx<R>::type* __restrict__ tw,
-                                                       const typename Cplx<R>::type* __restrict__ ph) {
+// Weighted DCT-II of each order row (the data of DLTSemi, seminaive.c:161-176),
+// by one N-point complex FFT of the even/odd reordered samples; one block
+// transforms rpb = V*NTH/N rows.  Loads and stores are batched per thread.
+template <typename R, int V, int NTH = kThreads>
+__global__ __launch_bounds__(NTH) void k_fdct_fwd(const typename Cplx<R>::type* __restrict__ X, R* __restrict__ Cb,
+                                                  const R* __restrict__ w_semi, int lgN, int cut, int nrows,
+                                                  const typename Cplx<R>::type* __restrict__ tw,
+                                                  const typename Cplx<R>::type* __restrict__ ph) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    const T* twl = stage_twiddles(smem, tw, 1 << lgN);
+    const T* twl = stage_twiddles<T, NTH>(smem, tw, 1 << lgN);
     T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
-    constexpr int E = V * kThreads;
+    constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
     xcd_block(bx, f);
     const int q0 = bx * rpb;
-    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
-        const int rl = idx >> lgN, j = idx & (N - 1);
-        const int q = q0 + rl;
-        const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
-        T x = mk<T>(0, 0);
-        if (q < nrows) {
-            const int r = frow_of(q, N, cut);
+    {
+        R xr[V], xi[V], ww[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {  // rows past nrows read the last row and weigh it by 0
+            const int idx = threadIdx.x + u * NTH;
+            const int q = q0 + (idx >> lgN), j = idx & (N - 1);
+            const int r = frow_of(min(q, nrows - 1), N, cut);
             const int am = r < B ? r : N - r;
-            x = cscale(X[((size_t)f * N + r) * N + j], w_semi[(am & 1) * N + j]);
+            const T xv = X[((size_t)f * N + r) * N + j];
+            xr[u] = xv.x;
+            xi[u] = xv.y;
+            ww[u] = q < nrows ? w_semi[(am & 1) * N + j] : (R)0;
         }
-        a[rl * S + p] = x;
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * NTH;
+            const int rl = idx >> lgN, j = idx & (N - 1);
+            const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+            a[rl * S + p] = mk<T>(xr[u] * ww[u], xi[u] * ww[u]);
+        }
     }
     __syncthreads();
-    fft_block<T, -1, V>(a, lgN, S, twl);
-    for (int idx = threadIdx.x; idx < rpb * B; idx += kThreads) {
+    fft_block<T, -1, V, NTH>(a, lgN, S, twl);
+#pragma unroll
+    for (int u = 0; u < V / 2; ++u) {  // rpb*B outputs: c[k] = e^{-i pi k/2N} v[k] + e^{+i pi k/2N} v[N-k]
+        const int idx = threadIdx.x + u * NTH;
         const int rl = idx >> (lgN - 1), k = idx & (B - 1);
         const int q = q0 + rl;
-        if (q >= nrows) continue;
-        const int r = frow_of(q, N, cut);
         const T* v = a + rl * S;
         const T e = ph[k];
         T c = cadd(cmul(e, v[k]), cmul(cconj(e), v[(N - k) & (N - 1)]));
         if (k == 0) c = cscale(c, (R)0.70710678118654752440);
-        R* row = Cb + ((size_t)f * N + r) * 2 * B;
-        const int pos = cpos(k, half);
-        row[pos] = c.x;
-        row[B + pos] = c.y;
+        if (q < nrows) {
+            R* row = Cb + ((size_t)f * N + frow_of(q, N, cut)) * 2 * B;
+            const int pos = cpos(k, half);
+            row[pos] = c.x;
+            row[B + pos] = c.y;
+        }
     }
 }
 
-template <typename R, int V>
-__global__ __launch_bounds__(kThreads) void k_fdct_inv(const R* __restrict__ Cb, typename Cplx<R>::type* __restrict__ X,
-                                                       const R* __restrict__ sinv, int lgN, int cut, int nrows,
-                                                       int fmt, const typename Cplx<R>::type* __restrict__ tw,
-                                                       const typename Cplx<R>::type* __restrict__ ph) {
+// DCT-III of each order row, x sin(theta) for odd orders, row signs and
+// 1/sqrt(2pi) (InvDLTSemi's tail, seminaive.c:107-114, and
+// FST_semi_memo.c:301-348): V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]), one
+// N-point inverse FFT, even/odd unshuffle.  REAL data also fills row N-r.
+template <typename R, int V, int NTH = kThreads>
+__global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, typename Cplx<R>::type* __restrict__ X,
+                                                  const R* __restrict__ sinv, int lgN, int cut, int nrows, int fmt,
+                                                  const typename Cplx<R>::type* __restrict__ tw,
+                                                  const typename Cplx<R>::type* __restrict__ ph) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    const T* twl = stage_twiddles(smem, tw, 1 << lgN);
+    const T* twl = stage_twiddles<T, NTH>(smem, tw, 1 << lgN);
     T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
-    constexpr int E = V * kThreads;
+    constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
     xcd_block(bx, f);
     const int q0 = bx * rpb;
-    // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]),  X[N] = 0, X[k >= B] = 0
-    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
-        const int rl = idx >> lgN, k = idx & (N - 1);
-        const int q = q0 + rl;
-        T v = mk<T>(0, 0);
-        if (q < nrows) {
-            const int r = frow_of(q, N, cut);
-            const R* row = Cb + ((size_t)f * N + r) * 2 * B;
-            T xk = mk<T>(0, 0), xn = mk<T>(0, 0);
-            if (k < B) xk = mk<T>(row[cpos(k, half)], row[B + cpos(k, half)]);
-            const int kn = N - k;
-            if (k > 0 && kn < B) xn = mk<T>(row[cpos(kn, half)], row[B + cpos(kn, half)]);
-            v = cmul(cconj(ph[k]), mk<T>(xk.x + xn.y, xk.y - xn.x));
+    {
+        R kr[V], ki[V], nr[V], ni[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {  // X[k] for k < B, X[N-k] for 0 < N-k < B (others 0)
+            const int idx = threadIdx.x + u * NTH;
+            const int q = q0 + (idx >> lgN), k = idx & (N - 1);
+            const R* row = Cb + ((size_t)f * N + frow_of(min(q, nrows - 1), N, cut)) * 2 * B;
+            const int kk = k < B ? k : 0, kn = (k > 0 && N - k < B) ? N - k : 0;
+            kr[u] = row[cpos(kk, half)];
+            ki[u] = row[B + cpos(kk, half)];
+            nr[u] = row[cpos(kn, half)];
+            ni[u] = row[B + cpos(kn, half)];
         }
-        a[rl * S + k] = v;
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            const int idx = threadIdx.x + u * NTH;
+            const int q = q0 + (idx >> lgN), rl = idx >> lgN, k = idx & (N - 1);
+            const bool hk = q < nrows && k < B, hn = q < nrows && k > 0 && N - k < B;
+            const R xr = hk ? kr[u] : (R)0, xi = hk ? ki[u] : (R)0;
+            const R yr = hn ? nr[u] : (R)0, yi = hn ? ni[u] : (R)0;
+            a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(xr + yi, xi - yr));
+        }
     }
     __syncthreads();
-    fft_block<T, +1, V>(a, lgN, S, twl);
+    fft_block<T, +1, V, NTH>(a, lgN, S, twl);
     const R inv_s2pi = (R)0.39894228040143267794;
-    for (int idx = threadIdx.x; idx < E; idx += kThreads) {
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        const int idx = threadIdx.x + u * NTH;
         const int rl = idx >> lgN, j = idx & (N - 1);
         const int q = q0 + rl;
+        const int n = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+        const T yv = a[rl * S + n];
         if (q >= nrows) continue;
         const int r = frow_of(q, N, cut);
         const int am = r < B ? r : N - r;
-        const int n = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
         R sc = inv_s2pi;
         if (am & 1) sc *= sinv[j];
         if (r > B && (r & 1)) sc = -sc;
-        const T y = cscale(a[rl * S + n], sc);
+        const T y = cscale(yv, sc);
         X[((size_t)f * N + r) * N + j] = y;
         if (fmt == 1 && r > 0) X[((size_t)f * N + (N - r)) * N + j] = cconj(y);
     }

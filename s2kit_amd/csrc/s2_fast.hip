@@ -326,8 +326,14 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
     const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
     {
         Span sp(tm, ST_DCT_FWD, s);
-        launch_lds(k_fdct_fwd<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const T*)X, Cb, ws, lgN, B,
-                   nrows, tw, ph);
+        if ((long)N * N >= 512L * V) {
+            const int rp2 = V * 512 / N;
+            launch_lds(k_fdct_fwd<R, V, 512>, dim3((nrows + rp2 - 1) / rp2, F), 512,
+                       sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const T*)X, Cb, ws, lgN, B, nrows, tw, ph);
+        } else {
+            launch_lds(k_fdct_fwd<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const T*)X, Cb, ws, lgN,
+                       B, nrows, tw, ph);
+        }
     }
     {
         const int ncb = gemm_ncb(B, g.prec);
@@ -361,8 +367,15 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
     {
         Span sp(tm, ST_DCT_INV, s);
-        launch_lds(k_fdct_inv<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const R*)Cb, X, sv, lgN, B,
-                   nrows, (int)fmt, tw, ph);
+        if ((long)N * N >= 512L * V) {
+            const int rp2 = V * 512 / N;
+            launch_lds(k_fdct_inv<R, V, 512>, dim3((nrows + rp2 - 1) / rp2, F), 512,
+                       sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const R*)Cb, X, sv, lgN, B, nrows, (int)fmt, tw,
+                       ph);
+        } else {
+            launch_lds(k_fdct_inv<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const R*)Cb, X, sv, lgN,
+                       B, nrows, (int)fmt, tw, ph);
+        }
     }
     {
         Span sp(tm, ST_PHI_INV, s);
