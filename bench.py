@@ -229,6 +229,67 @@ def conv_uneven(args):
     print(json.dumps(line), flush=True)
 
 
+def dropin_inputs(seed=3):
+    """Host inputs of the two CPU-runnable cases: a bw=64 coefficient set
+    (uniform[-1, 1]) and a bw=128 signal/filter pair (uniform signal
+    coefficients; zonal filter h_l = exp(-l(l+1)/(2*20^2)) * U[0.5, 1])."""
+    rng = np.random.default_rng(seed)
+    c64 = (rng.uniform(-1, 1, 64 * 64), rng.uniform(-1, 1, 64 * 64))
+    s128 = (rng.uniform(-1, 1, 128 * 128), rng.uniform(-1, 1, 128 * 128))
+    l = np.arange(128)
+    h = np.zeros(128 * 128)
+    h[:128] = np.exp(-l * (l + 1) / (2 * 20.0 ** 2)) * rng.uniform(0.5, 1.0, 128)
+    return c64, s128, h
+
+
+def dropin_latency(reps):
+    """Per-call time of the drop-ins on host buffers (PCIe copies included),
+    on the two CPU-runnable cases: InvFSTSemiMemo + FSTSemiMemo at bw=64
+    (cutoff = bw, tables built once by the drop-in builders), and
+    ConvOn2SphereSemiMemo at bw=128 (which builds its tables per call, as the
+    reference's does)."""
+    import s2kit_amd as S
+    (cr, ci), (sr, si), h = dropin_inputs()
+    T = S.HostTables(64)
+    w = S.weights(64)
+    rd, idd = S.inv_fst(cr, ci, 64, T)  # first call uploads the tables
+    S.fst(rd, idd, 64, T, w)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rd, idd = S.inv_fst(cr, ci, 64, T)
+        S.fst(rd, idd, 64, T, w)
+    pair = (time.perf_counter() - t0) / reps
+    T128 = S.HostTables(128)
+    gr, gi = S.inv_fst(sr, si, 128, T128)
+    hr, hi = S.inv_fst(h, np.zeros_like(h), 128, T128)
+    S.conv(gr, gi, hr, hi, 128)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        S.conv(gr, gi, hr, hi, 128)
+    conv = (time.perf_counter() - t0) / reps
+    return {"bw64_inv_plus_fwd_ms": pair * 1e3, "bw128_conv_ms": conv * 1e3, "reps": reps}
+
+
+def dropin_cpu(reps):
+    """The same two cases on the CPU oracle (one core), tables precomputed for
+    the transform pair, rebuilt per call for the convolution."""
+    from oracle import oracle as O
+    (cr, ci), (sr, si), h = dropin_inputs()
+    O.tables(64)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rd, idd = O.inv_fst(cr, ci, 64)
+        O.fst(rd, idd, 64)
+    pair = (time.perf_counter() - t0) / reps
+    gr, gi = O.inv_fst(sr, si, 128)
+    hr, hi = O.inv_fst(h, np.zeros_like(h), 128)
+    t0 = time.perf_counter()
+    for _ in range(max(1, reps // 4)):
+        O.conv(gr, gi, hr, hi, 128)
+    conv = (time.perf_counter() - t0) / max(1, reps // 4)
+    return {"bw64_inv_plus_fwd_ms": pair * 1e3, "bw128_conv_ms": conv * 1e3, "cores": 1}
+
+
 def cpu_worker(args):
     """One CPU process: `pairs` oracle inverse+forward transforms at bw."""
     bw, pairs, seed = args
@@ -272,6 +333,8 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
     ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
+    ap.add_argument("--dropin-reps", type=int, default=20,
+                    help="calls timed per drop-in case (bw=64 pair, bw=128 conv); 0: skip")
     ap.add_argument("--single-pairs", type=int, default=10,
                     help="FP64 pairs timed on one bw=512 field (table streaming); 0: skip")
     ap.add_argument("--workload", choices=["sht", "conv"], default="sht",
@@ -295,6 +358,8 @@ def main():
     if args.cpu_pairs > 0:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(bw, args.cpu_pairs, procs)
+        if args.dropin_reps > 0:
+            cpu["dropin_cases"] = dropin_cpu(args.dropin_reps)
 
     import torch
 
@@ -381,6 +446,9 @@ def main():
         plain = pf / (time.perf_counter() - t1)
         p1.close()
 
+    # ---- the drop-ins on host buffers (PCIe included) ------------------------
+    dropin = dropin_latency(args.dropin_reps) if args.dropin_reps > 0 else None
+
     # ---- one field at bw=512: table streaming and per-form error -------------
     single = single_field_bw512(dev, args.single_pairs) if args.single_pairs > 0 else None
 
@@ -424,6 +492,7 @@ def main():
         "fp64_batch1_pairs_per_s": plain,
         "speedup_vs_fp64_batch1": (value / plain) if plain else None,
         "single_field_bw512": single,
+        "dropin_host_buffers": dropin,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
