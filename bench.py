@@ -48,7 +48,7 @@ def stage_bytes(stage, bw, nfields, elem, form):
     coef = 16 * bw * bw
     xrow = elem * n * rows  # phi-FFT output rows that the next stage reads
     crow = elem * bw * rows  # cosine coefficients (seminaive forms)
-    if form == "mixed":
+    if form in ("mixed", "split"):
         per_field = {"phi_fwd": grid + elem * n * n, "leg_fwd": xrow + coef, "leg_inv": coef + xrow,
                      "phi_inv": xrow + grid}[stage]
     else:
@@ -60,6 +60,8 @@ def stage_bytes(stage, bw, nfields, elem, form):
 # kernel behind each timed stage, per form (names as rocprofv3 reports them)
 KERNELS = {
     "mixed": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "leg_fwd": "k_mix_fwd", "leg_inv": "k_mix_inv",
+              "phi_inv": "k_fphi_inv<float, 16, 512>"},
+    "split": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "leg_fwd": "k_split_fwd", "leg_inv": "k_split_inv",
               "phi_inv": "k_fphi_inv<float, 16, 512>"},
     "fp32": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "dct_fwd": "k_fdct_fwd<float, 16>", "leg_fwd": "k_gemm_fwd<float>",
              "leg_inv": "k_gemm_inv<float>", "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16, 512>"},
@@ -126,7 +128,7 @@ def single_field_bw512(dev, pairs):
     p64.inverse(c0r, c0i, gre, gim)
     ref_r, ref_i = torch.empty_like(c0r), torch.empty_like(c0i)
     p64.forward(gre, gim, ref_r, ref_i)
-    for name, prec in (("fp32", S.FP32), ("mixed", S.MIXED)):
+    for name, prec in (("fp32", S.FP32), ("mixed", S.MIXED), ("split", S.SPLIT_BF16)):
         p = S.Plan(bw, 1, precision=prec)
         r, i = torch.empty_like(c0r), torch.empty_like(c0i)
         g2r, g2i = torch.empty_like(gre), torch.empty_like(gim)
@@ -329,7 +331,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bw", type=int, default=256)
     ap.add_argument("--fields", type=int, default=1024)
-    ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="mixed")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "mixed", "split"], default="mixed")
     ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
     ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
@@ -370,7 +372,7 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     n = 2 * bw
-    prec = {"fp64": S.FP64, "fp32": S.FP32, "mixed": S.MIXED}[args.precision]
+    prec = {"fp64": S.FP64, "fp32": S.FP32, "mixed": S.MIXED, "split": S.SPLIT_BF16}[args.precision]
 
     if args.copy_calibration:  # reads 1 GiB and writes 1 GiB, nothing else
         src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
@@ -483,7 +485,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f32"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f32", "split": "f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"batched fwd+inv SHT, bw={bw}, {F} fields, {n}x{n} complex FP64 grids, "
                                "power-law (l+1)^-2 coefficients", "bw": bw, "fields": F,

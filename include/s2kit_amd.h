@@ -42,7 +42,12 @@ typedef enum {
     /* mixed: FP32 ring FFTs and the naive Legendre transform (naive.c) as an
      * FP32-MFMA GEMM over m on tables from the FP64 recurrence; FP64 in/out.
      * Error ~4e-7 relative.  bw a power of two in [64, 512]. */
-    S2AMD_MIXED = 2
+    S2AMD_MIXED = 2,
+    /* split BF16: the mixed form's pipeline with the Legendre GEMMs on BF16
+     * MFMA; each FP32 operand is split into three BF16 pieces and six piece
+     * products are accumulated in FP32, which reproduces the FP32 GEMM.
+     * Error ~4e-7 relative.  bw a power of two in [64, 512]. */
+    S2AMD_SPLIT_BF16 = 3
 } s2amd_precision;
 
 typedef struct s2amd_plan s2amd_plan;

@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libs2kit_amd.so")
 
 COMPLEX, REAL = 0, 1
-FP64, FP32, MIXED = 0, 1, 2
+FP64, FP32, MIXED, SPLIT_BF16 = 0, 1, 2, 3
 NSTAGES = 10
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -294,7 +294,7 @@ static bool use_fast(const Plan* p, int F) {
 
 static void plan_forward(Plan* p, const double* re, const double* im, double* cre, double* cim, int F, Fmt fmt,
                          hipStream_t s) {
-    if (p->precision == S2AMD_MIXED)
+    if (p->precision == S2AMD_MIXED || p->precision == S2AMD_SPLIT_BF16)
         forward_mixed(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, s, &p->timer);
     else if (use_fast(p, F))
         forward_fast(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
@@ -304,7 +304,7 @@ static void plan_forward(Plan* p, const double* re, const double* im, double* cr
 
 static void plan_inverse(Plan* p, const double* cre, const double* cim, double* re, double* im, int F, Fmt fmt,
                          hipStream_t s) {
-    if (p->precision == S2AMD_MIXED)
+    if (p->precision == S2AMD_MIXED || p->precision == S2AMD_SPLIT_BF16)
         inverse_mixed(p->c, p->g, cre, cim, re, im, F, fmt, p->X.p, s, &p->timer);
     else if (use_fast(p, F))
         inverse_fast(p->c, p->g, cre, cim, re, im, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
@@ -729,10 +729,13 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
         check_common(bw, 0);
         if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
         if (max_fields < 1) throw ArgError{"max_fields must be >= 1"};
-        if (precision != S2AMD_FP64 && precision != S2AMD_FP32 && precision != S2AMD_MIXED)
+        if (precision != S2AMD_FP64 && precision != S2AMD_FP32 && precision != S2AMD_MIXED &&
+            precision != S2AMD_SPLIT_BF16)
             throw ArgError{"unknown precision"};
         if (precision == S2AMD_MIXED && !mixed_supported(bw))
             throw ArgError{"the mixed form needs a power-of-two grid with bw in [64, 512]"};
+        if (precision == S2AMD_SPLIT_BF16 && !mixed_supported(bw))
+            throw ArgError{"the split-BF16 form needs a power-of-two grid with bw in [64, 512]"};
         if (precision == S2AMD_FP32 && !fast_supported(bw, cutoff, 1))
             throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 1024] and cutoff = bw"};
         std::unique_ptr<s2amd_plan> p(new s2amd_plan());
@@ -748,6 +751,8 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
             tables_generate(p->t, bw, cutoff, true, true, s);
             if (precision == S2AMD_MIXED) {
                 naive_tables_create(p->g, p->c, s);
+            } else if (precision == S2AMD_SPLIT_BF16) {
+                split_tables_create(p->g, p->c, s);
             } else {
                 p->fast = fast_supported(bw, cutoff, precision);
                 if (p->fast) gemm_tables_create(p->g, p->t, precision, s);
@@ -790,7 +795,9 @@ extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
 
 extern "C" size_t s2amd_plan_bytes(const s2amd_plan* p) {
     if (!p) return 0;
-    const size_t el = p->precision == S2AMD_FP64 ? sizeof(double) : sizeof(float);  // GEMM tables
+    const size_t el = p->precision == S2AMD_FP64        ? sizeof(double)
+                      : p->precision == S2AMD_SPLIT_BF16 ? sizeof(uint16_t)
+                                                         : sizeof(float);  // GEMM table elements
     return 2 * sizeof(double) * p->t.total + el * (p->g.nf + p->g.ni) + p->X.n + p->Cb.n + p->cr.n + p->ci.n +
            p->zr.n + p->zi.n;
 }

@@ -87,7 +87,7 @@ struct GemmMeta {
 };
 
 struct GemmTables {
-    int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32 (seminaive); 2: FP32 naive (mixed form)
+    int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32 (seminaive); 2: FP32 naive (mixed form); 3: split BF16 naive
     void* af = nullptr;
     void* ai = nullptr;
     GemmMeta* meta = nullptr;
@@ -187,5 +187,13 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
                    double* cim, int F, Fmt fmt, void* X, hipStream_t s, StageTimer* tm);
 void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
                    double* im, int F, Fmt fmt, void* X, hipStream_t s, StageTimer* tm);
+
+// ---- split-BF16 form (s2_split.hip): the mixed form's pipeline with the
+// Legendre GEMMs on BF16 MFMA, three-piece splits and six products ----------
+void split_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s);
+void split_fwd_gemm(const DevConsts& c, const GemmTables& g, const float2* X, double* cre, double* cim, int F,
+                    Fmt fmt, hipStream_t s);
+void split_inv_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, float2* X, int F,
+                    Fmt fmt, hipStream_t s);
 
 }  // namespace s2

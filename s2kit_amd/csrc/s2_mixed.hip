@@ -25,19 +25,15 @@
 #include <algorithm>
 #include <cmath>
 
-#include "s2_blocks.h"
+#include "s2_mix_common.h"
 #include "s2_recurrence.h"
 
 namespace s2 {
 
 typedef float fvec2 __attribute__((ext_vector_type(2)));
 
-// Waves per block, 16 data columns each.
-#ifndef S2_MIX_WAVES
-#define S2_MIX_WAVES 4
-#endif
 #ifndef S2_MIX_WPE_FWD
-#define S2_MIX_WPE_FWD 3
+#define S2_MIX_WPE_FWD 2
 #endif
 #ifndef S2_MIX_WPE_INV
 #define S2_MIX_WPE_INV 2
@@ -45,14 +41,6 @@ typedef float fvec2 __attribute__((ext_vector_type(2)));
 #ifndef S2_MIX_GJMAX
 #define S2_MIX_GJMAX 4
 #endif
-#ifndef S2_MIX_FFT_THREADS
-#define S2_MIX_FFT_THREADS 512
-#endif
-// Ring-FFT blocks of the mixed form: 16 values per thread, so a block holds
-// 16 rings at bw=256 and stores full 128-byte row segments.
-constexpr int kMixFftThreads = S2_MIX_FFT_THREADS;
-constexpr int kMixThreads = 64 * S2_MIX_WAVES;
-constexpr int kMixCols = 16 * S2_MIX_WAVES;
 
 // Tables.
 //   NF(m) [j < bw][W], W = 32 n0(m): tiles t < n0 hold degrees l = m+2(16t+x),
@@ -61,9 +49,8 @@ constexpr int kMixCols = 16 * S2_MIX_WAVES;
 //         fours -- entry x of tile 4g+v sits at 64g + 4x + v, a last pair at
 //         64G + 2x + v -- so one float4 (float2) per lane feeds four (two)
 //         16x16 MFMA tiles.  Entries past the last degree are zero.
-//   NI(m,p) [i < ki(m)][j < bw]: P_{m+p+2i}^m(theta_j), rings interleaved in
+//   NI(m,p) [i < ki(m)][j < bw]: P_{m+p+2i}^m(theta_j)/sqrt(2pi), rings interleaved in
 //         groups of 64 the same way; rows past the last degree are zero.
-__host__ __device__ inline int mix_n0(int m, int B) { return (gm_lp(m, 0, B) + 15) / 16; }
 #ifndef S2_MIX_INV_KC
 #define S2_MIX_INV_KC 8
 #endif
@@ -102,7 +89,7 @@ __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restr
         for (int l = m; l < B; ++l) {
             const int p = (l - m) & 1, i = (l - m) >> 1;
             NF[g0.offF + (long)j * W + mix_pos(p * g0.n0 + (i >> 4), i & 15, nt)] = (float)(cur * wj);
-            NI[(p ? g1.offI : g0.offI) + (long)i * B + ilv64(j)] = (float)cur;
+            NI[(p ? g1.offI : g0.offI) + (long)i * B + ilv64(j)] = (float)(cur * 0.39894228040143267794);
             if (l + 1 < B) {
                 const double nx = l2_an(m, l) * (cur * x) + l2_cn(m, l) * prev;
                 prev = cur;
@@ -110,21 +97,6 @@ __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restr
             }
         }
     }
-}
-
-// Workgroup id -> an order pair (m, bw-1-m) and a column block cb of nb per
-// order.  A workgroup runs both orders of its pair, so every workgroup has
-// about the same work (bw-m + m+1 degrees): blocks of very different lengths
-// would leave the dispatcher's round-robin placement unbalanced.  Ids go
-// round-robin over the 8 XCDs (id mod 8); XCD k takes the pairs m = 8q+k, all
-// column blocks of one pair back to back, so the pair's table tiles stay in
-// that XCD's L2.  bw % 16 == 0.
-__device__ __forceinline__ void mix_pair(int B, int nb, int& m, int& cb) {
-    const int id = blockIdx.x, k = id & 7, local = id >> 3;
-    const int q = local / nb;
-    cb = local - q * nb;
-    m = 8 * q + k;  // < bw/2; its partner is bw-1-m
-    (void)B;
 }
 
 // ---------------------------------------------------------------------------
@@ -227,45 +199,7 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
         __syncthreads();
     }
 
-    // Epilogue.  A column's coefficients of order m are one contiguous run
-    // of L = bw-m values (IndexOfHarmonicCoeff), so the tiles go through LDS
-    // (32 columns at a time, degree offset l-m = 2i+p along a row) and every
-    // run is written with consecutive lanes on consecutive addresses.
-    static_assert(kMixCols % 32 == 0 && KC >= 16, "staging needs 32 rows of W floats");
-    const size_t B2 = (size_t)B * B;
-    const double sg = (m & 1) ? -1.0 : 1.0;
-    const int L = B - m;
-    const int bp = index_of_coeff(m, m, B), bn = index_of_coeff(-m, m, B);
-    for (int h = 0; h < kMixCols / 32; ++h) {
-        if ((wave >> 1) == h) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const int p = t < N0 ? 0 : 1;
-                const int d = 2 * (16 * (t - p * N0) + x16) + p;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sm[((wave & 1) * 16 + 4 * kq + r) * W + d] = acc[t][r];
-            }
-        }
-        __syncthreads();
-        for (int cl = wave; cl < 32; cl += kMixThreads / 64) {
-            const int c = cb * kMixCols + 32 * h + cl;
-            if (c >= ncols) continue;
-            const int f = c / cpf, rem = c - f * cpf;
-            const int comp = rem & 1;
-            double* out = (comp ? cim : cre) + f * B2;
-            const float* row = sm + cl * W;
-            for (int d = lane; d < L; d += 64) {
-                const double v = (double)row[d];
-                if ((rem >> 1) == 0) {
-                    out[bp + d] = v;
-                    if (fmt == 1 && m > 0) out[bn + d] = comp ? -sg * v : sg * v;
-                } else {
-                    out[bn + d] = sg * v;
-                }
-            }
-        }
-        __syncthreads();
-    }
+    mix_fwd_store<N0>(acc, cre, cim, m, cb, B, F, fmt);
 }
 
 template <int MAXN0>
@@ -276,7 +210,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
     float* sm = reinterpret_cast<float*>(smem);
     const int B = 1 << lgB;
     int mp, cb;
-    mix_pair(B, nb, mp, cb);
+    mix_pair(nb, mp, cb);
     for (int h = 0; h < 2; ++h) {
         const int m = h ? B - 1 - mp : mp;
         const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
@@ -321,7 +255,6 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     constexpr int NV = 2 * KC * JW / 4;  // float4 per table chunk (both parities)
     constexpr int PT = NV / kMixThreads;
     static_assert(NV % kMixThreads == 0, "table chunk must split evenly over the block");
-    const int N = 2 * B;
     const int nz = B / JW;
     const int cb = cz / nz, jbase = (cz - cb * nz) * JW;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
@@ -351,24 +284,34 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
         const int rw = rest / (JW / 4), c4 = rest - rw * (JW / 4);
         tsrc[u] = NI + (p ? g1.offI : g0.offI) + (size_t)rw * B + jbase + 4 * c4;
     }
-    double cv[2 * S];
+    // rows -m of odd m carry the sign (-1)^m: applied to the operand
+    const unsigned nsign = ((((wave * 16 + x16) % cpf) >> 1) && (m & 1)) ? 0x80000000u : 0u;
+    dpair8 cv[S];  // degree offsets d0 + 2h (even parity), d0 + 2h + 1 (odd)
     fvec4 tv[PT];
     auto fetch = [&](int ch) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < PT; ++u) tv[u] = *reinterpret_cast<const fvec4*>(tsrc[u] + (size_t)ch * KC * B);
         const int d0 = 2 * (ch * KC + kq * S);
+        if (d0 + 2 * S <= L) {
 #pragma unroll
-        for (int e = 0; e < 2 * S; ++e) cv[e] = cp[min(d0 + e, L - 1)];
+            for (int h = 0; h < S; ++h) cv[h] = *reinterpret_cast<const dpair8*>(cp + d0 + 2 * h);
+        } else {  // the run ends inside this lane's degrees
+#pragma unroll
+            for (int h = 0; h < S; ++h)
+                cv[h] = dpair8{d0 + 2 * h < L ? cp[d0 + 2 * h] : 0.0, d0 + 2 * h + 1 < L ? cp[d0 + 2 * h + 1] : 0.0};
+        }
     };
     auto stash = [&](float* tb) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < PT; ++u) reinterpret_cast<fvec4*>(tb)[tid + u * kMixThreads] = tv[u];
     };
     float a[2][S];
-    auto convert = [&](int ch) __attribute__((always_inline)) {
-        const int d0 = 2 * (ch * KC + kq * S);
+    auto convert = [&]() __attribute__((always_inline)) {
 #pragma unroll
-        for (int e = 0; e < 2 * S; ++e) a[e & 1][e >> 1] = (d0 + e < L) ? (float)cv[e] : 0.f;
+        for (int h = 0; h < S; ++h) {
+            a[0][h] = __uint_as_float(__float_as_uint((float)cv[h].x) ^ nsign);
+            a[1][h] = __uint_as_float(__float_as_uint((float)cv[h].y) ^ nsign);
+        }
     };
 
     fvec4 acc[2][4 * GJ];
@@ -383,7 +326,7 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     const int nch = g0.ki / KC;
     for (int ch = 0; ch < nch; ++ch) {
         const float* tb = sm + (ch & 1) * TB;
-        convert(ch);
+        convert();
         const bool more = ch + 1 < nch;
         if (more) fetch(ch + 1);
         fvec4 tq[2][2][GJ];
@@ -414,50 +357,7 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
         __syncthreads();
     }
 
-    // Epilogue: this lane holds rings jbase + 16t + x16 (t < 4GJ) of columns
-    // 4kq + r; rows r, r+1 are the re, im parts of one (field, sign) row.
-    // Addresses are set up once per row pair, stores then use fixed offsets.
-    const float inv_s2pi = 0.39894228040143267794f;
-    const bool mirror = fmt == 1 && m > 0;
-    float2* plo[2];
-    float2* phi[2];
-    float2* qlo[2];
-    float2* qhi[2];
-    float scl[2];
-    bool ok[2];
-#pragma unroll
-    for (int rr = 0; rr < 2; ++rr) {
-        const int c = cb * kMixCols + wave * 16 + 4 * kq + 2 * rr;
-        ok[rr] = c < ncols;
-        const int cc = ok[rr] ? c : 0;
-        const int f = cc / cpf, rem = cc - f * cpf;
-        const int row = (rem >> 1) ? N - m : m;
-        scl[rr] = (row > B && (row & 1)) ? -inv_s2pi : inv_s2pi;
-        float2* xr = X + ((size_t)f * N + row) * N;
-        float2* xm = X + ((size_t)f * N + (N - m)) * N;
-        plo[rr] = xr + jbase + x16;
-        phi[rr] = xr + (N - 1 - jbase - x16);
-        qlo[rr] = xm + jbase + x16;
-        qhi[rr] = xm + (N - 1 - jbase - x16);
-    }
-#pragma unroll
-    for (int t = 0; t < 4 * GJ; ++t) {
-        const fvec4 e = acc[0][t], o = acc[1][t];
-#pragma unroll
-        for (int rr = 0; rr < 2; ++rr) {
-            if (!ok[rr]) continue;
-            const int r = 2 * rr;
-            const float sc = scl[rr];
-            const float2 yn = make_float2((e[r] + o[r]) * sc, (e[r + 1] + o[r + 1]) * sc);
-            const float2 ys = make_float2((e[r] - o[r]) * sc, (e[r + 1] - o[r + 1]) * sc);
-            plo[rr][16 * t] = yn;
-            phi[rr][-16 * t] = ys;
-            if (mirror) {
-                qlo[rr][16 * t] = make_float2(yn.x, -yn.y);
-                qhi[rr][-16 * t] = make_float2(ys.x, -ys.y);
-            }
-        }
-    }
+    mix_inv_store<4 * GJ>(acc, X, m, cb, jbase, B, F, fmt);
 }
 
 template <int GJ>
@@ -468,7 +368,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
     float* sm = reinterpret_cast<float*>(smem);
     const int B = 1 << lgB;
     int mp, cz;
-    mix_pair(B, nb * (B / (64 * GJ)), mp, cz);
+    mix_pair(nb * (B / (64 * GJ)), mp, cz);
     mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, mp, cz, sm);
     __syncthreads();  // the partner order's prologue reuses the table buffers
     mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, B - 1 - mp, cz, sm);
@@ -571,7 +471,10 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
     }
     {
         Span sp(tm, ST_LEG_FWD, s);
-        mix_fwd_gemm(c, g, X, cre, cim, F, fmt, s);
+        if (g.prec == 3)
+            split_fwd_gemm(c, g, X, cre, cim, F, fmt, s);
+        else
+            mix_fwd_gemm(c, g, X, cre, cim, F, fmt, s);
     }
 }
 
@@ -583,7 +486,10 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
     float2* X = (float2*)Xv;
     {
         Span sp(tm, ST_LEG_INV, s);
-        mix_inv_gemm(c, g, cre, cim, X, F, fmt, s);
+        if (g.prec == 3)
+            split_inv_gemm(c, g, cre, cim, X, F, fmt, s);
+        else
+            mix_inv_gemm(c, g, cre, cim, X, F, fmt, s);
     }
     {
         Span sp(tm, ST_PHI_INV, s);

@@ -102,6 +102,9 @@ def test_plan_rejects_bad_arguments(S):
     assert L.s2amd_plan_create(ctypes.byref(h), 16, 16, 0, 0) == -1
     assert L.s2amd_plan_create(ctypes.byref(h), 16, 16, 4, 9) == -1
     assert "precision" in S.last_error()
+    for prec, name in ((S.MIXED, "mixed"), (S.SPLIT_BF16, "split-BF16")):
+        assert L.s2amd_plan_create(ctypes.byref(h), 96, 96, 4, prec) == -1
+        assert name in S.last_error()
     assert L.s2amd_forward(None, None, None, None, None, 1, 0, None) == -1
     assert "plan is NULL" in S.last_error()
     assert L.s2amd_plan_destroy(None) == 0
