@@ -14,6 +14,9 @@ Also reported on the same JSON line:
   roofline    the dominant kernel's algorithmic HBM bytes per launch over its
               average duration (HIP events on the launch stream, recorded
               during the timed steps), against 8 TB/s;
+  other_forms the FP64 form and the other FP32-accurate form (mixed: FP32
+              MFMA GEMMs; split: BF16 MFMA with split operands) timed on the
+              same fields, with their errors against FP64;
   cpu_baseline  the CPU FP64 oracle (a port of the reference path) timed on
               the host, one process per core, on a bounded sample.
 
@@ -335,6 +338,8 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
     ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
+    ap.add_argument("--other-forms", type=int, default=1,
+                    help="also time the other batched forms (FP64, mixed / split BF16) on the same fields (0: skip)")
     ap.add_argument("--dropin-reps", type=int, default=20,
                     help="calls timed per drop-in case (bw=64 pair, bw=128 conv); 0: skip")
     ap.add_argument("--single-pairs", type=int, default=10,
@@ -431,6 +436,34 @@ def main():
     torch.cuda.synchronize()
     errors["round_trip"] = rel(cre, cim, c0r, c0i)
 
+    # ---- the other batched forms on the same fields ---------------------------
+    def pairs_per_s(pl, nsteps):
+        pl.forward(gre, gim, cre, cim)
+        pl.inverse(cre, cim, ore, oim)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(nsteps):
+            pl.forward(gre, gim, cre, cim)
+            pl.inverse(cre, cim, ore, oim)
+        torch.cuda.synchronize()
+        return F * nsteps / (time.perf_counter() - t1)
+
+    forms = {}
+    if args.other_forms:
+        if prec != S.FP64:
+            forms["fp64"] = {"pairs_per_s": pairs_per_s(p64, 2)}
+        for name, fprec in (("mixed", S.MIXED), ("split", S.SPLIT_BF16)):
+            if fprec == prec:
+                continue
+            pl = S.Plan(bw, F, precision=fprec)
+            v = pairs_per_s(pl, args.steps)
+            pl.forward(gre, gim, cre, cim)
+            pl.inverse(c0r, c0i, ore, oim)
+            torch.cuda.synchronize()
+            forms[name] = {"pairs_per_s": v, "forward_vs_fp64": rel(cre, cim, ref_cr, ref_ci),
+                           "inverse_vs_fp64": rel(ore, oim, gre, gim)}
+            pl.close()
+
     # ---- batch-1 FP64 form: one field per call -----------------------------
     plain = None
     if args.plain_fields > 0:
@@ -491,6 +524,8 @@ def main():
                                "power-law (l+1)^-2 coefficients", "bw": bw, "fields": F,
                    "precision": args.precision, "parallelism": "single GPU"},
         "errors": errors,
+        "other_forms": forms,
+        "speedup_vs_fp64_batched": (value / forms["fp64"]["pairs_per_s"]) if "fp64" in forms else None,
         "fp64_batch1_pairs_per_s": plain,
         "speedup_vs_fp64_batch1": (value / plain) if plain else None,
         "single_field_bw512": single,

@@ -86,9 +86,9 @@ int s2amd_conv(s2amd_plan* plan, const double* sre, const double* sim, const dou
  * group's forward / zonal / TransMult / inverse pipeline, scattering the
  * outputs -- as one HIP graph; _run replays it on a stream.  Requests are
  * copied at creation: their buffers must stay allocated while the batch is
- * used; their contents are read at every _run.  precision: S2AMD_FP64 or
- * S2AMD_MIXED; a group whose bandwidth the mixed form does not support (not a
- * power of two in [64, 512]) runs in FP64.  No counterpart in the reference,
+ * used; their contents are read at every _run.  precision: S2AMD_FP64,
+ * S2AMD_MIXED or S2AMD_SPLIT_BF16; a group whose bandwidth the last two do not
+ * support (not a power of two in [64, 512]) runs in FP64.  No counterpart in the reference,
  * which convolves one pair of host grids per call (FST_semi_memo.c:439-518). */
 typedef struct {
     int bw;

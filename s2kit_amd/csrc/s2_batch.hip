@@ -106,8 +106,8 @@ extern "C" int s2amd_conv_batch_create(s2amd_conv_batch** out, const s2amd_conv_
         *out = nullptr;
         if (n < 1) throw ArgError{"need at least one request"};
         if (!req) throw ArgError{"request array is NULL"};
-        if (precision != S2AMD_FP64 && precision != S2AMD_MIXED)
-            throw ArgError{"precision must be S2AMD_FP64 or S2AMD_MIXED"};
+        if (precision != S2AMD_FP64 && precision != S2AMD_MIXED && precision != S2AMD_SPLIT_BF16)
+            throw ArgError{"precision must be S2AMD_FP64, S2AMD_MIXED or S2AMD_SPLIT_BF16"};
         std::map<int, std::vector<int>> by_bw;
         for (int i = 0; i < n; ++i) {
             const s2amd_conv_request& r = req[i];
@@ -130,7 +130,7 @@ extern "C" int s2amd_conv_batch_create(s2amd_conv_batch** out, const s2amd_conv_
             const std::vector<int>& idx = kv.second;
             g.bw = kv.first;
             g.n = (int)idx.size();
-            g.precision = (precision == S2AMD_MIXED && mixed_ok(g.bw)) ? S2AMD_MIXED : S2AMD_FP64;
+            g.precision = (precision != S2AMD_FP64 && mixed_ok(g.bw)) ? precision : S2AMD_FP64;
             check_plan(s2amd_plan_create(&g.plan, g.bw, g.bw, g.n, g.precision));
             const long nn = 4L * g.bw * g.bw;
             const size_t bytes = sizeof(double) * nn * g.n;

@@ -4,9 +4,9 @@ request against the CPU oracle's ConvOn2SphereSemiMemo
 go through the gather/scatter kernels, and back-to-back buffers, used in
 place), and across graph replays with eager work in between.
 
-FP64 batches: 1e-11 relative (normwise) per request.  Mixed batches: 1e-6
-for bandwidths the mixed form supports; other bandwidths run in FP64 and
-keep the FP64 bound.
+FP64 batches: 1e-11 relative (normwise) per request.  Mixed and split-BF16
+batches: 1e-6 for bandwidths those forms support; other bandwidths run in
+FP64 and keep the FP64 bound.
 """
 import numpy as np
 import pytest
@@ -77,7 +77,7 @@ def check(reqs, expect, tol_of):
     return worst
 
 
-@pytest.mark.parametrize("prec", [0, 2])
+@pytest.mark.parametrize("prec", [0, 2, 3])
 def test_conv_batch_matches_oracle(env, O, prec):
     S, torch, dev = env
     bws = [32, 64, 128, 64, 32, 128, 64, 16]
@@ -86,10 +86,10 @@ def test_conv_batch_matches_oracle(env, O, prec):
     groups = batch.groups()
     assert [g[0] for g in groups] == [16, 32, 64, 128]
     assert [g[1] for g in groups] == [1, 2, 3, 2]
-    assert [g[2] for g in groups] == ([0, 0, 0, 0] if prec == 0 else [0, 0, 2, 2])
+    assert [g[2] for g in groups] == ([0, 0, 0, 0] if prec == 0 else [0, 0, prec, prec])
 
     def tol(bw):
-        return 1e-6 if (prec == 2 and bw >= 64) else 1e-11
+        return 1e-6 if (prec != 0 and bw >= 64) else 1e-11
 
     batch.run()
     torch.cuda.synchronize()
