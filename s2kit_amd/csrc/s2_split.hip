@@ -45,6 +45,12 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 #define S2_SPLIT_BLOCKS 1
 #endif
 constexpr int kSplitBlocks = S2_SPLIT_BLOCKS;
+// Table tiles whose six products are interleaved (a group's pieces are read
+// while the previous group's MFMAs run).
+#ifndef S2_SPLIT_G
+#define S2_SPLIT_G 1
+#endif
+constexpr int kSplitG = S2_SPLIT_G;
 // Waves per SIMD the register allocation of each GEMM kernel aims for.
 #ifndef S2_SPLIT_WPE_FWD
 #define S2_SPLIT_WPE_FWD 2
@@ -128,6 +134,12 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8 (&a)[3]) {
         a[1][e] = md;
         a[2][e] = (__bf16)(r - (float)md);
     }
+}
+
+// Product k of the six, smallest first: l*h, h*l, m*m, m*h, h*m, h*h.
+__device__ __forceinline__ fvec4 mfma_piece(int k, const bf16x8 (&a)[3], const bf16x8 (&b)[3], fvec4 c) {
+    constexpr int ia[6] = {2, 0, 1, 1, 0, 0}, ib[6] = {0, 2, 1, 0, 1, 0};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ia[k]], b[ib[k]], c, 0, 0, 0);
 }
 
 __device__ __forceinline__ fvec4 mfma6(const bf16x8 (&a)[3], const bf16x8 (&b)[3], fvec4 c) {
@@ -246,13 +258,33 @@ __device__ __forceinline__ void split_fwd_body(const float2* __restrict__ X, con
                 const int s = it * spb + ch * SPC + q;
                 if (s + 1 < nst) copy_stage(s + 1);
                 const unsigned char* buf = sm + (s & 1) * kSplitStage;
+                // tiles in groups of kSplitG (stage tile counts are even): the six
+                // products run across the group, so consecutive MFMAs are independent;
+                // the next group's table pieces are read while this group's MFMAs run
+                constexpr int GN = kSplitStageTiles / kSplitG;
+                bf16x8 bq[2][kSplitG][3];
 #pragma unroll
-                for (int tt = 0; tt < kSplitStageTiles; ++tt) {
-                    const int t = q * kSplitStageTiles + tt;
-                    if (t < NT) {
-                        bf16x8 b[3];
-                        stage_tile(buf, tt, lane, b);
-                        acc[t] = mfma6(ap[t < N0 ? 0 : 1], b, acc[t]);
+                for (int g = 0; g < kSplitG; ++g)
+                    if (q * kSplitStageTiles + g < NT) stage_tile(buf, g, lane, bq[0][g]);
+#pragma unroll
+                for (int gi = 0; gi < GN; ++gi) {
+                    const int t0 = q * kSplitStageTiles + gi * kSplitG;
+                    if (t0 < NT) {
+#pragma unroll
+                        for (int g = 0; g < kSplitG; ++g) {
+                            const int tn = (gi + 1) * kSplitG + g;
+                            if (gi + 1 < GN && q * kSplitStageTiles + tn < NT)
+                                stage_tile(buf, tn, lane, bq[(gi + 1) & 1][g]);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of these MFMAs
+#pragma unroll
+                        for (int k = 0; k < 6; ++k)
+#pragma unroll
+                            for (int g = 0; g < kSplitG; ++g) {
+                                const int t = t0 + g;
+                                if (t < NT) acc[t] = mfma_piece(k, ap[t < N0 ? 0 : 1], bq[gi & 1][g], acc[t]);
+                            }
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                 }
                 stage_sync();
@@ -397,11 +429,24 @@ __device__ __forceinline__ void split_inv_body(const double* __restrict__ cre, c
                 const int p = r / SPP, hh = r - p * SPP;
                 if (s + 1 < nst) copy_stage(s + 1);
                 const unsigned char* buf = sm + (s & 1) * SB;
+                constexpr int GN = TPS / kSplitG;
+                bf16x8 bq[2][kSplitG][3];
 #pragma unroll
-                for (int tt = 0; tt < TPS; ++tt) {
-                    bf16x8 b[3];
-                    stage_tile(buf, tt, lane, b);
-                    acc[p][hh * TPS + tt] = mfma6(ap[p], b, acc[p][hh * TPS + tt]);
+                for (int g = 0; g < kSplitG; ++g) stage_tile(buf, g, lane, bq[0][g]);
+#pragma unroll
+                for (int gi = 0; gi < GN; ++gi) {
+#pragma unroll
+                    for (int g = 0; g < kSplitG; ++g)
+                        if (gi + 1 < GN) stage_tile(buf, (gi + 1) * kSplitG + g, lane, bq[(gi + 1) & 1][g]);
+                    __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of these MFMAs
+#pragma unroll
+                    for (int k = 0; k < 6; ++k)
+#pragma unroll
+                        for (int g = 0; g < kSplitG; ++g) {
+                            const int t = hh * TPS + gi * kSplitG + g;
+                            acc[p][t] = mfma_piece(k, ap[p], bq[gi & 1][g], acc[p][t]);
+                        }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 stage_sync();
             }
