@@ -73,6 +73,7 @@ struct DevConsts {
     float2* ph32 = nullptr;
     float* w_semi32 = nullptr;   // w_semi rounded to FP32
     float* sinv32 = nullptr;
+    int* trow = nullptr;         // [bw][bw+1]: start of row i of order m's transposed table
 };
 
 // Zero-padded, parity-split Legendre matrices for the MFMA GEMMs of the fast

@@ -261,6 +261,197 @@ __global__ __launch_bounds__(256) void k_naive_fwd(const double2* __restrict__ X
 
 // Cosine coefficients of every seminaive order from the transposed tables.
 // grid (cut, ceil(F/fc)).  Output rows of Cb as the forward uses them.
+// ---- projections for a few fields: the tables streamed as coalesced GEMVs ----
+// With one to four fields a call reads every table entry once and uses it
+// for at most 16 products, so the projections are bound by the table stream.
+// 16 lanes take one table row (128 contiguous bytes per load instead of one
+// row per lane; every load of a row issued at once), a block takes rpb rows
+// of one order, and the order's
+// data (DCT output, or coefficients) sits in LDS.
+
+typedef double dpair8 __attribute__((ext_vector_type(2))) __attribute__((aligned(8)));
+
+// A table row of rs entries, 16 lanes: lane j holds entries 32it + 2j and
+// 32it + 2j + 1 (zero past the row), all MAXI loads issued at once.
+template <int MAXI>
+__device__ __forceinline__ void row_load(const double* __restrict__ tp, int rs, int j, dpair8 (&tv)[MAXI]) {
+#pragma unroll
+    for (int it = 0; it < MAXI; ++it) {
+        const int t = 32 * it + 2 * j;
+        if (t + 1 < rs)
+            tv[it] = *reinterpret_cast<const dpair8*>(tp + t);
+        else if (t < rs)
+            tv[it] = dpair8{tp[t], 0.0};
+        else
+            tv[it] = dpair8{0.0, 0.0};
+    }
+}
+
+__device__ __forceinline__ double sum16(double v) {
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+    return v;
+}
+
+// Forward (DLTSemi's projections, seminaive.c:183-197) for orders m < cutoff:
+// row l of order m against the order's cosine coefficients Cb[f][+-m][k].
+template <int NF, int MAXI>
+__global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict__ Cb, const double* __restrict__ T,
+                                                      const long* __restrict__ toff, double* __restrict__ cre,
+                                                      double* __restrict__ cim, int N, int B, int F, int fmt, int rpb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* cs = reinterpret_cast<double2*>(smem);
+    const int m = blockIdx.y, f0 = blockIdx.z * NF;
+    const int L = B - m;
+    const int l0 = blockIdx.x * rpb;  // first row (degree offset l - m) of this block
+    if (l0 >= L) return;
+    const int nf = min(NF, F - f0);
+    const int nsign = (m > 0 && fmt == 0) ? 2 : 1;
+    for (int i = threadIdx.x; i < nf * nsign * B; i += blockDim.x) {
+        const int k = i % B, rest = i / B, s = rest % nsign, fl = rest / nsign;
+        cs[i] = Cb[((size_t)(f0 + fl) * N + (s ? N - m : m)) * B + k];
+    }
+    __syncthreads();
+    const double* tm = T + toff[m];
+    const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    const int lend = min(L, l0 + rpb);
+    for (int li = l0 + g; li < lend; li += 16) {
+        const int l = m + li;
+        const int rs = row_size(m, l);
+        const double* tp = tm + table_offset(m, l);
+        const double2* c = cs + (li & 1);
+        double ar[NF][2], ai[NF][2];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) ar[f][s] = ai[f][s] = 0.0;
+        dpair8 tv[MAXI];
+        row_load<MAXI>(tp, rs, j, tv);
+#pragma unroll
+        for (int it = 0; it < MAXI; ++it)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int t = 32 * it + 2 * j + e;
+                if (t < rs) {
+#pragma unroll
+                    for (int f = 0; f < NF; ++f)
+#pragma unroll
+                        for (int s = 0; s < 2; ++s)
+                            if (f < nf && s < nsign) {
+                                const double2 cv = c[(f * nsign + s) * B + 2 * t];
+                                ar[f][s] += cv.x * tv[it][e];
+                                ai[f][s] += cv.y * tv[it][e];
+                            }
+                }
+            }
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                ar[f][s] = sum16(ar[f][s]);
+                ai[f][s] = sum16(ai[f][s]);
+            }
+        if (j != 0) continue;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            if (f >= nf) break;
+            const size_t fb = (size_t)(f0 + f) * B2;
+            const int p = index_of_coeff(m, l, B);
+            cre[fb + p] = ar[f][0];
+            cim[fb + p] = ai[f][0];
+            if (nsign == 2) {
+                const int q = index_of_coeff(-m, l, B);
+                cre[fb + q] = sg * ar[f][1];
+                cim[fb + q] = sg * ai[f][1];
+            } else if (fmt == 1 && m > 0) {  // REAL: f(l,-m) = (-1)^m conj f(l,m)
+                const int q = index_of_coeff(-m, l, B);
+                cre[fb + q] = sg * ar[f][0];
+                cim[fb + q] = -sg * ai[f][0];
+            }
+        }
+    }
+}
+
+// Inverse (InvDLTSemi's transposed projections, seminaive.c:74-95) for orders
+// m < cutoff: row i of order m's transposed table against the coefficients
+// m+first, m+first+2, ...; trow[m][i] is where row i starts in the order.
+template <int NF, int MAXI>
+__global__ __launch_bounds__(256) void k_leg_inv_rows(const double* __restrict__ cre, const double* __restrict__ cim,
+                                                      const double* __restrict__ TT, const long* __restrict__ toff,
+                                                      const int* __restrict__ trow, double2* __restrict__ Cb, int N,
+                                                      int B, int F, int fmt, int rpb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* co = reinterpret_cast<double2*>(smem);
+    const int m = blockIdx.y, f0 = blockIdx.z * NF;
+    const int i0 = blockIdx.x * rpb;
+    if (i0 >= B) return;
+    const int nf = min(NF, F - f0);
+    const int nsign = (m > 0 && fmt == 0) ? 2 : 1;
+    const int L = B - m;
+    const size_t B2 = (size_t)B * B;
+    for (int i = threadIdx.x; i < nf * nsign * L; i += blockDim.x) {
+        const int li = i % L, rest = i / L, s = rest % nsign, fl = rest / nsign;
+        const int p = index_of_coeff(s ? -m : m, m + li, B);
+        co[i] = make_double2(cre[(f0 + fl) * B2 + p], cim[(f0 + fl) * B2 + p]);
+    }
+    __syncthreads();
+    const double* tt = TT + toff[m];
+    const int* rp = trow + (size_t)m * (B + 1);
+    const double scale = 0.5 / sqrt((double)B);
+    const double scale0 = 1.0 / sqrt((double)N);
+    const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+    const int iend = min(B, i0 + rpb);
+    for (int i = i0 + g; i < iend; i += 16) {
+        double vr[NF][2], vi[NF][2];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) vr[f][s] = vi[f][s] = 0.0;
+        if (!(i == B - 1 && (m & 1))) {
+            const int first = transpose_first(i, m);
+            const int r0 = rp[i], rs = rp[i + 1] - r0;
+            const double* tp = tt + r0;
+            const double2* a = co + first;
+            dpair8 tv[MAXI];
+            row_load<MAXI>(tp, rs, j, tv);
+#pragma unroll
+            for (int it = 0; it < MAXI; ++it)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int k = 32 * it + 2 * j + e;
+                    if (k < rs) {
+#pragma unroll
+                        for (int f = 0; f < NF; ++f)
+#pragma unroll
+                            for (int s = 0; s < 2; ++s)
+                                if (f < nf && s < nsign) {
+                                    const double2 av = a[(f * nsign + s) * L + 2 * k];
+                                    vr[f][s] += av.x * tv[it][e];
+                                    vi[f][s] += av.y * tv[it][e];
+                                }
+                    }
+                }
+            const double sc = (i == 0) ? scale0 : scale;
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    vr[f][s] = sum16(vr[f][s]) * sc;
+                    vi[f][s] = sum16(vi[f][s]) * sc;
+                }
+        }
+        if (j != 0) continue;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (f < nf && s < nsign)
+                    Cb[((size_t)(f0 + f) * N + (s ? N - m : m)) * B + i] = make_double2(vr[f][s], vi[f][s]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_leg_inv(const double* __restrict__ cre, const double* __restrict__ cim,
                                                  const double* __restrict__ TT, const long* __restrict__ toff,
                                                  double2* __restrict__ Cb, int N, int B, int F, int fc, int fmt) {
@@ -720,6 +911,14 @@ void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s)
         sv[t] = sin((2.0 * t + 1.0) * M_PI / (2.0 * N));
         sv32[t] = (float)sv[t];
     }
+    std::vector<int> trow((size_t)bw * (bw + 1));
+    for (int m = 0; m < bw; ++m) {
+        int* r = trow.data() + (size_t)m * (bw + 1);
+        r[0] = 0;
+        for (int i = 0; i < bw; ++i) r[i + 1] = r[i] + transpose_row_size(i, m, bw);
+    }
+    S2_HIP(hipMalloc(&c.trow, sizeof(int) * trow.size()));
+    S2_HIP(hipMemcpy(c.trow, trow.data(), sizeof(int) * trow.size(), hipMemcpyHostToDevice));
     std::vector<int> deg((size_t)bw * bw);
     for (int m = -(bw - 1); m < bw; ++m)
         for (int l = m < 0 ? -m : m; l < bw; ++l) deg[index_of_coeff(m, l, bw)] = l;
@@ -746,7 +945,7 @@ void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s)
 
 void consts_destroy(DevConsts& c) {
     for (void* p : {(void*)c.w_semi, (void*)c.w_naive, (void*)c.tw, (void*)c.ph, (void*)c.sinv, (void*)c.deg,
-                    (void*)c.tw32, (void*)c.ph32, (void*)c.w_semi32, (void*)c.sinv32})
+                    (void*)c.tw32, (void*)c.ph32, (void*)c.w_semi32, (void*)c.sinv32, (void*)c.trow})
         if (p) (void)hipFree(p);
     c = DevConsts{};
 }
@@ -853,6 +1052,28 @@ static void check_launch_shape(const DevConsts& c) {
     if (c.N > 2048) throw ArgError{"bandwidth above 1024 is not supported"};
 }
 
+// Calls of at most kRowsMaxFields fields use the row-GEMV projections, nf
+// fields per block (their data must fit 64 KiB of LDS: 2 signs x bw x 16 B each).
+constexpr int kRowsMaxFields = 4;
+static int rows_fields(int F, int B) {
+    const int cap = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
+    const int nf = F <= 1 ? 1 : F <= 2 ? 2 : 4;
+    return nf <= cap ? nf : (cap >= 2 ? 2 : 1);
+}
+
+// Table rows per block: long enough that a block's table stream dwarfs its
+// data fill (measured on MI355X: 64 rows at bw <= 128 up to 512 at bw = 1024).
+static int rows_per_block(int B) { return std::max(64, B / 2); }
+
+// Row loads per lane: a row holds at most bw/2 + 1 entries (32 per load round).
+template <typename L, typename K>
+static void rows_launch(int B, int nf, L&& launch, K f1a, K f2a, K f4a, K f1b, K f2b, K f4b, K f1c, K f2c) {
+    const int maxi = (B / 2 + 1 + 31) / 32;
+    if (maxi <= 5) launch(nf == 1 ? f1a : nf == 2 ? f2a : f4a);
+    else if (maxi <= 9) launch(nf == 1 ? f1b : nf == 2 ? f2b : f4b);
+    else launch(nf == 1 ? f1c : f2c);  // bw > 512: at most 2 fields fit the LDS
+}
+
 void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* cre,
                   double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
@@ -872,10 +1093,20 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
             X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
         }
-        const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
         Span sp(tm, ST_LEG_FWD, s);
-        k_leg_fwd<<<dim3(cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
-            Cb, t.fwd, t.off, cre, cim, N, B, F, fc, f, 0);
+        if (F <= kRowsMaxFields) {
+            const int nf = rows_fields(F, B);
+            const int rpb = rows_per_block(B);
+            const dim3 grid((B + rpb - 1) / rpb, cut, (F + nf - 1) / nf);
+            const size_t lds = sizeof(double2) * 2 * B * nf;
+            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb); },
+                        k_leg_fwd_rows<1, 5>, k_leg_fwd_rows<2, 5>, k_leg_fwd_rows<4, 5>, k_leg_fwd_rows<1, 9>,
+                        k_leg_fwd_rows<2, 9>, k_leg_fwd_rows<4, 9>, k_leg_fwd_rows<1, 17>, k_leg_fwd_rows<2, 17>);
+        } else {
+            const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
+            k_leg_fwd<<<dim3(cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
+                Cb, t.fwd, t.off, cre, cim, N, B, F, fc, f, 0);
+        }
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
@@ -899,7 +1130,17 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         const size_t lds = head + sizeof(double2) * 2 * B * std::min(fc, F);
         {
         Span sp(tm, ST_LEG_INV, s);
-        k_leg_inv<<<dim3(cut, (F + fc - 1) / fc), 256, lds, s>>>(cre, cim, t.inv, t.off, Cb, N, B, F, fc, f);
+        if (F <= kRowsMaxFields) {
+            const int nf = rows_fields(F, B);
+            const int rpb = rows_per_block(B);
+            const dim3 grid((B + rpb - 1) / rpb, cut, (F + nf - 1) / nf);
+            const size_t lr = sizeof(double2) * 2 * B * nf;
+            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lr, s>>>(cre, cim, t.inv, t.off, c.trow, Cb, N, B, F, f, rpb); },
+                        k_leg_inv_rows<1, 5>, k_leg_inv_rows<2, 5>, k_leg_inv_rows<4, 5>, k_leg_inv_rows<1, 9>,
+                        k_leg_inv_rows<2, 9>, k_leg_inv_rows<4, 9>, k_leg_inv_rows<1, 17>, k_leg_inv_rows<2, 17>);
+        } else {
+            k_leg_inv<<<dim3(cut, (F + fc - 1) / fc), 256, lds, s>>>(cre, cim, t.inv, t.off, Cb, N, B, F, fc, f);
+        }
         S2_HIP(hipGetLastError());
         }
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
