@@ -16,7 +16,10 @@
 
 #include "s2kit/util.h"
 
+#ifndef S2KIT_AMD_FFTW_PLAN
+#define S2KIT_AMD_FFTW_PLAN
 typedef struct fftw_plan_s* fftw_plan;
+#endif
 
 #ifdef __cplusplus
 extern "C" {

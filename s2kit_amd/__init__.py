@@ -56,6 +56,11 @@ SIGNATURES = {
     "InvFSTSemiMemo": (None, [_dp, _dp, _dp, _dp, _i, _dpp, _dp, _i, _i, _vp, _vp]),
     "FZTSemiMemo": (None, [_dp, _dp, _dp, _dp, _i, _dp, _dp, _i, _vp, _dp]),
     "ConvOn2SphereSemiMemo": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _i, _dp]),
+    # FST_semi_fly.h
+    "FSTSemiFly": (None, [_dp, _dp, _dp, _dp, _i, _dp, _i, _i, _vp, _vp, _dp]),
+    "InvFSTSemiFly": (None, [_dp, _dp, _dp, _dp, _i, _dp, _i, _i, _vp, _vp]),
+    "FZTSemiFly": (None, [_dp, _dp, _dp, _dp, _i, _dp, _i, _vp, _dp]),
+    "ConvOn2SphereSemiFly": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _i, _dp]),
     # s2kit_amd.h
     "s2amd_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i]),
     "s2amd_plan_destroy": (_i, [_vp]),
@@ -190,12 +195,40 @@ def fzt(rdata, idata, bw, tables, w=None, data_format=COMPLEX):
     return rr, ir
 
 
-def conv(rdata, idata, rfilter, ifilter, bw):
-    """ConvOn2SphereSemiMemo on host arrays; returns (rres, ires)."""
+def conv(rdata, idata, rfilter, ifilter, bw, fly=False):
+    """ConvOn2SphereSemiMemo (or ConvOn2SphereSemiFly) on host arrays; returns (rres, ires)."""
     a = [_f64(x).ravel() for x in (rdata, idata, rfilter, ifilter)]
     n = 2 * bw
     rr, ir = np.empty(n * n), np.empty(n * n)
-    lib().ConvOn2SphereSemiMemo(*[_p(x) for x in a], _p(rr), _p(ir), bw, None)
+    fn = lib().ConvOn2SphereSemiFly if fly else lib().ConvOn2SphereSemiMemo
+    fn(*[_p(x) for x in a], _p(rr), _p(ir), bw, None)
+    return rr, ir
+
+
+def fst_fly(rdata, idata, bw, cutoff, w=None, data_format=COMPLEX):
+    """FSTSemiFly on host arrays (the library makes the tables); returns (rcoeffs, icoeffs)."""
+    rdata, idata = _f64(rdata).ravel(), _f64(idata).ravel()
+    w = weights(bw) if w is None else _f64(w)
+    rc, ic = np.empty(bw * bw), np.empty(bw * bw)
+    lib().FSTSemiFly(_p(rdata), _p(idata), _p(rc), _p(ic), bw, None, data_format, cutoff, None, None, _p(w))
+    return rc, ic
+
+
+def inv_fst_fly(rcoeffs, icoeffs, bw, cutoff, data_format=COMPLEX):
+    """InvFSTSemiFly on host arrays; returns (rdata, idata)."""
+    rc, ic = _f64(rcoeffs), _f64(icoeffs)
+    n = 2 * bw
+    rd, idd = np.empty(n * n), np.empty(n * n)
+    lib().InvFSTSemiFly(_p(rc), _p(ic), _p(rd), _p(idd), bw, None, data_format, cutoff, None, None)
+    return rd, idd
+
+
+def fzt_fly(rdata, idata, bw, w=None, data_format=COMPLEX):
+    """FZTSemiFly on host arrays; returns (rres, ires) of length bw."""
+    rdata, idata = _f64(rdata).ravel(), _f64(idata).ravel()
+    w = weights(bw) if w is None else _f64(w)
+    rr, ir = np.empty(bw), np.empty(bw)
+    lib().FZTSemiFly(_p(rdata), _p(idata), _p(rr), _p(ir), bw, None, data_format, None, _p(w))
     return rr, ir
 
 

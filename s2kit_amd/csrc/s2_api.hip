@@ -23,6 +23,7 @@
 #include "s2_internal.h"
 
 extern "C" {
+#include "s2kit/FST_semi_fly.h"
 #include "s2kit/FST_semi_memo.h"
 #include "s2kit/cospml.h"
 #include "s2kit/pml.h"
@@ -524,6 +525,66 @@ extern "C" double** Transpose_SemiNaive_Naive_Pml_Table(double** seminaive_naive
 // ===========================================================================
 // drop-in: transforms
 // ===========================================================================
+// FSTSemiMemo / FSTSemiFly on device tables t (forward layout).
+static void dropin_forward(Ctx& c, const DevTables& t, const double* rdata, const double* idata, double* rcoeffs,
+                           double* icoeffs, int bw, DataFormat data_format, const double* weights) {
+    const int N = 2 * bw;
+    const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
+    const DevConsts& k0 = device_consts(c, bw, weights);
+    c.re.reserve(g);
+    c.im.reserve(g);
+    c.cre.reserve(k);
+    c.cim.reserve(k);
+    c.X.reserve(2 * g);
+    c.Cb.reserve(sizeof(double2) * N * bw);
+    S2_HIP(hipMemcpyAsync(c.re.p, rdata, g, hipMemcpyHostToDevice, c.stream));
+    S2_HIP(hipMemcpyAsync(c.im.p, idata, g, hipMemcpyHostToDevice, c.stream));
+    forward_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.cre.as<double>(), c.cim.as<double>(), 1,
+                 (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+    S2_HIP(hipMemcpyAsync(rcoeffs, c.cre.p, k, hipMemcpyDeviceToHost, c.stream));
+    S2_HIP(hipMemcpyAsync(icoeffs, c.cim.p, k, hipMemcpyDeviceToHost, c.stream));
+}
+
+// InvFSTSemiMemo / InvFSTSemiFly on device tables t (transposed layout).
+static void dropin_inverse(Ctx& c, const DevTables& t, const double* rcoeffs, const double* icoeffs, double* rdata,
+                           double* idata, int bw, DataFormat data_format) {
+    const int N = 2 * bw;
+    const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
+    const DevConsts& k0 = device_consts(c, bw, nullptr);
+    c.re.reserve(g);
+    c.im.reserve(g);
+    c.cre.reserve(k);
+    c.cim.reserve(k);
+    c.X.reserve(2 * g);
+    c.Cb.reserve(sizeof(double2) * N * bw);
+    S2_HIP(hipMemcpyAsync(c.cre.p, rcoeffs, k, hipMemcpyHostToDevice, c.stream));
+    S2_HIP(hipMemcpyAsync(c.cim.p, icoeffs, k, hipMemcpyHostToDevice, c.stream));
+    inverse_fp64(k0, t, c.cre.as<double>(), c.cim.as<double>(), c.re.as<double>(), c.im.as<double>(), 1,
+                 (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+    S2_HIP(hipMemcpyAsync(rdata, c.re.p, g, hipMemcpyDeviceToHost, c.stream));
+    S2_HIP(hipMemcpyAsync(idata, c.im.p, g, hipMemcpyDeviceToHost, c.stream));
+}
+
+// FZTSemiMemo / FZTSemiFly on device tables t whose order 0 is the cosine table.
+static void dropin_zonal(Ctx& c, const DevTables& t, const double* rdata, const double* idata, double* rres,
+                         double* ires, int bw, DataFormat data_format, const double* weights) {
+    const int N = 2 * bw;
+    const size_t g = sizeof(double) * N * N;
+    const DevConsts& k0 = device_consts(c, bw, weights);
+    c.re.reserve(g);
+    c.im.reserve(g);
+    c.cre.reserve(sizeof(double) * bw);
+    c.cim.reserve(sizeof(double) * bw);
+    c.X.reserve(2 * g);
+    c.Cb.reserve(sizeof(double2) * N * bw);
+    S2_HIP(hipMemcpyAsync(c.re.p, rdata, g, hipMemcpyHostToDevice, c.stream));
+    S2_HIP(hipMemcpyAsync(c.im.p, idata, g, hipMemcpyHostToDevice, c.stream));
+    zonal_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.cre.as<double>(), c.cim.as<double>(), 1,
+               (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
+    S2_HIP(hipMemcpyAsync(rres, c.cre.p, sizeof(double) * bw, hipMemcpyDeviceToHost, c.stream));
+    S2_HIP(hipMemcpyAsync(ires, c.cim.p, sizeof(double) * bw, hipMemcpyDeviceToHost, c.stream));
+}
+
 extern "C" void FSTSemiMemo(double* rdata, double* idata, double* rcoeffs, double* icoeffs, const int bw,
                             double** seminaive_naive_table, double* workspace, DataFormat data_format,
                             const int cutoff, fftw_plan* DCT_plan, fftw_plan* FFT_plan, double* weights) {
@@ -539,25 +600,12 @@ extern "C" void FSTSemiMemo(double* rdata, double* idata, double* rcoeffs, doubl
         need(icoeffs, "icoeffs");
         need(seminaive_naive_table, "seminaive_naive_table");
         need(weights, "weights");
-        const int N = 2 * bw;
-        const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
         ensure_device(c);
         const DevTables& t = device_tables(c, bw, cutoff, 0, seminaive_naive_table, bw);
-        const DevConsts& k0 = device_consts(c, bw, weights);
-        c.re.reserve(g);
-        c.im.reserve(g);
-        c.cre.reserve(k);
-        c.cim.reserve(k);
-        c.X.reserve(2 * g);
-        c.Cb.reserve(sizeof(double2) * N * bw);
-        S2_HIP(hipMemcpyAsync(c.re.p, rdata, g, hipMemcpyHostToDevice, c.stream));
-        S2_HIP(hipMemcpyAsync(c.im.p, idata, g, hipMemcpyHostToDevice, c.stream));
-        forward_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.cre.as<double>(), c.cim.as<double>(), 1,
-                     (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
-        S2_HIP(hipMemcpyAsync(rcoeffs, c.cre.p, k, hipMemcpyDeviceToHost, c.stream));
-        S2_HIP(hipMemcpyAsync(icoeffs, c.cim.p, k, hipMemcpyDeviceToHost, c.stream));
+        dropin_forward(c, t, rdata, idata, rcoeffs, icoeffs, bw, data_format, weights);
     });
 }
+
 
 extern "C" void InvFSTSemiMemo(double* rcoeffs, double* icoeffs, double* rdata, double* idata, const int bw,
                                double** transpose_seminaive_naive_table, double* workspace,
@@ -574,25 +622,12 @@ extern "C" void InvFSTSemiMemo(double* rcoeffs, double* icoeffs, double* rdata, 
         need(rcoeffs, "rcoeffs");
         need(icoeffs, "icoeffs");
         need(transpose_seminaive_naive_table, "transpose_seminaive_naive_table");
-        const int N = 2 * bw;
-        const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
         ensure_device(c);
         const DevTables& t = device_tables(c, bw, cutoff, 1, transpose_seminaive_naive_table, bw);
-        const DevConsts& k0 = device_consts(c, bw, nullptr);
-        c.re.reserve(g);
-        c.im.reserve(g);
-        c.cre.reserve(k);
-        c.cim.reserve(k);
-        c.X.reserve(2 * g);
-        c.Cb.reserve(sizeof(double2) * N * bw);
-        S2_HIP(hipMemcpyAsync(c.cre.p, rcoeffs, k, hipMemcpyHostToDevice, c.stream));
-        S2_HIP(hipMemcpyAsync(c.cim.p, icoeffs, k, hipMemcpyHostToDevice, c.stream));
-        inverse_fp64(k0, t, c.cre.as<double>(), c.cim.as<double>(), c.re.as<double>(), c.im.as<double>(), 1,
-                     (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
-        S2_HIP(hipMemcpyAsync(rdata, c.re.p, g, hipMemcpyDeviceToHost, c.stream));
-        S2_HIP(hipMemcpyAsync(idata, c.im.p, g, hipMemcpyDeviceToHost, c.stream));
+        dropin_inverse(c, t, rcoeffs, icoeffs, rdata, idata, bw, data_format);
     });
 }
+
 
 extern "C" void FZTSemiMemo(double* rdata, double* idata, double* rres, double* ires, const int bw,
                             double* cos_pml_table, double* workspace, DataFormat data_format, fftw_plan* DCT_plan,
@@ -607,26 +642,13 @@ extern "C" void FZTSemiMemo(double* rdata, double* idata, double* rres, double* 
         need(ires, "ires");
         need(cos_pml_table, "cos_pml_table");
         need(weights, "weights");
-        const int N = 2 * bw;
-        const size_t g = sizeof(double) * N * N;
         const double* p0 = cos_pml_table;
         ensure_device(c);
         const DevTables& t = device_tables(c, bw, bw, 2, &p0, 1);
-        const DevConsts& k0 = device_consts(c, bw, weights);
-        c.re.reserve(g);
-        c.im.reserve(g);
-        c.cre.reserve(sizeof(double) * bw);
-        c.cim.reserve(sizeof(double) * bw);
-        c.X.reserve(2 * g);
-        c.Cb.reserve(sizeof(double2) * N * bw);
-        S2_HIP(hipMemcpyAsync(c.re.p, rdata, g, hipMemcpyHostToDevice, c.stream));
-        S2_HIP(hipMemcpyAsync(c.im.p, idata, g, hipMemcpyHostToDevice, c.stream));
-        zonal_fp64(k0, t, c.re.as<double>(), c.im.as<double>(), c.cre.as<double>(), c.cim.as<double>(), 1,
-                   (Fmt)data_format, c.X.as<double2>(), c.Cb.as<double2>(), c.stream);
-        S2_HIP(hipMemcpyAsync(rres, c.cre.p, sizeof(double) * bw, hipMemcpyDeviceToHost, c.stream));
-        S2_HIP(hipMemcpyAsync(ires, c.cim.p, sizeof(double) * bw, hipMemcpyDeviceToHost, c.stream));
+        dropin_zonal(c, t, rdata, idata, rres, ires, bw, data_format, weights);
     });
 }
+
 
 extern "C" void TransMult(double* rdatacoeffs, double* idatacoeffs, double* rfiltercoeffs, double* ifiltercoeffs,
                           double* rres, double* ires, const int bw) {
@@ -655,27 +677,31 @@ extern "C" void TransMult(double* rdatacoeffs, double* idatacoeffs, double* rfil
     });
 }
 
-// Library-generated (bw, cutoff = bw) tables for ConvOn2SphereSemiMemo.
-static const DevTables& conv_tables(Ctx& c, int bw) {
+// Library-generated tables (both directions) for the drop-ins that take no
+// tables: ConvOn2SphereSemiMemo and the FST_semi_fly functions, which the
+// reference builds on every call (FST_semi_memo.c:498-500, FST_semi_fly.c:
+// 96-187, 259-331); the same values, generated on the GPU once per (bw, cutoff).
+static const DevTables& gen_tables(Ctx& c, int bw, int cutoff) {
     for (auto it = c.tables.begin(); it != c.tables.end(); ++it)
-        if (it->bw == bw && it->cutoff == bw && it->dir == 3 && it->device == c.device) {
+        if (it->bw == bw && it->cutoff == cutoff && it->dir == 3 && it->device == c.device) {
             c.tables.splice(c.tables.begin(), c.tables, it);
             return c.tables.front().t;
         }
     CachedTables e;
     e.bw = bw;
-    e.cutoff = bw;
+    e.cutoff = cutoff;
     e.dir = 3;
     e.device = c.device;
-    tables_generate(e.t, bw, bw, true, true, c.stream);
+    tables_generate(e.t, bw, cutoff, true, true, c.stream);
     c.tables.push_front(std::move(e));
     return c.tables.front().t;
 }
 
-extern "C" void ConvOn2SphereSemiMemo(double* rdata, double* idata, double* rfilter, double* ifilter, double* rres,
-                                      double* ires, const int bw, double* workspace) {
-    (void)workspace;
-    run_dropin("ConvOn2SphereSemiMemo", [&](Ctx& c) {
+// ConvOn2SphereSemiMemo / ConvOn2SphereSemiFly: REAL data, cutoff = bw
+// (FST_semi_memo.c:502-507, FST_semi_fly.c:505-511).
+static void dropin_conv(const char* fn, double* rdata, double* idata, double* rfilter, double* ifilter, double* rres,
+                        double* ires, int bw) {
+    run_dropin(fn, [&](Ctx& c) {
         check_common(bw, 0);
         for (const void* p : {(const void*)rdata, (const void*)idata, (const void*)rfilter, (const void*)ifilter,
                               (const void*)rres, (const void*)ires})
@@ -683,7 +709,7 @@ extern "C" void ConvOn2SphereSemiMemo(double* rdata, double* idata, double* rfil
         const int N = 2 * bw;
         const size_t g = sizeof(double) * N * N, k = sizeof(double) * bw * bw;
         ensure_device(c);
-        const DevTables& t = conv_tables(c, bw);
+        const DevTables& t = gen_tables(c, bw, bw);
         const DevConsts& k0 = device_consts(c, bw, nullptr);
         c.re.reserve(g);
         c.im.reserve(g);
@@ -712,6 +738,74 @@ extern "C" void ConvOn2SphereSemiMemo(double* rdata, double* idata, double* rfil
         S2_HIP(hipMemcpyAsync(rres, c.re.p, g, hipMemcpyDeviceToHost, c.stream));
         S2_HIP(hipMemcpyAsync(ires, c.im.p, g, hipMemcpyDeviceToHost, c.stream));
     });
+}
+
+extern "C" void ConvOn2SphereSemiMemo(double* rdata, double* idata, double* rfilter, double* ifilter, double* rres,
+                                      double* ires, const int bw, double* workspace) {
+    (void)workspace;
+    dropin_conv("ConvOn2SphereSemiMemo", rdata, idata, rfilter, ifilter, rres, ires, bw);
+}
+
+// ===========================================================================
+// drop-in: FST_semi_fly.h -- the same transforms with tables the library makes
+// ===========================================================================
+extern "C" void FSTSemiFly(double* rdata, double* idata, double* rcoeffs, double* icoeffs, const int bw,
+                           double* workspace, DataFormat data_format, const int cutoff, fftw_plan* DCT_plan,
+                           fftw_plan* FFT_plan, double* weights) {
+    (void)workspace;
+    (void)DCT_plan;
+    (void)FFT_plan;
+    run_dropin("FSTSemiFly", [&](Ctx& c) {
+        check_common(bw, (int)data_format);
+        if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+        need(rdata, "rdata");
+        need(idata, "idata");
+        need(rcoeffs, "rcoeffs");
+        need(icoeffs, "icoeffs");
+        need(weights, "weights");
+        ensure_device(c);
+        dropin_forward(c, gen_tables(c, bw, cutoff), rdata, idata, rcoeffs, icoeffs, bw, data_format, weights);
+    });
+}
+
+extern "C" void InvFSTSemiFly(double* rcoeffs, double* icoeffs, double* rdata, double* idata, const int bw,
+                              double* workspace, DataFormat data_format, const int cutoff, fftw_plan* inv_DCT_plan,
+                              fftw_plan* inv_FFT_plan) {
+    (void)workspace;
+    (void)inv_DCT_plan;
+    (void)inv_FFT_plan;
+    run_dropin("InvFSTSemiFly", [&](Ctx& c) {
+        check_common(bw, (int)data_format);
+        if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
+        need(rdata, "rdata");
+        need(idata, "idata");
+        need(rcoeffs, "rcoeffs");
+        need(icoeffs, "icoeffs");
+        ensure_device(c);
+        dropin_inverse(c, gen_tables(c, bw, cutoff), rcoeffs, icoeffs, rdata, idata, bw, data_format);
+    });
+}
+
+extern "C" void FZTSemiFly(double* rdata, double* idata, double* rres, double* ires, const int bw,
+                           double* workspace, DataFormat data_format, fftw_plan* DCT_plan, double* weights) {
+    (void)workspace;
+    (void)DCT_plan;
+    run_dropin("FZTSemiFly", [&](Ctx& c) {
+        check_common(bw, (int)data_format);
+        need(rdata, "rdata");
+        need(idata, "idata");
+        need(rres, "rres");
+        need(ires, "ires");
+        need(weights, "weights");
+        ensure_device(c);
+        dropin_zonal(c, gen_tables(c, bw, bw), rdata, idata, rres, ires, bw, data_format, weights);
+    });
+}
+
+extern "C" void ConvOn2SphereSemiFly(double* rdata, double* idata, double* rfilter, double* ifilter, double* rres,
+                                     double* ires, const int bw, double* workspace) {
+    (void)workspace;
+    dropin_conv("ConvOn2SphereSemiFly", rdata, idata, rfilter, ifilter, rres, ires, bw);
 }
 
 // ===========================================================================

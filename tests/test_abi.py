@@ -11,7 +11,7 @@ import pytest
 from conftest import ROOT
 
 HEADERS = [os.path.join(ROOT, "include", "s2kit", h) for h in
-           ("FST_semi_memo.h", "cospml.h", "util.h", "weights.h", "pml.h")] + \
+           ("FST_semi_memo.h", "FST_semi_fly.h", "cospml.h", "util.h", "weights.h", "pml.h")] + \
           [os.path.join(ROOT, "include", "s2kit_amd.h")]
 
 
@@ -35,7 +35,8 @@ def S():
 
 def test_every_declared_symbol_is_exported(S):
     names = declared_functions()
-    assert {"FSTSemiMemo", "InvFSTSemiMemo", "FZTSemiMemo", "ConvOn2SphereSemiMemo", "TableSize",
+    assert {"FSTSemiMemo", "InvFSTSemiMemo", "FZTSemiMemo", "ConvOn2SphereSemiMemo", "FSTSemiFly",
+            "InvFSTSemiFly", "FZTSemiFly", "ConvOn2SphereSemiFly", "TableSize",
             "Transpose_RowSize", "s2amd_forward"} <= names
     L = S.lib()
     for n in sorted(names):

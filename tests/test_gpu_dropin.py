@@ -234,3 +234,50 @@ def test_table_cache_sees_in_place_rewrite(S, O):
     ot.fwd *= 2.0
     b = fst_with(S, rd, idd, bw, ot, 0, bw, w)
     assert crel(b[0], b[1], 2 * a[0], 2 * a[1]) < 1e-14
+
+
+# ---- FST_semi_fly.h: the same transforms with library-made tables -----------
+FLY_CASES = [(8, 8, 0), (17, 17, 1), (64, 22, 0), (64, 22, 1), (128, 54, 0), (64, 0, 0)]
+
+
+@pytest.mark.parametrize("bw,cutoff,fmt", FLY_CASES)
+def test_fly_matches_oracle(S, O, bw, cutoff, fmt):
+    # cutoff = bw/2 - 10 is test_s2_semi_fly.c:128's choice
+    print(f"fly bw={bw} cutoff={cutoff} fmt={fmt}", flush=True)
+    kind = 2 if fmt == 1 else 0
+    rc, ic = O.make_coeffs(bw + cutoff, bw, kind)
+    rd, idd = S.inv_fst_fly(rc, ic, bw, cutoff, data_format=fmt)
+    ord_, oid = O.inv_fst(rc, ic, bw, data_format=fmt, cutoff=cutoff)
+    assert crel(rd, idd, ord_, oid) < 1e-11
+    r2, i2 = S.fst_fly(ord_, oid, bw, cutoff, data_format=fmt)
+    orc, oic = O.fst(ord_, oid, bw, data_format=fmt, cutoff=cutoff)
+    assert crel(r2, i2, orc, oic) < 1e-11
+    # band-limited data: the round trip returns the coefficients (even bw only:
+    # for odd bw the reference's transposed rows are short, Transpose_RowSize)
+    if bw % 2 == 0:
+        assert crel(r2, i2, rc, ic) < 1e-11
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_fzt_fly(S, O, fmt):
+    bw = 64
+    rng = np.random.default_rng(6)
+    n = 2 * bw
+    rd, idd = rng.uniform(-1, 1, n * n), rng.uniform(-1, 1, n * n)
+    zr, zi = S.fzt_fly(rd, idd, bw, data_format=fmt)
+    ozr, ozi = O.fzt(rd, idd, bw, data_format=fmt)
+    assert crel(zr, zi, ozr, ozi) < 1e-11
+    if fmt == 1:
+        assert np.all(zi == 0.0)
+
+
+def test_conv_fly_golden(S):
+    # the conv goldens hold for ConvOn2SphereSemiFly too (same transform)
+    bw = 64
+    s = golden(f"conv{bw}_signal.npy")
+    f = np.repeat(golden(f"conv{bw}_filter.npy"), 2 * bw)
+    o = golden(f"conv{bw}_out.npy")
+    z = np.zeros_like(s)
+    rr, ir = S.conv(s, z, f, z, bw, fly=True)
+    assert np.abs(rr - o).max() < 1e-14
+    assert np.abs(ir).max() < 1e-14
