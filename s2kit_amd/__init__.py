@@ -56,6 +56,11 @@ SIGNATURES = {
     "InvFSTSemiMemo": (None, [_dp, _dp, _dp, _dp, _i, _dpp, _dp, _i, _i, _vp, _vp]),
     "FZTSemiMemo": (None, [_dp, _dp, _dp, _dp, _i, _dp, _dp, _i, _vp, _dp]),
     "ConvOn2SphereSemiMemo": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _i, _dp]),
+    # seminaive.h, naive.h
+    "DLTSemi": (None, [_dp, _i, _i, _dp, _dp, _dp, _dp, _vp]),
+    "InvDLTSemi": (None, [_dp, _i, _i, _dp, _dp, _dp, _dp, _vp]),
+    "DLTNaive": (None, [_dp, _i, _i, _dp, _dp, _dp, _dp]),
+    "InvDLTNaive": (None, [_dp, _i, _i, _dp, _dp]),
     # FST_semi_fly.h
     "FSTSemiFly": (None, [_dp, _dp, _dp, _dp, _i, _dp, _i, _i, _vp, _vp, _dp]),
     "InvFSTSemiFly": (None, [_dp, _dp, _dp, _dp, _i, _dp, _i, _i, _vp, _vp]),
@@ -203,6 +208,40 @@ def conv(rdata, idata, rfilter, ifilter, bw, fly=False):
     fn = lib().ConvOn2SphereSemiFly if fly else lib().ConvOn2SphereSemiMemo
     fn(*[_p(x) for x in a], _p(rr), _p(ir), bw, None)
     return rr, ir
+
+
+def dlt_semi(data, bw, m, table, w=None):
+    """DLTSemi of one order on host arrays; table from GenerateCosPmlTable(bw, m)."""
+    data, table = _f64(data), _f64(table)
+    w = weights(bw) if w is None else _f64(w)
+    out = np.empty(bw - m)
+    lib().DLTSemi(_p(data), bw, m, _p(out), None, _p(table), _p(w), None)
+    return out
+
+
+def inv_dlt_semi(coeffs, bw, m, ttable, sin_values):
+    """InvDLTSemi of one order on host arrays; ttable from TransposeCosPmlTable."""
+    coeffs, ttable, sin_values = _f64(coeffs), _f64(ttable), _f64(sin_values)
+    out = np.empty(2 * bw)
+    lib().InvDLTSemi(_p(coeffs), bw, m, _p(out), _p(ttable), _p(sin_values), None, None)
+    return out
+
+
+def dlt_naive(data, bw, m, pml, w=None):
+    """DLTNaive of one order on host arrays; pml from GeneratePmlTable(bw, m)."""
+    data, pml = _f64(data), _f64(pml)
+    w = weights(bw) if w is None else _f64(w)
+    out = np.empty(bw - m)
+    lib().DLTNaive(_p(data), bw, m, _p(w), _p(out), _p(pml), None)
+    return out
+
+
+def inv_dlt_naive(coeffs, bw, m, pml):
+    """InvDLTNaive of one order on host arrays."""
+    coeffs, pml = _f64(coeffs), _f64(pml)
+    out = np.empty(2 * bw)
+    lib().InvDLTNaive(_p(coeffs), bw, m, _p(out), _p(pml))
+    return out
 
 
 def fst_fly(rdata, idata, bw, cutoff, w=None, data_format=COMPLEX):

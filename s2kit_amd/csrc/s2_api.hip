@@ -25,6 +25,8 @@
 extern "C" {
 #include "s2kit/FST_semi_fly.h"
 #include "s2kit/FST_semi_memo.h"
+#include "s2kit/naive.h"
+#include "s2kit/seminaive.h"
 #include "s2kit/cospml.h"
 #include "s2kit/pml.h"
 #include "s2kit/util.h"
@@ -744,6 +746,110 @@ extern "C" void ConvOn2SphereSemiMemo(double* rdata, double* idata, double* rfil
                                       double* ires, const int bw, double* workspace) {
     (void)workspace;
     dropin_conv("ConvOn2SphereSemiMemo", rdata, idata, rfilter, ifilter, rres, ires, bw);
+}
+
+// ===========================================================================
+// drop-in: seminaive.h, naive.h -- one order at a time
+// ===========================================================================
+static void check_order(int bw, int m) {
+    check_common(bw, 0);
+    if (m < 0 || m >= bw) throw ArgError{"m must be in [0, bw)"};
+}
+
+extern "C" void DLTSemi(double* data, const int bw, const int m, double* result, double* workspace,
+                        double* cos_pml_table, double* weights, fftw_plan* plan) {
+    (void)workspace;
+    (void)plan;
+    run_dropin("DLTSemi", [&](Ctx& c) {
+        check_order(bw, m);
+        need(data, "data");
+        need(result, "result");
+        need(cos_pml_table, "cos_pml_table");
+        need(weights, "weights");
+        ensure_device(c);
+        const int N = 2 * bw;
+        const size_t nt = table_size(m, bw);
+        c.aux1.reserve(sizeof(double) * N);
+        c.aux2.reserve(sizeof(double) * nt);
+        c.aux3.reserve(sizeof(double) * 2 * N);
+        c.aux4.reserve(sizeof(double) * (bw - m));
+        S2_HIP(hipMemcpyAsync(c.aux1.p, data, sizeof(double) * N, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux2.p, cos_pml_table, sizeof(double) * nt, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux3.p, weights, sizeof(double) * 2 * N, hipMemcpyHostToDevice, c.stream));
+        dlt_semi_one(c.aux1.as<double>(), c.aux3.as<double>(), c.aux2.as<double>(), c.aux4.as<double>(), bw, m,
+                     c.stream);
+        S2_HIP(hipMemcpyAsync(result, c.aux4.p, sizeof(double) * (bw - m), hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void InvDLTSemi(double* coeffs, const int bw, const int m, double* result, double* trans_cos_pml_table,
+                           double* sin_values, double* workspace, fftw_plan* plan) {
+    (void)workspace;
+    (void)plan;
+    run_dropin("InvDLTSemi", [&](Ctx& c) {
+        check_order(bw, m);
+        need(coeffs, "coeffs");
+        need(result, "result");
+        need(trans_cos_pml_table, "trans_cos_pml_table");
+        need(sin_values, "sin_values");
+        ensure_device(c);
+        const int N = 2 * bw;
+        const size_t nt = table_size(m, bw);
+        c.aux1.reserve(sizeof(double) * (bw - m));
+        c.aux2.reserve(sizeof(double) * nt);
+        c.aux3.reserve(sizeof(double) * N);
+        c.aux4.reserve(sizeof(double) * N);
+        S2_HIP(hipMemcpyAsync(c.aux1.p, coeffs, sizeof(double) * (bw - m), hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux2.p, trans_cos_pml_table, sizeof(double) * nt, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux3.p, sin_values, sizeof(double) * N, hipMemcpyHostToDevice, c.stream));
+        inv_dlt_semi_one(c.aux1.as<double>(), c.aux2.as<double>(), c.aux3.as<double>(), c.aux4.as<double>(), bw,
+                         m, c.stream);
+        S2_HIP(hipMemcpyAsync(result, c.aux4.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void DLTNaive(double* data, const int bw, const int m, double* weights, double* result,
+                         double* pml_table, double* workspace) {
+    (void)workspace;
+    run_dropin("DLTNaive", [&](Ctx& c) {
+        check_order(bw, m);
+        need(data, "data");
+        need(weights, "weights");
+        need(result, "result");
+        need(pml_table, "pml_table");
+        ensure_device(c);
+        const int N = 2 * bw;
+        const size_t nt = (size_t)(bw - m) * N;
+        c.aux1.reserve(sizeof(double) * N);
+        c.aux2.reserve(sizeof(double) * nt);
+        c.aux3.reserve(sizeof(double) * N);
+        c.aux4.reserve(sizeof(double) * (bw - m));
+        S2_HIP(hipMemcpyAsync(c.aux1.p, data, sizeof(double) * N, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux2.p, pml_table, sizeof(double) * nt, hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux3.p, weights, sizeof(double) * N, hipMemcpyHostToDevice, c.stream));
+        dlt_naive_one(c.aux1.as<double>(), c.aux3.as<double>(), c.aux2.as<double>(), c.aux4.as<double>(), bw, m,
+                      c.stream);
+        S2_HIP(hipMemcpyAsync(result, c.aux4.p, sizeof(double) * (bw - m), hipMemcpyDeviceToHost, c.stream));
+    });
+}
+
+extern "C" void InvDLTNaive(double* coeffs, const int bw, const int m, double* result, double* pml_table) {
+    run_dropin("InvDLTNaive", [&](Ctx& c) {
+        check_order(bw, m);
+        need(coeffs, "coeffs");
+        need(result, "result");
+        need(pml_table, "pml_table");
+        ensure_device(c);
+        const int N = 2 * bw;
+        const size_t nt = (size_t)(bw - m) * N;
+        c.aux1.reserve(sizeof(double) * (bw - m));
+        c.aux2.reserve(sizeof(double) * nt);
+        c.aux4.reserve(sizeof(double) * N);
+        S2_HIP(hipMemcpyAsync(c.aux1.p, coeffs, sizeof(double) * (bw - m), hipMemcpyHostToDevice, c.stream));
+        S2_HIP(hipMemcpyAsync(c.aux2.p, pml_table, sizeof(double) * nt, hipMemcpyHostToDevice, c.stream));
+        inv_dlt_naive_one(c.aux1.as<double>(), c.aux2.as<double>(), c.aux4.as<double>(), bw, m, c.stream);
+        S2_HIP(hipMemcpyAsync(result, c.aux4.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
+    });
 }
 
 // ===========================================================================

@@ -170,6 +170,13 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
 void trans_mult(const DevConsts& c, const double* dr, const double* di, const double* hr, const double* hi,
                 long fstride, double* rr, double* ri, int F, hipStream_t s, StageTimer* tm = nullptr);
 
+// ---- one order (s2_kernels.hip): DLTSemi, InvDLTSemi, DLTNaive, InvDLTNaive on device arrays ----
+void dlt_semi_one(const double* data, const double* w, const double* tab, double* out, int B, int m, hipStream_t s);
+void inv_dlt_semi_one(const double* coeffs, const double* ttab, const double* sinv, double* out, int B, int m,
+                      hipStream_t s);
+void dlt_naive_one(const double* data, const double* w, const double* pml, double* out, int B, int m, hipStream_t s);
+void inv_dlt_naive_one(const double* coeffs, const double* pml, double* out, int B, int m, hipStream_t s);
+
 // ---- fast path (s2_fast.hip): power-of-two grids, cutoff = bw ----------
 bool fast_supported(int bw, int cutoff, int prec);
 void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t s);

@@ -261,6 +261,129 @@ __global__ __launch_bounds__(256) void k_naive_fwd(const double2* __restrict__ X
 
 // Cosine coefficients of every seminaive order from the transposed tables.
 // grid (cut, ceil(F/fc)).  Output rows of Cb as the forward uses them.
+// ---- one order at a time: DLTSemi / InvDLTSemi / DLTNaive / InvDLTNaive ----
+// The reference's per-order Legendre transforms (seminaive.c, naive.c) as
+// single-block FP64 kernels.  The cosine transforms are the REDFT10 / REDFT01
+// definitions summed directly over the bw cosine indices that are nonzero,
+// with cos(pi q / 2N) from an LDS table of the first quadrant.
+__device__ __forceinline__ double cos_q(const double* c1, long q, int N) {  // cos(pi q / (2N))
+    long r = q % (4L * N);
+    if (r > 2L * N) r = 4L * N - r;
+    return (r > N) ? -c1[2 * N - r] : c1[r];
+}
+
+// DLTSemi (seminaive.c:153-198): weights, DCT-II, projections on the
+// zero-stripped cosine table of order m.  LDS: (N+1) + N + (bw+2) doubles.
+__global__ __launch_bounds__(256) void k_dlt_semi_one(const double* __restrict__ data, const double* __restrict__ w,
+                                                      const double* __restrict__ tab, double* __restrict__ out,
+                                                      int B, int m) {
+    extern __shared__ double sh[];
+    const int N = 2 * B;
+    double* c1 = sh;
+    double* wd = c1 + N + 1;
+    double* cd = wd + N;
+    for (int r = threadIdx.x; r <= N; r += blockDim.x) c1[r] = cospi((double)r / (2.0 * N));
+    for (int i = threadIdx.x; i < N; i += blockDim.x) wd[i] = data[i] * w[(m & 1) ? N + i : i];
+    __syncthreads();
+    const int nk = min(N, B + 2);
+    const double nrm = 1.0 / sqrt(2.0 * N);
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+        double v = 0.0;
+        for (int j = 0; j < N; ++j) v += wd[j] * cos_q(c1, (long)k * (2 * j + 1), N);
+        v *= 2.0 * nrm;
+        cd[k] = (k == 0) ? v * 0.70710678118654752440 : v;
+    }
+    __syncthreads();
+    for (int i = m + threadIdx.x; i < B; i += blockDim.x) {
+        const int toggle = (i - m) & 1;
+        const double* p = tab + table_offset(m, i);
+        double v = 0.0;
+        for (int j = 0; j < i / 2; ++j) v += cd[2 * j + toggle] * p[j];
+        if (!((i - m) & 1) || !(m & 1)) v += cd[2 * (i / 2) + toggle] * p[i / 2];
+        out[i - m] = v;
+    }
+}
+
+// InvDLTSemi (seminaive.c:56-115): transposed projections to the cosine
+// series, DCT-III, times sin(theta) for odd m.  LDS: (N+1) + bw doubles.
+__global__ __launch_bounds__(256) void k_inv_dlt_semi_one(const double* __restrict__ coeffs,
+                                                          const double* __restrict__ ttab,
+                                                          const double* __restrict__ sinv, double* __restrict__ out,
+                                                          int B, int m) {
+    extern __shared__ double sh[];
+    const int N = 2 * B;
+    double* c1 = sh;
+    double* fc = c1 + N + 1;
+    for (int r = threadIdx.x; r <= N; r += blockDim.x) c1[r] = cospi((double)r / (2.0 * N));
+    const double coeff = 0.5 / sqrt((double)B);
+    for (int i = threadIdx.x; i < B; i += blockDim.x) {
+        double v = 0.0;
+        if (!(i == B - 1 && (m & 1))) {
+            long off = 0;
+            for (int r = 0; r < i; ++r) off += transpose_row_size(r, m, B);
+            const double* a = coeffs + ((i > m) ? (i - m) + (m & 1) : (i & 1));
+            const int rs = transpose_row_size(i, m, B);
+            for (int j = 0; j < rs; ++j) v += a[2 * j] * ttab[off + j];
+            v = (i == 0) ? v / sqrt((double)N) : v * coeff;
+        }
+        fc[i] = v;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        double v = 0.0;
+        for (int j = 1; j < B; ++j) v += fc[j] * cos_q(c1, (long)j * (2 * k + 1), N);
+        v = fc[0] + 2.0 * v;
+        out[k] = (m & 1) ? v * sinv[k] : v;
+    }
+}
+
+// DLTNaive (naive.c:34-59): weighted samples against the (bw-m) x N sampled
+// table.  InvDLTNaive (naive.c:78-95): the table's rows summed with the
+// coefficients.
+__global__ __launch_bounds__(256) void k_dlt_naive_one(const double* __restrict__ data, const double* __restrict__ w,
+                                                       const double* __restrict__ pml, double* __restrict__ out,
+                                                       int B, int m) {
+    const int N = 2 * B;
+    for (int i = threadIdx.x; i < B - m; i += blockDim.x) {
+        const double* row = pml + (size_t)i * N;
+        double v = 0.0;
+        for (int j = 0; j < N; ++j) v += data[j] * w[j] * row[j];
+        out[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_inv_dlt_naive_one(const double* __restrict__ coeffs,
+                                                           const double* __restrict__ pml, double* __restrict__ out,
+                                                           int B, int m) {
+    const int N = 2 * B;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        double v = 0.0;
+        for (int i = 0; i < B - m; ++i) v += coeffs[i] * pml[(size_t)i * N + j];
+        out[j] = v;
+    }
+}
+
+void dlt_semi_one(const double* data, const double* w, const double* tab, double* out, int B, int m, hipStream_t s) {
+    const int N = 2 * B;
+    k_dlt_semi_one<<<1, 256, sizeof(double) * ((N + 1) + N + (B + 2)), s>>>(data, w, tab, out, B, m);  // <= 41 KiB
+    S2_HIP(hipGetLastError());
+}
+void inv_dlt_semi_one(const double* coeffs, const double* ttab, const double* sinv, double* out, int B, int m,
+                      hipStream_t s) {
+    const int N = 2 * B;
+    k_inv_dlt_semi_one<<<1, 256, sizeof(double) * ((N + 1) + B), s>>>(coeffs, ttab, sinv, out, B, m);
+    S2_HIP(hipGetLastError());
+}
+void dlt_naive_one(const double* data, const double* w, const double* pml, double* out, int B, int m,
+                   hipStream_t s) {
+    k_dlt_naive_one<<<1, 256, 0, s>>>(data, w, pml, out, B, m);
+    S2_HIP(hipGetLastError());
+}
+void inv_dlt_naive_one(const double* coeffs, const double* pml, double* out, int B, int m, hipStream_t s) {
+    k_inv_dlt_naive_one<<<1, 256, 0, s>>>(coeffs, pml, out, B, m);
+    S2_HIP(hipGetLastError());
+}
+
 // ---- projections for a few fields: the tables streamed as coalesced GEMVs ----
 // With one to four fields a call reads every table entry once and uses it
 // for at most 16 products, so the projections are bound by the table stream.

@@ -11,7 +11,8 @@ import pytest
 from conftest import ROOT
 
 HEADERS = [os.path.join(ROOT, "include", "s2kit", h) for h in
-           ("FST_semi_memo.h", "FST_semi_fly.h", "cospml.h", "util.h", "weights.h", "pml.h")] + \
+           ("FST_semi_memo.h", "FST_semi_fly.h", "cospml.h", "util.h", "weights.h", "pml.h",
+            "seminaive.h", "naive.h")] + \
           [os.path.join(ROOT, "include", "s2kit_amd.h")]
 
 

@@ -281,3 +281,29 @@ def test_conv_fly_golden(S):
     rr, ir = S.conv(s, z, f, z, bw, fly=True)
     assert np.abs(rr - o).max() < 1e-14
     assert np.abs(ir).max() < 1e-14
+
+
+# ---- seminaive.h, naive.h: one order at a time -------------------------------
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+@pytest.mark.parametrize("bw,m", [(8, 0), (8, 3), (17, 4), (17, 16), (64, 0), (64, 33), (256, 1), (256, 200)])
+def test_per_order_dlts_match_oracle(S, O, bw, m):
+    print(f"per-order DLTs bw={bw} m={m}", flush=True)
+    rng = np.random.default_rng(bw * 1000 + m)
+    n = 2 * bw
+    w = O.weights(bw)
+    data = rng.uniform(-1, 1, n)
+    coeffs = rng.uniform(-1, 1, bw - m)
+    sinv = np.sin((2 * np.arange(n) + 1) * np.pi / (2 * n))
+    ct = O.cospml_table(bw, m)
+    tt = O.transpose_cospml_table(bw, m, ct)
+    pml = O.pml_table(bw, m)
+    assert rel(S.dlt_semi(data, bw, m, ct, w), O.dlt_semi(data, bw, m, ct, w)) < 1e-11
+    assert rel(S.inv_dlt_semi(coeffs, bw, m, tt, sinv), O.inv_dlt_semi(coeffs, bw, m, tt, sinv)) < 1e-11
+    assert rel(S.dlt_naive(data, bw, m, pml, w), O.dlt_naive(data, bw, m, w, pml)) < 1e-11
+    assert rel(S.inv_dlt_naive(coeffs, bw, m, pml), O.inv_dlt_naive(coeffs, bw, m, pml)) < 1e-11
+    # the two forward algorithms agree on the grid's samples of one order
+    if bw % 2 == 0:
+        assert rel(S.dlt_semi(data, bw, m, ct, w), S.dlt_naive(data, bw, m, pml, w)) < 1e-11
