@@ -16,7 +16,8 @@ Also reported on the same JSON line:
               during the timed steps), against 8 TB/s;
   other_forms the FP64 form and the other FP32-accurate form (mixed: FP32
               MFMA GEMMs; split: BF16 MFMA with split operands) timed on the
-              same fields, with their errors against FP64;
+              same fields, with their errors against FP64; and the FP64 form
+              at cutoff = bw/4 (naive branch for orders >= 64);
   cpu_baseline  the CPU FP64 oracle (a port of the reference path) timed on
               the host, one process per core, on a bounded sample.
 
@@ -463,6 +464,16 @@ def main():
             forms[name] = {"pairs_per_s": v, "forward_vs_fp64": rel(cre, cim, ref_cr, ref_ci),
                            "inverse_vs_fp64": rel(ore, oim, gre, gim)}
             pl.close()
+        # FP64 with the naive branch for orders >= bw/4 (the reference's own
+        # round-trip cutoff); the same quadrature as cutoff = bw
+        pl = S.Plan(bw, F, precision=S.FP64, cutoff=bw // 4)
+        v = pairs_per_s(pl, 2)
+        pl.forward(gre, gim, cre, cim)
+        pl.inverse(c0r, c0i, ore, oim)
+        torch.cuda.synchronize()
+        forms["fp64_cutoff_bw/4"] = {"pairs_per_s": v, "forward_vs_fp64": rel(cre, cim, ref_cr, ref_ci),
+                                     "inverse_vs_fp64": rel(ore, oim, gre, gim)}
+        pl.close()
 
     # ---- batch-1 FP64 form: one field per call -----------------------------
     plain = None

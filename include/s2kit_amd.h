@@ -37,7 +37,7 @@ typedef enum {
     S2AMD_FP64 = 0,
     /* seminaive in FP32 (FFTs, DCTs, FP32-MFMA projections). Measured error
      * 1e-6 .. 4e-5 relative: the cosine-space sums cancel.  bw a power of two
-     * in [32, 1024], cutoff = bw. */
+     * in [32, 1024]. */
     S2AMD_FP32 = 1,
     /* mixed: FP32 ring FFTs and the naive Legendre transform (naive.c) as an
      * FP32-MFMA GEMM over m on tables from the FP64 recurrence; FP64 in/out.
@@ -53,8 +53,12 @@ typedef enum {
 typedef struct s2amd_plan s2amd_plan;
 
 /* Plan for bandwidth bw (2..1024), up to max_fields fields per call,
- * seminaive for orders < cutoff (cutoff = bw: all orders).  Generates the
- * forward and transposed tables on the current device. */
+ * seminaive for orders < cutoff and naive (naive.c) for orders >= cutoff
+ * (cutoff = bw: all seminaive).  Generates the forward and transposed tables
+ * on the current device.  FP64 and FP32 plans on power-of-two grids with bw
+ * in [32, 1024] run both branches as MFMA GEMMs at any cutoff; the mixed and
+ * split-BF16 forms use the naive transform for every order whatever the
+ * cutoff. */
 int s2amd_plan_create(s2amd_plan** plan, int bw, int cutoff, int max_fields, int precision);
 /* Waits for the device (work queued on any stream may use the plan), then frees. */
 int s2amd_plan_destroy(s2amd_plan* plan);

@@ -937,7 +937,7 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
         if (precision == S2AMD_SPLIT_BF16 && !mixed_supported(bw))
             throw ArgError{"the split-BF16 form needs a power-of-two grid with bw in [64, 512]"};
         if (precision == S2AMD_FP32 && !fast_supported(bw, cutoff, 1))
-            throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 1024] and cutoff = bw"};
+            throw ArgError{"the FP32 form needs a power-of-two grid with bw in [32, 1024]"};
         std::unique_ptr<s2amd_plan> p(new s2amd_plan());
         p->bw = bw;
         p->cutoff = cutoff;

@@ -5,7 +5,14 @@
 //   k_gemm_inv -> k_fdct_inv -> k_fphi_inv     inverse
 //
 //   k_gemm_fwd   seminaive projections as MFMA GEMMs            (seminaive.c:183-197)
+//                naive projections (orders >= cutoff) likewise  (naive.c:34-59)
 //   k_gemm_inv   transposed-table cosine sums as MFMA GEMMs     (seminaive.c:74-98)
+//                naive synthesis (orders >= cutoff) into X      (naive.c:78-95)
+//
+// With cutoff < bw the DCT stages run over the seminaive order rows only;
+// the naive orders' GEMMs read (forward) or write (inverse) the ring-FFT
+// rows X directly.  One launch per direction covers all orders, the branch
+// taken per workgroup.
 //
 // Legendre GEMMs.  For order m split the degrees by parity p = (l-m) mod 2:
 // degree l = m+p+2i reads cosine coefficients c[2t+p], t < RowSize(m,l), so
@@ -30,12 +37,28 @@ namespace s2 {
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_pack_gemm(const double* __restrict__ T, const double* __restrict__ TT, const long* __restrict__ toff,
-                            const GemmMeta* __restrict__ meta, R* __restrict__ AF, R* __restrict__ AI, int B) {
+                            const GemmMeta* __restrict__ meta, R* __restrict__ AF, R* __restrict__ AI, int B,
+                            int cut) {
     extern __shared__ __align__(16) unsigned char smem[];
     int* rowpos = reinterpret_cast<int*>(smem);
     const int m = blockIdx.x, p = blockIdx.y;
     const int N = 2 * B, half = B / 2;
     const GemmMeta g = meta[m * 2 + p];
+    if (m >= cut) {  // naive order: sampled P_l^m rows (pml.c:41-79), both directions the same table
+        const int Li16 = (g.Lp + 15) & ~15, Li4 = (g.Lp + 3) & ~3;
+        const double* pm = T + toff[m];
+        const double* pt = TT + toff[m];
+        for (int idx = threadIdx.x; idx < B * Li16; idx += blockDim.x) {
+            const int j = idx / Li16, i = idx - j * Li16;
+            AF[g.offF + idx] = (R)(i < g.Lp ? pm[(size_t)(p + 2 * i) * N + j] : 0.0);
+        }
+        const double inv_s2pi = 0.39894228040143267794;
+        for (int idx = threadIdx.x; idx < Li4 * B; idx += blockDim.x) {
+            const int i = idx / B, j = idx - i * B;
+            AI[g.offI + idx] = (R)(i < g.Lp ? pt[(size_t)(p + 2 * i) * N + j] * inv_s2pi : 0.0);
+        }
+        return;
+    }
     for (int i = threadIdx.x; i < B; i += blockDim.x) rowpos[i + 1] = transpose_row_size(i, m, B);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -81,21 +104,33 @@ __host__ __device__ constexpr int lds_pad() {
     return sizeof(R) == 4 ? 4 : 2;
 }
 
-// Forward: D'[c][i] = sum_t data[c][t] * A(m,p)[i][t].  grid (ceil(cols/ncb), bw).
-// LDS: Bd[c][k2], k2 = parity-split cosine position (p*bw/2 + t).
+// Coefficient (m, l) of data column col (field, +-m row, re/im) from the
+// projection v; REAL data also fills the -m run (f(l,-m) = (-1)^m conj f(l,m)).
+__device__ __forceinline__ void fwd_store(double v, int col, int m, int l, int B, int cpf, int fmt,
+                                          double* __restrict__ cre, double* __restrict__ cim) {
+    const size_t B2 = (size_t)B * B;
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    const int f = col / cpf, rem = col - f * cpf;
+    const int comp = rem & 1;
+    double* out = comp ? cim : cre;
+    if ((rem >> 1) == 0) {
+        out[f * B2 + index_of_coeff(m, l, B)] = v;
+        if (fmt == 1 && m > 0) out[f * B2 + index_of_coeff(-m, l, B)] = comp ? -sg * v : sg * v;
+    } else {
+        out[f * B2 + index_of_coeff(-m, l, B)] = sg * v;
+    }
+}
+
+// Seminaive order m < cutoff.  LDS: Bd[c][k2], k2 = parity-split cosine
+// position (p*bw/2 + t).
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb, const R* __restrict__ AF,
-                                                       const GemmMeta* __restrict__ meta, double* __restrict__ cre,
-                                                       double* __restrict__ cim, int lgB, int F, int ncb, int fmt) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    R* Bd = reinterpret_cast<R*>(smem);
+__device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __restrict__ Cb, const R* __restrict__ AF,
+                                              const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                              double* __restrict__ cim, int m, int cbase, int lgB, int F, int ncb,
+                                              int fmt) {
     const int B = 1 << lgB, N = 2 * B, half = B >> 1;
-    int bx, m;
-    xcd_block_m(bx, m);
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
-    const int cbase = bx * ncb;
-    if (cbase >= ncols) return;
     const int rs = B + lds_pad<R>();
     for (int idx = threadIdx.x; idx < ncb * B; idx += kThreads) {
         const int c = idx >> lgB, k2 = idx & (B - 1);
@@ -111,8 +146,6 @@ __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nct = ncb >> 4;
-    const size_t B2 = (size_t)B * B;
-    const double sg = (m & 1) ? -1.0 : 1.0;
     for (int p = 0; p < 2; ++p) {
         const GemmMeta g = meta[m * 2 + p];
         if (g.Lp == 0) continue;
@@ -132,46 +165,107 @@ __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
             }
             const int i = i0 + (lane & 15);
             if (i >= g.Lp) continue;
-            const int l = m + p + 2 * i;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = cbase + c0 + Mma<R>::row(lane, r);
-                if (col >= ncols) continue;
-                const int f = col / cpf, rem = col - f * cpf;
-                const int comp = rem & 1;
-                double* out = comp ? cim : cre;
-                const double v = (double)acc[r];
-                if ((rem >> 1) == 0) {
-                    out[f * B2 + index_of_coeff(m, l, B)] = v;
-                    if (fmt == 1 && m > 0) out[f * B2 + index_of_coeff(-m, l, B)] = comp ? -sg * v : sg * v;
-                } else {
-                    out[f * B2 + index_of_coeff(-m, l, B)] = sg * v;
-                }
+                if (col < ncols) fwd_store((double)acc[r], col, m, m + p + 2 * i, B, cpf, fmt, cre, cim);
             }
         }
     }
 }
 
-// Inverse: D'[c][t] = sum_i coef[c][i] * AI(m,p)[i][t].  grid (ceil(cols/ncb), bw).
-// LDS: Cd[c][q], q = i (p = 0) or roundup4(Lp0) + i (p = 1); q < bw always.
+// Naive order m >= cutoff (naive.c:34-59), from the ring-FFT rows X.  The
+// grid is symmetric, x_{N-1-j} = -x_j, so P_l^m(x_{N-1-j}) = (-1)^{l-m}
+// P_l^m(x_j): degree l = m+p+2i is the sum over the bw rings j < bw of
+// (w_j X_j + (-1)^p w_{N-1-j} X_{N-1-j}) P_l^m(x_j).  One parity at a time,
+// LDS: Bd[c][j].
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
-                                                       const R* __restrict__ AI, const GemmMeta* __restrict__ meta,
-                                                       R* __restrict__ Cb, int lgB, int F, int ncb, int fmt) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    R* Cd = reinterpret_cast<R*>(smem);
-    const int B = 1 << lgB, N = 2 * B, half = B >> 1;
-    int bx, m;
-    xcd_block_m(bx, m);
+__device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typename Cplx<R>::type* __restrict__ X,
+                                               const double* __restrict__ wn, const R* __restrict__ AF,
+                                               const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                               double* __restrict__ cim, int m, int cbase, int lgB, int F, int ncb,
+                                               int fmt) {
+    using T = typename Cplx<R>::type;
+    const int B = 1 << lgB, N = 2 * B;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
+    const int rs = B + lds_pad<R>();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nct = ncb >> 4;
+    for (int p = 0; p < 2; ++p) {
+        const GemmMeta g = meta[m * 2 + p];
+        if (g.Lp == 0) break;
+        if (p) __syncthreads();  // parity 0's reads of Bd are done
+        for (int idx = threadIdx.x; idx < ncb * B; idx += kThreads) {
+            const int c = idx >> lgB, j = idx & (B - 1);
+            const int col = cbase + c;
+            R v = 0;
+            if (col < ncols) {
+                const int f = col / cpf, rem = col - f * cpf;
+                const T* xr = X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N;
+                const T a = xr[j], b = xr[N - 1 - j];
+                const R wa = (R)wn[j] * ((rem & 1) ? a.y : a.x);
+                const R wb = (R)wn[N - 1 - j] * ((rem & 1) ? b.y : b.x);
+                v = p ? wa - wb : wa + wb;
+            }
+            Bd[c * rs + j] = v;
+        }
+        __syncthreads();
+        const int Li16 = (g.Lp + 15) & ~15;
+        const int ntiles = (Li16 >> 4) * nct;
+        for (int item = wave; item < ntiles; item += kThreads / 64) {
+            const int it = item / nct, ct = item - it * nct;
+            const int i0 = it * 16, c0 = ct * 16;
+            typename Mma<R>::acc acc = {0, 0, 0, 0};
+            const R* bcol = Bd + (c0 + (lane & 15)) * rs;
+            const R* arow = AF + g.offF + i0 + (lane & 15);
+            for (int t0 = 0; t0 < B; t0 += 4) {
+                const int tk = t0 + (lane >> 4);
+                acc = Mma<R>::mma(bcol[tk], arow[(size_t)tk * Li16], acc);
+            }
+            const int i = i0 + (lane & 15);
+            if (i >= g.Lp) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = cbase + c0 + Mma<R>::row(lane, r);
+                if (col < ncols) fwd_store((double)acc[r], col, m, m + p + 2 * i, B, cpf, fmt, cre, cim);
+            }
+        }
+    }
+}
+
+// Forward: D'[c][i] = sum_k data[c][k] * A(m,p)[i][k].  grid (ceil(cols/ncb), bw).
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
+                                                       const typename Cplx<R>::type* __restrict__ X,
+                                                       const double* __restrict__ wn, const R* __restrict__ AF,
+                                                       const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                                       double* __restrict__ cim, int lgB, int cut, int F, int ncb,
+                                                       int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    R* Bd = reinterpret_cast<R*>(smem);
+    int bx, m;
+    xcd_block_m(bx, m);
     const int cbase = bx * ncb;
-    if (cbase >= ncols) return;
-    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
+    if (m < cut)
+        gemm_fwd_semi<R>(Bd, Cb, AF, meta, cre, cim, m, cbase, lgB, F, ncb, fmt);
+    else
+        gemm_fwd_naive<R>(Bd, X, wn, AF, meta, cre, cim, m, cbase, lgB, F, ncb, fmt);
+}
+
+// Coefficient columns of order m into LDS: Cd[c][q], q = i (p = 0) or
+// roundup4(Lp0) + i (p = 1), with the padding rows of both parities zeroed;
+// q < bw always.  `negodd`: negate the -m columns of odd orders.
+template <typename R>
+__device__ __forceinline__ void inv_fill(R* __restrict__ Cd, const double* __restrict__ cre,
+                                         const double* __restrict__ cim, const GemmMeta& g0, const GemmMeta& g1,
+                                         int m, int cbase, int B, int ncb, int cpf, int ncols, bool negodd) {
     const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
     const int L = B - m;
     const size_t B2 = (size_t)B * B;
     const int rs = B + lds_pad<R>();
+    const bool neg = negodd && (m & 1);
     for (int idx = threadIdx.x; idx < ncb * L; idx += kThreads) {
         const int c = idx / L, li = idx - c * L;
         const int col = cbase + c;
@@ -181,16 +275,32 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
             const int f = col / cpf, rem = col - f * cpf;
             const int mm = (rem >> 1) ? -m : m;
             const double* src = (rem & 1) ? cim : cre;
-            v = (R)src[f * B2 + index_of_coeff(mm, m + li, B)];
+            const double x = src[f * B2 + index_of_coeff(mm, m + li, B)];
+            v = (R)((neg && (rem >> 1)) ? -x : x);
         }
         Cd[c * rs + q] = v;
     }
-    // zero the padding rows of both parities (at most 3 + 3 per column)
     for (int idx = threadIdx.x; idx < ncb * 8; idx += kThreads) {
         const int c = idx >> 3, k = idx & 7;
         const int p = k >> 2, i = (p ? g1.Lp : g0.Lp) + (k & 3);
         if (i < (p ? Li4_1 : Li4_0)) Cd[c * rs + (p ? Li4_0 : 0) + i] = 0;
     }
+}
+
+// Seminaive order m < cutoff: D'[c][t] = sum_i coef[c][i] * AI(m,p)[i][t],
+// cosine coefficients into the Cb row of (field, +-m).
+template <typename R>
+__device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* __restrict__ cre,
+                                              const double* __restrict__ cim, const R* __restrict__ AI,
+                                              const GemmMeta* __restrict__ meta, R* __restrict__ Cb, int m, int cbase,
+                                              int lgB, int F, int ncb, int fmt) {
+    const int B = 1 << lgB, N = 2 * B, half = B >> 1;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
+    const int rs = B + lds_pad<R>();
+    inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, B, ncb, cpf, ncols, false);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nct = ncb >> 4;
@@ -225,24 +335,105 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
     }
 }
 
+// Naive order m >= cutoff (naive.c:78-95), straight into the ring-FFT rows
+// X: per ring j < bw the parity sums E (p = 0) and O (p = 1) over the table
+// rows (which carry 1/sqrt(2pi)) give ring j <- E+O and ring N-1-j <- E-O.
+// Row N-m of an odd order carries a minus sign (FST_semi_memo.c:301-348),
+// put on its coefficients; REAL data also fills the conjugate row N-m.
+template <typename R>
+__device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double* __restrict__ cre,
+                                               const double* __restrict__ cim, const R* __restrict__ AI,
+                                               const GemmMeta* __restrict__ meta, typename Cplx<R>::type* __restrict__ X,
+                                               int m, int cbase, int lgB, int F, int ncb, int fmt) {
+    const int B = 1 << lgB, N = 2 * B;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf;
+    const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
+    const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
+    const int rs = B + lds_pad<R>();
+    inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, B, ncb, cpf, ncols, true);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nct = ncb >> 4;
+    const int ntt = B >> 4;
+    const bool mirror = fmt == 1 && m > 0;
+    for (int item = wave; item < ntt * nct; item += kThreads / 64) {
+        const int tt = item / nct, ct = item - tt * nct;
+        const int t0 = tt * 16, c0 = ct * 16;
+        typename Mma<R>::acc e = {0, 0, 0, 0}, o = {0, 0, 0, 0};
+        const R* acol = Cd + (c0 + (lane & 15)) * rs;
+        const R* b0 = AI + g0.offI + t0 + (lane & 15);
+        for (int i0 = 0; i0 < Li4_0; i0 += 4) {
+            const int ik = i0 + (lane >> 4);
+            e = Mma<R>::mma(acol[ik], b0[(size_t)ik * B], e);
+        }
+        const R* b1 = AI + g1.offI + t0 + (lane & 15);
+        for (int i0 = 0; i0 < Li4_1; i0 += 4) {
+            const int ik = i0 + (lane >> 4);
+            o = Mma<R>::mma(acol[Li4_0 + ik], b1[(size_t)ik * B], o);
+        }
+        const int j = t0 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = cbase + c0 + Mma<R>::row(lane, r);
+            if (col >= ncols) continue;
+            const int f = col / cpf, rem = col - f * cpf, comp = rem & 1;
+            R* xr = reinterpret_cast<R*>(X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N) + comp;
+            const R lo = e[r] + o[r], hi = e[r] - o[r];
+            xr[2 * j] = lo;
+            xr[2 * (N - 1 - j)] = hi;
+            if (mirror) {  // conj into row N-m
+                R* xm = reinterpret_cast<R*>(X + ((size_t)f * N + (N - m)) * N) + comp;
+                xm[2 * j] = comp ? -lo : lo;
+                xm[2 * (N - 1 - j)] = comp ? -hi : hi;
+            }
+        }
+    }
+}
+
+// Inverse.  grid (ceil(cols/ncb), bw).
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
+                                                       const R* __restrict__ AI, const GemmMeta* __restrict__ meta,
+                                                       R* __restrict__ Cb, typename Cplx<R>::type* __restrict__ X,
+                                                       int lgB, int cut, int F, int ncb, int fmt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    R* Cd = reinterpret_cast<R*>(smem);
+    int bx, m;
+    xcd_block_m(bx, m);
+    const int cbase = bx * ncb;
+    if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
+    if (m < cut)
+        gemm_inv_semi<R>(Cd, cre, cim, AI, meta, Cb, m, cbase, lgB, F, ncb, fmt);
+    else
+        gemm_inv_naive<R>(Cd, cre, cim, AI, meta, X, m, cbase, lgB, F, ncb, fmt);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
 void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t s) {
-    const int B = t.bw, half = B / 2;
+    const int B = t.bw, half = B / 2, cut = t.cutoff;
     g.bw = B;
     g.prec = prec;
+    g.cut = cut;
     g.h_meta.assign(2 * B, GemmMeta{});
     long of = 0, oi = 0;
     for (int m = 0; m < B; ++m)
         for (int p = 0; p < 2; ++p) {
             GemmMeta& e = g.h_meta[m * 2 + p];
             e.Lp = gm_lp(m, p, B);
-            const int llast = m + p + 2 * (e.Lp - 1);
-            e.Kf = e.Lp > 0 ? row_size(m, llast) : 0;
             e.offF = of;
             e.offI = oi;
+            if (m >= cut) {  // naive: bw folded rings in both directions
+                e.Kf = B;
+                of += (long)B * ((e.Lp + 15) & ~15);
+                oi += (long)((e.Lp + 3) & ~3) * B;
+                continue;
+            }
+            const int llast = m + p + 2 * (e.Lp - 1);
+            e.Kf = e.Lp > 0 ? row_size(m, llast) : 0;
             of += (long)((e.Kf + 3) & ~3) * ((e.Lp + 15) & ~15);
             oi += (long)((e.Lp + 3) & ~3) * ((half + 15) & ~15);
         }
@@ -255,9 +446,11 @@ void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t
     S2_HIP(hipMemcpyAsync(g.meta, g.h_meta.data(), sizeof(GemmMeta) * 2 * B, hipMemcpyHostToDevice, s));
     const size_t lds = sizeof(int) * (B + 1);
     if (prec == 0)
-        k_pack_gemm<double><<<dim3(B, 2), 256, lds, s>>>(t.fwd, t.inv, t.off, g.meta, (double*)g.af, (double*)g.ai, B);
+        k_pack_gemm<double><<<dim3(B, 2), 256, lds, s>>>(t.fwd, t.inv, t.off, g.meta, (double*)g.af, (double*)g.ai, B,
+                                                          cut);
     else
-        k_pack_gemm<float><<<dim3(B, 2), 256, lds, s>>>(t.fwd, t.inv, t.off, g.meta, (float*)g.af, (float*)g.ai, B);
+        k_pack_gemm<float><<<dim3(B, 2), 256, lds, s>>>(t.fwd, t.inv, t.off, g.meta, (float*)g.af, (float*)g.ai, B,
+                                                         cut);
     S2_HIP(hipGetLastError());
     S2_HIP(hipStreamSynchronize(s));
 }
@@ -294,8 +487,8 @@ static size_t gemm_lds(int B, int prec) {
 
 bool fast_supported(int bw, int cutoff, int prec) {
     const int N = 2 * bw;
-    // N*N must hold at least one block tile of rows (N >= 64)
-    if (bw < 32 || (N & (N - 1)) || N > 2048 || cutoff != bw) return false;
+    // N*N must hold at least one block tile of rows (N >= 64); any cutoff in [0, bw]
+    if (bw < 32 || (N & (N - 1)) || N > 2048 || cutoff < 0 || cutoff > bw) return false;
     return gemm_ncb(bw, prec) >= 16;  // GEMM LDS tile of at least 16 columns
 }
 
@@ -323,16 +516,17 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
             launch_lds(k_fphi_fwd<R, V>, dim3(N / rpb, F), kThreads, lds, s, re, im, X, lgN, tw);
         }
     }
-    const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
-    {
+    // weighted DCT of the seminaive order rows only
+    const int cut = g.cut, nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+    if (cut > 0) {
         Span sp(tm, ST_DCT_FWD, s);
         if ((long)N * N >= 512L * V) {
             const int rp2 = V * 512 / N;
             launch_lds(k_fdct_fwd<R, V, 512>, dim3((nrows + rp2 - 1) / rp2, F), 512,
-                       sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const T*)X, Cb, ws, lgN, B, nrows, tw, ph);
+                       sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const T*)X, Cb, ws, lgN, cut, nrows, tw, ph);
         } else {
             launch_lds(k_fdct_fwd<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const T*)X, Cb, ws, lgN,
-                       B, nrows, tw, ph);
+                       cut, nrows, tw, ph);
         }
     }
     {
@@ -340,7 +534,8 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
         const int cols = 4 * F;
         Span sp(tm, ST_LEG_FWD, s);
         launch_lds(k_gemm_fwd<R>, dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s, (const R*)Cb,
-                   (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, lg2(B), F, ncb, (int)fmt);
+                   (const T*)X, (const double*)c.w_naive, (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, lg2(B),
+                   cut, F, ncb, (int)fmt);
     }
 }
 
@@ -362,19 +557,20 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
         const int cols = 4 * F;
         Span sp(tm, ST_LEG_INV, s);
         launch_lds(k_gemm_inv<R>, dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s, cre, cim,
-                   (const R*)g.ai, (const GemmMeta*)g.meta, Cb, lg2(B), F, ncb, (int)fmt);
+                   (const R*)g.ai, (const GemmMeta*)g.meta, Cb, X, lg2(B), g.cut, F, ncb, (int)fmt);
     }
-    const int nrows = (fmt == Fmt::Complex) ? 2 * B - 1 : B;
-    {
+    // DCT-III of the seminaive order rows (the naive rows are already in X)
+    const int cut = g.cut, nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+    if (cut > 0) {
         Span sp(tm, ST_DCT_INV, s);
         if ((long)N * N >= 512L * V) {
             const int rp2 = V * 512 / N;
             launch_lds(k_fdct_inv<R, V, 512>, dim3((nrows + rp2 - 1) / rp2, F), 512,
-                       sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const R*)Cb, X, sv, lgN, B, nrows, (int)fmt, tw,
+                       sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const R*)Cb, X, sv, lgN, cut, nrows, (int)fmt, tw,
                        ph);
         } else {
             launch_lds(k_fdct_inv<R, V>, dim3((nrows + rpb - 1) / rpb, F), kThreads, lds, s, (const R*)Cb, X, sv, lgN,
-                       B, nrows, (int)fmt, tw, ph);
+                       cut, nrows, (int)fmt, tw, ph);
         }
     }
     {

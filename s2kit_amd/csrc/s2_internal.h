@@ -89,6 +89,7 @@ struct GemmMeta {
 
 struct GemmTables {
     int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32 (seminaive); 2: FP32 naive (mixed form); 3: split BF16 naive
+    int cut = 0;           // prec 0, 1: orders m >= cut are naive (sampled P_l^m tables)
     void* af = nullptr;
     void* ai = nullptr;
     GemmMeta* meta = nullptr;

@@ -32,8 +32,13 @@ def to_dev(torch, dev, arrs):
     return torch.tensor(np.stack(arrs), dtype=torch.float64, device=dev)
 
 
+# cutoff < bw: the naive branch for orders >= cutoff (bw/4 is test_s2_semi_memo.c:74-75's
+# choice, bw/2 - 10 test_s2_semi_fly.c:128's; 0 makes every order naive)
 @pytest.mark.parametrize("bw,cutoff,fmt,prec", [(8, 8, 0, 0), (17, 17, 0, 0), (32, 32, 0, 0), (64, 64, 0, 0),
                                                 (64, 16, 0, 0), (64, 64, 1, 0), (128, 128, 0, 0), (512, 512, 0, 0),
+                                                (32, 0, 0, 0), (32, 31, 1, 0), (128, 54, 1, 0), (256, 64, 0, 0),
+                                                (512, 128, 0, 0), (17, 4, 0, 0), (64, 16, 0, 1), (128, 0, 1, 1),
+                                                (256, 64, 0, 1),
                                                 (32, 32, 0, 1), (64, 64, 0, 1), (128, 128, 1, 1), (256, 256, 0, 1),
                                                 (64, 64, 0, 2), (128, 128, 1, 2), (256, 256, 0, 2),
                                                 (256, 256, 1, 2), (64, 64, 0, 3), (128, 128, 1, 3),
@@ -182,3 +187,41 @@ def test_large_bw_round_trip(env, bw, prec, tol):
     print(f"bw={bw} prec={prec}: round-trip rel err {err:.3e}", flush=True)
     assert err < tol
     plan.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-11), (1, 1e-4)])
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_cutoff_plans_agree_bw256(env, prec, tol, fmt):
+    # Seminaive and naive orders compute the same quadrature sums, so plans at
+    # cutoff bw/4 and 0 must reproduce the cutoff = bw plan (both directions)
+    # at the full B=256 shape, 16 fields.
+    S, torch, dev = env
+    from s2kit_amd import synth
+    bw, F = 256, 16
+    n = 2 * bw
+    cre, cim = synth.power_law_coeffs(bw, F, 91, dev)
+    if fmt == 1:  # Hermitian coefficients of a real field
+        g0 = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+        g1 = torch.empty_like(g0)
+        full = S.Plan(bw, F)
+        full.inverse(cre, cim, g0, g1)
+        full.forward(g0, torch.zeros_like(g1), cre, cim)
+        full.close()
+    out = {}
+    for cut in (bw, bw // 4, 0):
+        plan = S.Plan(bw, F, precision=prec, cutoff=cut)
+        gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+        gim = torch.empty_like(gre)
+        plan.inverse(cre, cim, gre, gim, data_format=fmt)
+        c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+        plan.forward(gre, gim, c2r, c2i, data_format=fmt)
+        torch.cuda.synchronize()
+        out[cut] = (torch.complex(gre, gim), torch.complex(c2r, c2i))
+        plan.close()
+    g_ref, c_ref = out[bw]
+    for cut in (bw // 4, 0):
+        g, c = out[cut]
+        eg = float((g - g_ref).abs().max() / g_ref.abs().max())
+        ec = float((c - c_ref).abs().max() / c_ref.abs().max())
+        print(f"cutoff={cut} prec={prec} fmt={fmt}: grid {eg:.3e}, coefficients {ec:.3e}", flush=True)
+        assert eg < tol and ec < tol
