@@ -13,8 +13,8 @@
 // Tables (cospml.c:161-362, pml.c:41-79, weights.c:32-46) are generated on
 // the GPU in FP64 by k_gen_cospml / k_transpose / k_gen_pml / k_weights.
 //
-// FFTs are Stockham radix-2 passes in LDS for power-of-two lengths and a
-// direct DFT otherwise.  Both DCTs use one complex FFT of length N = 2B on a
+// FFTs are Stockham passes in LDS: radix 2 for power-of-two lengths, mixed
+// radix (one pass per prime factor, radix 4 for pairs of 2s) otherwise.  Both DCTs use one complex FFT of length N = 2B on a
 // reordered sequence (Makhoul); the DCTs are real-linear with real kernels,
 // so one complex row carries the real and imaginary transforms together.
 #include <cmath>
@@ -31,12 +31,120 @@ namespace s2 {
 // block-wide FFTs in LDS
 // ---------------------------------------------------------------------------
 
+// Division by a divisor 1 <= d < 2^16 of a value x < 2^16: floor(x/d) =
+// (x * ceil(2^32/d)) >> 32 (the rounding error x*(ceil(2^32/d)*d - 2^32)/2^32
+// stays below 1/d).
+__device__ __forceinline__ unsigned long long magic16(unsigned d) { return (0x100000000ull + d - 1) / d; }
+__device__ __forceinline__ int div16(int x, unsigned long long mg) { return (int)(((unsigned long long)x * mg) >> 32); }
+
+// One Stockham pass of radix R over nfft length-N transforms, a -> b, Ns the
+// product of the radices already applied.  Butterfly j = jq*Ns + k (k < Ns,
+// j < N/R) reads a[j + q*N/R] (q < R), applies the twiddles
+// exp(SIGN 2 pi i q k / (Ns*R)), runs an R-point DFT and writes output p to
+// b[jq*Ns*R + p*Ns + k].  tw[t] = exp(-2 pi i t / N).
+template <typename T, int SIGN, int R>
+__device__ __forceinline__ void mr_pass(const T* a, T* b, int N, int Ns, int nfft, const T* __restrict__ tw) {
+    using F = typename Real<T>::type;
+    const int NR = N / R, step = N / (Ns * R);
+    const unsigned long long mNR = magic16(NR), mNs = magic16(Ns);
+    const int total = nfft * NR;
+    for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
+        const int f = div16(bf, mNR), j = bf - f * NR;
+        const int jq = div16(j, mNs), k = j - jq * Ns;
+        const T* in = a + f * N + j;
+        T v[R];
+        v[0] = in[0];
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+            T w = tw[q * k * step];  // q*k*step < N
+            if (SIGN > 0) w.y = -w.y;
+            v[q] = cmul(in[q * NR], w);
+        }
+        T y[R];
+        if constexpr (R == 2) {
+            y[0] = cadd(v[0], v[1]);
+            y[1] = csub(v[0], v[1]);
+        } else if constexpr (R == 4) {
+            const T s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+            const T s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+            // SIGN * i * d13
+            const T r13 = SIGN < 0 ? mk<T>(d13.y, -d13.x) : mk<T>(-d13.y, d13.x);
+            y[0] = cadd(s02, s13);
+            y[2] = csub(s02, s13);
+            y[1] = cadd(d02, r13);
+            y[3] = csub(d02, r13);
+        } else {  // odd R: conjugate pairs (q, R-q), w[t] = exp(SIGN 2 pi i t / R)
+            constexpr int H = (R - 1) / 2;
+            T w[R];
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                w[t] = tw[t * NR];
+                if (SIGN > 0) w[t].y = -w[t].y;
+            }
+            T sp[H], dm[H];
+            y[0] = v[0];
+#pragma unroll
+            for (int q = 1; q <= H; ++q) {
+                sp[q - 1] = cadd(v[q], v[R - q]);
+                dm[q - 1] = csub(v[q], v[R - q]);
+                y[0] = cadd(y[0], sp[q - 1]);
+            }
+            // v_q w^t + v_{R-q} w^-t = sp Re(w^t) + i dm Im(w^t), t = pq mod R
+#pragma unroll
+            for (int p = 1; p < R; ++p) {
+                F sr = v[0].x, si = v[0].y;
+#pragma unroll
+                for (int q = 1; q <= H; ++q) {
+                    const T c = w[(p * q) % R];
+                    sr += sp[q - 1].x * c.x - dm[q - 1].y * c.y;
+                    si += sp[q - 1].y * c.x + dm[q - 1].x * c.y;
+                }
+                y[p] = mk<T>(sr, si);
+            }
+        }
+        T* out = b + f * N + jq * Ns * R + k;
+#pragma unroll
+        for (int p = 0; p < R; ++p) out[p * Ns] = y[p];
+    }
+}
+
+// The same pass for a radix known only at run time (prime factors above 7):
+// one output per thread, sum_q a[j + q*N/R] * exp(SIGN 2 pi i q (k + p*Ns) / (Ns*R)),
+// the butterfly with its twiddle merged in.
+template <typename T, int SIGN>
+__device__ __forceinline__ void mr_pass_any(const T* a, T* b, int N, int Ns, int R, int nfft,
+                                            const T* __restrict__ tw) {
+    using F = typename Real<T>::type;
+    const int NR = N / R, NsR = Ns * R, step = N / NsR;
+    const unsigned long long mN = magic16(N), mNs = magic16(Ns), mR = magic16(R);
+    const int total = nfft * N;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int f = div16(idx, mN), o = idx - f * N;
+        const int t1 = div16(o, mNs), k = o - t1 * Ns;
+        const int jq = div16(t1, mR), p = t1 - jq * R;
+        const T* in = a + f * N + jq * Ns + k;
+        const int e0 = k + p * Ns;  // < Ns*R
+        F sr = 0, si = 0;
+        int e = 0;
+        for (int q = 0; q < R; ++q) {
+            T w = tw[e * step];
+            if (SIGN > 0) w.y = -w.y;
+            const T v = in[q * NR];
+            sr += v.x * w.x - v.y * w.y;
+            si += v.x * w.y + v.y * w.x;
+            e += e0;
+            if (e >= NsR) e -= NsR;
+        }
+        b[f * N + o] = mk<T>(sr, si);
+    }
+}
+
 // nfft independent length-N transforms stored back to back in a (b is
 // scratch of the same size).  SIGN = -1: forward, +1: inverse (unnormalised).
 // tw[t] = exp(-2 pi i t / N).  Returns the buffer holding the result.
-template <typename T, int SIGN>
+template <typename T, int SIGN, bool MR>
 __device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T* __restrict__ tw) {
-    if (pow2) {
+    if (!MR) {  // power-of-two N
         const int half = N >> 1;
         const int total = nfft * half;
         for (int s = 0, Ns = 1; s < logN; ++s, Ns <<= 1) {
@@ -62,26 +170,28 @@ __device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T
         }
         return a;
     }
-    // direct DFT for lengths that are not powers of two
-    const int total = nfft * N;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int f = idx / N, k = idx - f * N;
-        const T* in = a + f * N;
-        typename Real<T>::type sr = 0, si = 0;
-        int e = 0;
-        for (int j = 0; j < N; ++j) {
-            T w = tw[e];
-            if (SIGN > 0) w.y = -w.y;
-            const T v = in[j];
-            sr += v.x * w.x - v.y * w.y;
-            si += v.x * w.y + v.y * w.x;
-            e += k;
-            if (e >= N) e -= N;
+    // Other lengths: mixed-radix Stockham (autosort), one pass per factor R
+    // of N (4 while 4 divides, then 2, then odd factors in ascending order).
+    // Radices 2, 3 and 4 run as butterflies; the rest one output per thread
+    // (more butterfly instances in one kernel made hipcc spill).
+    int rem = N, Ns = 1;
+    while (rem > 1) {
+        int R = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : 3;
+        while (rem % R) R += 2;
+        switch (R) {
+            case 2: mr_pass<T, SIGN, 2>(a, b, N, Ns, nfft, tw); break;
+            case 3: mr_pass<T, SIGN, 3>(a, b, N, Ns, nfft, tw); break;
+            case 4: mr_pass<T, SIGN, 4>(a, b, N, Ns, nfft, tw); break;
+            default: mr_pass_any<T, SIGN>(a, b, N, Ns, R, nfft, tw); break;
         }
-        b[f * N + k] = mk<T>(sr, si);
+        __syncthreads();
+        T* t = a;
+        a = b;
+        b = t;
+        rem /= R;
+        Ns *= R;
     }
-    __syncthreads();
-    return b;
+    return a;
 }
 
 // Row of the order-major buffers that holds semi/naive row number q:
@@ -94,7 +204,7 @@ __device__ __forceinline__ int row_of(int q, int N, int cut) { return q < cut ? 
 // ---------------------------------------------------------------------------
 
 // grid (ceil(N/rpb), F): rpb rings of one field per block.
-template <typename T>
+template <typename T, bool MR>
 __global__ __launch_bounds__(256) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
                                                  T* __restrict__ X, int N, int logN, int pow2, int rpb,
                                                  const T* __restrict__ tw) {
@@ -107,7 +217,7 @@ __global__ __launch_bounds__(256) void k_phi_fwd(const double* __restrict__ re, 
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x)
         a[i] = mk<T>((typename Real<T>::type)re[base + i], (typename Real<T>::type)im[base + i]);
     __syncthreads();
-    T* r = block_fft<T, -1>(a, b, N, logN, pow2, nr, tw);
+    T* r = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, tw);
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         const int row = i / nr, jl = i - row * nr;
         X[((size_t)f * N + row) * N + j0 + jl] = r[jl * N + row];
@@ -117,7 +227,7 @@ __global__ __launch_bounds__(256) void k_phi_fwd(const double* __restrict__ re, 
 // Weighted DCT-II of every seminaive order row, first bw outputs kept.
 // grid (ceil(nrows/rpb), F).  w_semi holds the even then the odd weights,
 // premultiplied by sqrt(2 pi)/N (FFT scaling) and 1/sqrt(2N) (DCT scaling).
-template <typename T>
+template <typename T, bool MR>
 __global__ __launch_bounds__(256) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
                                                  const double* __restrict__ w_semi, const T* __restrict__ tw,
                                                  const T* __restrict__ ph, int N, int logN, int pow2, int B,
@@ -137,7 +247,7 @@ __global__ __launch_bounds__(256) void k_dct_fwd(const T* __restrict__ X, T* __r
         a[rl * N + p] = x;
     }
     __syncthreads();
-    T* V = block_fft<T, -1>(a, b, N, logN, pow2, nr, tw);
+    T* V = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, tw);
     for (int i = threadIdx.x; i < nr * B; i += blockDim.x) {
         const int rl = i / B, k = i - rl * B;
         const int r = row_of(q0 + rl, N, cut);
@@ -627,7 +737,7 @@ __global__ __launch_bounds__(256) void k_leg_inv(const double* __restrict__ cre,
 // DCT-III of every seminaive order row (inputs 0..B-1, zero above), then
 // x sin(theta) for odd orders, x (-1) on odd negative rows, x 1/sqrt(2pi).
 // REAL mirrors row N-m = conj(row m).  grid (ceil(nrows/rpb), F).
-template <typename T>
+template <typename T, bool MR>
 __global__ __launch_bounds__(256) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
                                                  const T* __restrict__ tw, const T* __restrict__ ph,
                                                  const double* __restrict__ sinv, int N, int logN, int pow2, int B,
@@ -649,7 +759,7 @@ __global__ __launch_bounds__(256) void k_dct_inv(const T* __restrict__ Cb, T* __
         a[rl * N + k] = cmul(cconj(ph[k]), t);
     }
     __syncthreads();
-    T* u = block_fft<T, +1>(a, b, N, logN, pow2, nr, tw);
+    T* u = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, tw);
     const R inv_s2pi = (R)0.39894228040143267794;  // 1/sqrt(2 pi)
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         const int rl = i / N, j = i - rl * N;
@@ -707,7 +817,7 @@ __global__ __launch_bounds__(256) void k_naive_inv(const double* __restrict__ cr
 
 // Inverse ring FFT (+i, unnormalised) per ring; row B (Nyquist) is zero.
 // grid (ceil(N/rpb), F).
-template <typename T>
+template <typename T, bool MR>
 __global__ __launch_bounds__(256) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
                                                  double* __restrict__ im, int N, int logN, int pow2, int B, int rpb,
                                                  const T* __restrict__ tw) {
@@ -721,7 +831,7 @@ __global__ __launch_bounds__(256) void k_phi_inv(const T* __restrict__ X, double
         a[jl * N + r] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + jl];
     }
     __syncthreads();
-    T* y = block_fft<T, +1>(a, b, N, logN, pow2, nr, tw);
+    T* y = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, tw);
     const size_t base = ((size_t)f * N + j0) * N;
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         re[base + i] = (double)y[i].x;
@@ -1001,6 +1111,17 @@ static int ilog2_exact(int n) {
     return ((1 << l) == n) ? l : -1;
 }
 
+// Launch the power-of-two (kp) or the mixed-radix (km) instance of a
+// generic FFT kernel: separate instances keep the mixed-radix passes'
+// registers out of the power-of-two kernels.
+template <typename KP, typename KM, typename... A>
+static void fft_launch(bool pow2, KP kp, KM km, dim3 grid, size_t lds, hipStream_t s, A... args) {
+    if (pow2)
+        kp<<<grid, 256, lds, s>>>(args...);
+    else
+        km<<<grid, 256, lds, s>>>(args...);
+}
+
 static int rings_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
 
 void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s) {
@@ -1205,15 +1326,16 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     {
     Span sp(tm, ST_PHI_FWD, s);
-    k_phi_fwd<double2><<<dim3((N + rpb - 1) / rpb, F), 256, lds_fft, s>>>(re, im, X, N, c.logN, c.pow2, rpb, c.tw);
+    fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, s,
+               re, im, X, N, c.logN, c.pow2, rpb, c.tw);
     S2_HIP(hipGetLastError());
     }
     if (cut > 0) {
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         {
         Span sp(tm, ST_DCT_FWD, s);
-        k_dct_fwd<double2><<<dim3((nrows + rpb - 1) / rpb, F), 256, lds_fft, s>>>(
-            X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+        fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
+                   lds_fft, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
         }
         Span sp(tm, ST_LEG_FWD, s);
@@ -1268,8 +1390,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         }
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_INV, s);
-        k_dct_inv<double2><<<dim3((nrows + rpb - 1) / rpb, F), 256, lds_fft, s>>>(
-            Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+        fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
+                   lds_fft, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
@@ -1280,8 +1402,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
-    k_phi_inv<double2><<<dim3((N + rpb - 1) / rpb, F), 256, lds_fft, s>>>(X, re, im, N, c.logN, c.pow2, B, rpb,
-                                                                          c.tw);
+    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, s,
+               X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
     S2_HIP(hipGetLastError());
 }
 
@@ -1295,8 +1417,8 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
     S2_HIP(hipGetLastError());
     const int rpb = rings_per_block(N);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
-    k_dct_fwd<double2><<<dim3(1, F), 256, lds_fft, s>>>(X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B,
-                                                        t.cutoff, 1, 1);
+    fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3(1, F), lds_fft, s, X, Cb, c.w_semi,
+               c.tw, c.ph, N, c.logN, c.pow2, B, t.cutoff, 1, 1);
     S2_HIP(hipGetLastError());
     const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
     k_leg_fwd<<<dim3(1, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(

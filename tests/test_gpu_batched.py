@@ -38,7 +38,8 @@ def to_dev(torch, dev, arrs):
                                                 (64, 16, 0, 0), (64, 64, 1, 0), (128, 128, 0, 0), (512, 512, 0, 0),
                                                 (32, 0, 0, 0), (32, 31, 1, 0), (128, 54, 1, 0), (256, 64, 0, 0),
                                                 (512, 128, 0, 0), (17, 4, 0, 0), (64, 16, 0, 1), (128, 0, 1, 1),
-                                                (256, 64, 0, 1),
+                                                (256, 64, 0, 1), (100, 100, 0, 0), (255, 255, 1, 0),
+                                                (90, 30, 0, 0), (45, 45, 1, 0), (28, 28, 0, 0), (21, 7, 1, 0),
                                                 (32, 32, 0, 1), (64, 64, 0, 1), (128, 128, 1, 1), (256, 256, 0, 1),
                                                 (64, 64, 0, 2), (128, 128, 1, 2), (256, 256, 0, 2),
                                                 (256, 256, 1, 2), (64, 64, 0, 3), (128, 128, 1, 3),
