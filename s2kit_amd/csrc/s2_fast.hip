@@ -35,6 +35,14 @@ namespace s2 {
 // ---------------------------------------------------------------------------
 // GEMM tables
 // ---------------------------------------------------------------------------
+// Extents shared by the packing and the GEMMs (bw = B): cosine indices of
+// parity p number hE = ceil(B/2) (p = 0) and hO = floor(B/2) (p = 1);
+// inverse seminaive rows are roundup16(hE) long; naive forward tables have
+// roundup4(B) ring rows and naive inverse rows roundup16(B) rings, zero past B.
+__host__ __device__ inline int pk_tp16(int B) { return ((B + 1) / 2 + 15) & ~15; }
+__host__ __device__ inline int pk_b4(int B) { return (B + 3) & ~3; }
+__host__ __device__ inline int pk_b16(int B) { return (B + 15) & ~15; }
+
 template <typename R>
 __global__ void k_pack_gemm(const double* __restrict__ T, const double* __restrict__ TT, const long* __restrict__ toff,
                             const GemmMeta* __restrict__ meta, R* __restrict__ AF, R* __restrict__ AI, int B,
@@ -42,20 +50,21 @@ __global__ void k_pack_gemm(const double* __restrict__ T, const double* __restri
     extern __shared__ __align__(16) unsigned char smem[];
     int* rowpos = reinterpret_cast<int*>(smem);
     const int m = blockIdx.x, p = blockIdx.y;
-    const int N = 2 * B, half = B / 2;
+    const int N = 2 * B;
     const GemmMeta g = meta[m * 2 + p];
     if (m >= cut) {  // naive order: sampled P_l^m rows (pml.c:41-79), both directions the same table
         const int Li16 = (g.Lp + 15) & ~15, Li4 = (g.Lp + 3) & ~3;
+        const int B4 = pk_b4(B), B16 = pk_b16(B);
         const double* pm = T + toff[m];
         const double* pt = TT + toff[m];
-        for (int idx = threadIdx.x; idx < B * Li16; idx += blockDim.x) {
+        for (int idx = threadIdx.x; idx < B4 * Li16; idx += blockDim.x) {
             const int j = idx / Li16, i = idx - j * Li16;
-            AF[g.offF + idx] = (R)(i < g.Lp ? pm[(size_t)(p + 2 * i) * N + j] : 0.0);
+            AF[g.offF + idx] = (R)(i < g.Lp && j < B ? pm[(size_t)(p + 2 * i) * N + j] : 0.0);
         }
         const double inv_s2pi = 0.39894228040143267794;
-        for (int idx = threadIdx.x; idx < Li4 * B; idx += blockDim.x) {
-            const int i = idx / B, j = idx - i * B;
-            AI[g.offI + idx] = (R)(i < g.Lp ? pt[(size_t)(p + 2 * i) * N + j] * inv_s2pi : 0.0);
+        for (int idx = threadIdx.x; idx < Li4 * B16; idx += blockDim.x) {
+            const int i = idx / B16, j = idx - i * B16;
+            AI[g.offI + idx] = (R)(i < g.Lp && j < B ? pt[(size_t)(p + 2 * i) * N + j] * inv_s2pi : 0.0);
         }
         return;
     }
@@ -67,7 +76,7 @@ __global__ void k_pack_gemm(const double* __restrict__ T, const double* __restri
     }
     __syncthreads();
     const int Li16 = (g.Lp + 15) & ~15, Kf4 = (g.Kf + 3) & ~3;
-    const int Li4 = (g.Lp + 3) & ~3, Tp16 = (half + 15) & ~15;
+    const int Li4 = (g.Lp + 3) & ~3, Tp16 = pk_tp16(B), cnt = p ? B / 2 : (B + 1) / 2;
     const double* tm = T + toff[m];
     const double* ttm = TT + toff[m];
     for (int idx = threadIdx.x; idx < Kf4 * Li16; idx += blockDim.x) {
@@ -81,7 +90,7 @@ __global__ void k_pack_gemm(const double* __restrict__ T, const double* __restri
     for (int idx = threadIdx.x; idx < Li4 * Tp16; idx += blockDim.x) {
         const int i = idx / Tp16, t = idx - i * Tp16;
         double v = 0.0;
-        if (i < g.Lp && t < half) {
+        if (i < g.Lp && t < cnt) {
             const int ci = 2 * t + p;
             const int l = m + p + 2 * i;
             const int l0 = m + transpose_first(ci, m);
@@ -104,6 +113,35 @@ __host__ __device__ constexpr int lds_pad() {
     return sizeof(R) == 4 ? 4 : 2;
 }
 
+// Extents of the GEMMs at bandwidth B.  Cosine index k = 2t + p: hE = ceil(B/2)
+// even and hO = floor(B/2) odd ones.  Power-of-two B uses the fast pipeline's
+// parity-split Cb planes; any other B (gen) the per-field path's natural-order
+// complex DCT rows, with an LDS row long enough for the K loops' round-up
+// (B + 3) and the inverse's two padded parities (B + 6), zero past B.
+struct GGeo {
+    int B, N, hE, hO, rs, B4, B16, Tp16;
+};
+template <typename R>
+__host__ __device__ inline GGeo ggeo(int B, bool gen) {
+    GGeo g;
+    g.B = B;
+    g.N = 2 * B;
+    g.hE = (B + 1) / 2;
+    g.hO = B / 2;
+    g.rs = gen ? B + 8 : B + lds_pad<R>();
+    g.B4 = (B + 3) & ~3;
+    g.B16 = (B + 15) & ~15;
+    g.Tp16 = (g.hE + 15) & ~15;
+    return g;
+}
+
+// Orders m of a GEMM launch: grid (column blocks, roundup8(bw)); blocks past
+// the last order return at once.
+__device__ __forceinline__ bool gemm_block(int B, int& bx, int& m) {
+    xcd_block_m(bx, m);
+    return m < B;
+}
+
 // Coefficient (m, l) of data column col (field, +-m row, re/im) from the
 // projection v; REAL data also fills the -m run (f(l,-m) = (-1)^m conj f(l,m)).
 __device__ __forceinline__ void fwd_store(double v, int col, int m, int l, int B, int cpf, int fmt,
@@ -122,26 +160,43 @@ __device__ __forceinline__ void fwd_store(double v, int col, int m, int l, int B
 }
 
 // Seminaive order m < cutoff.  LDS: Bd[c][k2], k2 = parity-split cosine
-// position (p*bw/2 + t).
-template <typename R>
+// position (p*hE + t).
+template <typename R, bool GEN>
 __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __restrict__ Cb, const R* __restrict__ AF,
                                               const GemmMeta* __restrict__ meta, double* __restrict__ cre,
-                                              double* __restrict__ cim, int m, int cbase, int lgB, int F, int ncb,
-                                              int fmt) {
-    const int B = 1 << lgB, N = 2 * B, half = B >> 1;
+                                              double* __restrict__ cim, int m, int cbase, const GGeo& G, int F,
+                                              int ncb, int fmt) {
+    const int B = G.B, N = G.N;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
-    const int rs = B + lds_pad<R>();
-    for (int idx = threadIdx.x; idx < ncb * B; idx += kThreads) {
-        const int c = idx >> lgB, k2 = idx & (B - 1);
-        const int col = cbase + c;
-        R v = 0;
-        if (col < ncols) {
-            const int f = col / cpf, rem = col - f * cpf;
-            const int r = (rem >> 1) ? N - m : m;
-            v = Cb[(((size_t)f * N + r) * 2 + (rem & 1)) * B + k2];
+    const int rs = G.rs;
+    if constexpr (GEN) {  // natural-order complex rows; zero past B
+        const double2* C2 = reinterpret_cast<const double2*>(Cb);
+        for (int idx = threadIdx.x; idx < ncb * rs; idx += kThreads) {
+            const int c = idx / rs, k = idx - c * rs;
+            const int col = cbase + c;
+            R v = 0;
+            if (col < ncols && k < B) {
+                const int f = col / cpf, rem = col - f * cpf;
+                const int r = (rem >> 1) ? N - m : m;
+                const double2 x = C2[((size_t)f * N + r) * B + k];
+                v = (R)((rem & 1) ? x.y : x.x);
+            }
+            Bd[c * rs + (k < B ? ((k & 1) ? G.hE + (k >> 1) : (k >> 1)) : k)] = v;
         }
-        Bd[c * rs + k2] = v;
+    } else {
+        const int lgB = __builtin_ctz(B);
+        for (int idx = threadIdx.x; idx < ncb * B; idx += kThreads) {
+            const int c = idx >> lgB, k2 = idx & (B - 1);
+            const int col = cbase + c;
+            R v = 0;
+            if (col < ncols) {
+                const int f = col / cpf, rem = col - f * cpf;
+                const int r = (rem >> 1) ? N - m : m;
+                v = Cb[(((size_t)f * N + r) * 2 + (rem & 1)) * B + k2];
+            }
+            Bd[c * rs + k2] = v;
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -157,7 +212,7 @@ __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __res
             const int ilast = min(i0 + 15, g.Lp - 1);
             const int kend = (row_size(m, m + p + 2 * ilast) + 3) & ~3;
             typename Mma<R>::acc acc = {0, 0, 0, 0};
-            const R* bcol = Bd + (c0 + (lane & 15)) * rs + p * half;
+            const R* bcol = Bd + (c0 + (lane & 15)) * rs + p * G.hE;
             const R* arow = AF + g.offF + i0 + (lane & 15);
             for (int t0 = 0; t0 < kend; t0 += 4) {
                 const int tk = t0 + (lane >> 4);
@@ -178,29 +233,29 @@ __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __res
 // grid is symmetric, x_{N-1-j} = -x_j, so P_l^m(x_{N-1-j}) = (-1)^{l-m}
 // P_l^m(x_j): degree l = m+p+2i is the sum over the bw rings j < bw of
 // (w_j X_j + (-1)^p w_{N-1-j} X_{N-1-j}) P_l^m(x_j).  One parity at a time,
-// LDS: Bd[c][j].
+// LDS: Bd[c][j], zero for j >= bw.
 template <typename R>
 __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typename Cplx<R>::type* __restrict__ X,
                                                const double* __restrict__ wn, const R* __restrict__ AF,
                                                const GemmMeta* __restrict__ meta, double* __restrict__ cre,
-                                               double* __restrict__ cim, int m, int cbase, int lgB, int F, int ncb,
-                                               int fmt) {
+                                               double* __restrict__ cim, int m, int cbase, const GGeo& G, int F,
+                                               int ncb, int fmt) {
     using T = typename Cplx<R>::type;
-    const int B = 1 << lgB, N = 2 * B;
+    const int B = G.B, N = G.N, B4 = G.B4;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
-    const int rs = B + lds_pad<R>();
+    const int rs = G.rs;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nct = ncb >> 4;
     for (int p = 0; p < 2; ++p) {
         const GemmMeta g = meta[m * 2 + p];
         if (g.Lp == 0) break;
         if (p) __syncthreads();  // parity 0's reads of Bd are done
-        for (int idx = threadIdx.x; idx < ncb * B; idx += kThreads) {
-            const int c = idx >> lgB, j = idx & (B - 1);
+        for (int idx = threadIdx.x; idx < ncb * B4; idx += kThreads) {
+            const int c = idx / B4, j = idx - c * B4;
             const int col = cbase + c;
             R v = 0;
-            if (col < ncols) {
+            if (col < ncols && j < B) {
                 const int f = col / cpf, rem = col - f * cpf;
                 const T* xr = X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N;
                 const T a = xr[j], b = xr[N - 1 - j];
@@ -219,7 +274,7 @@ __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typenam
             typename Mma<R>::acc acc = {0, 0, 0, 0};
             const R* bcol = Bd + (c0 + (lane & 15)) * rs;
             const R* arow = AF + g.offF + i0 + (lane & 15);
-            for (int t0 = 0; t0 < B; t0 += 4) {
+            for (int t0 = 0; t0 < B4; t0 += 4) {
                 const int tk = t0 + (lane >> 4);
                 acc = Mma<R>::mma(bcol[tk], arow[(size_t)tk * Li16], acc);
             }
@@ -234,37 +289,39 @@ __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typenam
     }
 }
 
-// Forward: D'[c][i] = sum_k data[c][k] * A(m,p)[i][k].  grid (ceil(cols/ncb), bw).
-template <typename R>
+// Forward: D'[c][i] = sum_k data[c][k] * A(m,p)[i][k].  grid (ceil(cols/ncb), roundup8(bw)).
+template <typename R, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
                                                        const typename Cplx<R>::type* __restrict__ X,
                                                        const double* __restrict__ wn, const R* __restrict__ AF,
                                                        const GemmMeta* __restrict__ meta, double* __restrict__ cre,
-                                                       double* __restrict__ cim, int lgB, int cut, int F, int ncb,
+                                                       double* __restrict__ cim, int B, int cut, int F, int ncb,
                                                        int fmt) {
     extern __shared__ __align__(16) unsigned char smem[];
     R* Bd = reinterpret_cast<R*>(smem);
     int bx, m;
-    xcd_block_m(bx, m);
+    if (!gemm_block(B, bx, m)) return;
     const int cbase = bx * ncb;
     if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
+    const GGeo G = ggeo<R>(B, GEN);
     if (m < cut)
-        gemm_fwd_semi<R>(Bd, Cb, AF, meta, cre, cim, m, cbase, lgB, F, ncb, fmt);
+        gemm_fwd_semi<R, GEN>(Bd, Cb, AF, meta, cre, cim, m, cbase, G, F, ncb, fmt);
     else
-        gemm_fwd_naive<R>(Bd, X, wn, AF, meta, cre, cim, m, cbase, lgB, F, ncb, fmt);
+        gemm_fwd_naive<R>(Bd, X, wn, AF, meta, cre, cim, m, cbase, G, F, ncb, fmt);
 }
 
 // Coefficient columns of order m into LDS: Cd[c][q], q = i (p = 0) or
 // roundup4(Lp0) + i (p = 1), with the padding rows of both parities zeroed;
-// q < bw always.  `negodd`: negate the -m columns of odd orders.
+// q < bw + 6.  `negodd`: negate the -m columns of odd orders.
 template <typename R>
 __device__ __forceinline__ void inv_fill(R* __restrict__ Cd, const double* __restrict__ cre,
                                          const double* __restrict__ cim, const GemmMeta& g0, const GemmMeta& g1,
-                                         int m, int cbase, int B, int ncb, int cpf, int ncols, bool negodd) {
+                                         int m, int cbase, const GGeo& G, int ncb, int cpf, int ncols, bool negodd) {
+    const int B = G.B;
     const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
     const int L = B - m;
     const size_t B2 = (size_t)B * B;
-    const int rs = B + lds_pad<R>();
+    const int rs = G.rs;
     const bool neg = negodd && (m & 1);
     for (int idx = threadIdx.x; idx < ncb * L; idx += kThreads) {
         const int c = idx / L, li = idx - c * L;
@@ -288,27 +345,28 @@ __device__ __forceinline__ void inv_fill(R* __restrict__ Cd, const double* __res
 }
 
 // Seminaive order m < cutoff: D'[c][t] = sum_i coef[c][i] * AI(m,p)[i][t],
-// cosine coefficients into the Cb row of (field, +-m).
-template <typename R>
+// cosine coefficients k = 2t + p into the Cb row of (field, +-m).
+template <typename R, bool GEN>
 __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* __restrict__ cre,
                                               const double* __restrict__ cim, const R* __restrict__ AI,
                                               const GemmMeta* __restrict__ meta, R* __restrict__ Cb, int m, int cbase,
-                                              int lgB, int F, int ncb, int fmt) {
-    const int B = 1 << lgB, N = 2 * B, half = B >> 1;
+                                              const GGeo& G, int F, int ncb, int fmt) {
+    const int B = G.B, N = G.N;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
     const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
     const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
-    const int rs = B + lds_pad<R>();
-    inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, B, ncb, cpf, ncols, false);
+    const int rs = G.rs;
+    inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, G, ncb, cpf, ncols, false);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nct = ncb >> 4;
-    const int Tp16 = (half + 15) & ~15;
+    const int Tp16 = G.Tp16;
     const int ntt = Tp16 >> 4;
     for (int p = 0; p < 2; ++p) {
         const GemmMeta g = p ? g1 : g0;
         const int Li4 = p ? Li4_1 : Li4_0;
+        const int cnt = p ? G.hO : G.hE;
         const int r0 = g.Lp > 0 ? row_size(m, m + p) : 0;
         for (int item = wave; item < ntt * nct; item += kThreads / 64) {
             const int tt = item / nct, ct = item - tt * nct;
@@ -322,14 +380,17 @@ __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* 
                 acc = Mma<R>::mma(acol[ik], brow[(size_t)ik * Tp16], acc);
             }
             const int t = t0 + (lane & 15);
-            if (t >= half) continue;
+            if (t >= cnt) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = cbase + c0 + Mma<R>::row(lane, r);
                 if (col >= ncols) continue;
                 const int f = col / cpf, rem = col - f * cpf;
                 const int row = (rem >> 1) ? N - m : m;
-                Cb[(((size_t)f * N + row) * 2 + (rem & 1)) * B + p * half + t] = acc[r];
+                if constexpr (GEN)
+                    reinterpret_cast<R*>(Cb)[(((size_t)f * N + row) * B + 2 * t + p) * 2 + (rem & 1)] = acc[r];
+                else
+                    Cb[(((size_t)f * N + row) * 2 + (rem & 1)) * B + p * G.hE + t] = acc[r];
             }
         }
     }
@@ -344,18 +405,18 @@ template <typename R>
 __device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double* __restrict__ cre,
                                                const double* __restrict__ cim, const R* __restrict__ AI,
                                                const GemmMeta* __restrict__ meta, typename Cplx<R>::type* __restrict__ X,
-                                               int m, int cbase, int lgB, int F, int ncb, int fmt) {
-    const int B = 1 << lgB, N = 2 * B;
+                                               int m, int cbase, const GGeo& G, int F, int ncb, int fmt) {
+    const int B = G.B, N = G.N, B16 = G.B16;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
     const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
     const int Li4_0 = (g0.Lp + 3) & ~3, Li4_1 = (g1.Lp + 3) & ~3;
-    const int rs = B + lds_pad<R>();
-    inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, B, ncb, cpf, ncols, true);
+    const int rs = G.rs;
+    inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, G, ncb, cpf, ncols, true);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nct = ncb >> 4;
-    const int ntt = B >> 4;
+    const int ntt = B16 >> 4;
     const bool mirror = fmt == 1 && m > 0;
     for (int item = wave; item < ntt * nct; item += kThreads / 64) {
         const int tt = item / nct, ct = item - tt * nct;
@@ -365,14 +426,15 @@ __device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double*
         const R* b0 = AI + g0.offI + t0 + (lane & 15);
         for (int i0 = 0; i0 < Li4_0; i0 += 4) {
             const int ik = i0 + (lane >> 4);
-            e = Mma<R>::mma(acol[ik], b0[(size_t)ik * B], e);
+            e = Mma<R>::mma(acol[ik], b0[(size_t)ik * B16], e);
         }
         const R* b1 = AI + g1.offI + t0 + (lane & 15);
         for (int i0 = 0; i0 < Li4_1; i0 += 4) {
             const int ik = i0 + (lane >> 4);
-            o = Mma<R>::mma(acol[Li4_0 + ik], b1[(size_t)ik * B], o);
+            o = Mma<R>::mma(acol[Li4_0 + ik], b1[(size_t)ik * B16], o);
         }
         const int j = t0 + (lane & 15);
+        if (j >= B) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int col = cbase + c0 + Mma<R>::row(lane, r);
@@ -391,22 +453,23 @@ __device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double*
     }
 }
 
-// Inverse.  grid (ceil(cols/ncb), bw).
-template <typename R>
+// Inverse.  grid (ceil(cols/ncb), roundup8(bw)).
+template <typename R, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
                                                        const R* __restrict__ AI, const GemmMeta* __restrict__ meta,
                                                        R* __restrict__ Cb, typename Cplx<R>::type* __restrict__ X,
-                                                       int lgB, int cut, int F, int ncb, int fmt) {
+                                                       int B, int cut, int F, int ncb, int fmt) {
     extern __shared__ __align__(16) unsigned char smem[];
     R* Cd = reinterpret_cast<R*>(smem);
     int bx, m;
-    xcd_block_m(bx, m);
+    if (!gemm_block(B, bx, m)) return;
     const int cbase = bx * ncb;
     if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
+    const GGeo G = ggeo<R>(B, GEN);
     if (m < cut)
-        gemm_inv_semi<R>(Cd, cre, cim, AI, meta, Cb, m, cbase, lgB, F, ncb, fmt);
+        gemm_inv_semi<R, GEN>(Cd, cre, cim, AI, meta, Cb, m, cbase, G, F, ncb, fmt);
     else
-        gemm_inv_naive<R>(Cd, cre, cim, AI, meta, X, m, cbase, lgB, F, ncb, fmt);
+        gemm_inv_naive<R>(Cd, cre, cim, AI, meta, X, m, cbase, G, F, ncb, fmt);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,10 +477,11 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
 // ---------------------------------------------------------------------------
 
 void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t s) {
-    const int B = t.bw, half = B / 2, cut = t.cutoff;
+    const int B = t.bw, cut = t.cutoff;
     g.bw = B;
     g.prec = prec;
     g.cut = cut;
+    g.gen = !gemm_pow2(B);
     g.h_meta.assign(2 * B, GemmMeta{});
     long of = 0, oi = 0;
     for (int m = 0; m < B; ++m)
@@ -428,14 +492,14 @@ void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t
             e.offI = oi;
             if (m >= cut) {  // naive: bw folded rings in both directions
                 e.Kf = B;
-                of += (long)B * ((e.Lp + 15) & ~15);
-                oi += (long)((e.Lp + 3) & ~3) * B;
+                of += (long)pk_b4(B) * ((e.Lp + 15) & ~15);
+                oi += (long)((e.Lp + 3) & ~3) * pk_b16(B);
                 continue;
             }
             const int llast = m + p + 2 * (e.Lp - 1);
             e.Kf = e.Lp > 0 ? row_size(m, llast) : 0;
             of += (long)((e.Kf + 3) & ~3) * ((e.Lp + 15) & ~15);
-            oi += (long)((e.Lp + 3) & ~3) * ((half + 15) & ~15);
+            oi += (long)((e.Lp + 3) & ~3) * pk_tp16(B);
         }
     g.nf = of;
     g.ni = oi;
@@ -472,24 +536,66 @@ size_t fast_cb_bytes(int bw, int prec) {
     return N * 2 * bw * (prec == 0 ? sizeof(double) : sizeof(float));
 }
 
+// LDS row bytes of a GEMM data column.
+static int gemm_row(int B, int prec) {
+    const bool gen = !gemm_pow2(B);
+    return prec == 0 ? 8 * ggeo<double>(B, gen).rs : 4 * ggeo<float>(B, gen).rs;
+}
+
 // Columns per GEMM block: the largest multiple of 16 (at most 64) whose
 // padded LDS tile fits in 64 KiB, or failing that in gfx950's 160 KiB.
 static int gemm_ncb(int B, int prec) {
-    const int row = prec == 0 ? 8 * (B + lds_pad<double>()) : 4 * (B + lds_pad<float>());
+    const int row = gemm_row(B, prec);
     const int small = std::min(64, 65536 / row) & ~15;
     return small >= 16 ? small : (std::min(64, 163840 / row) & ~15);
 }
 
-static size_t gemm_lds(int B, int prec) {
-    const int row = prec == 0 ? 8 * (B + lds_pad<double>()) : 4 * (B + lds_pad<float>());
-    return (size_t)gemm_ncb(B, prec) * row;
-}
+static size_t gemm_lds(int B, int prec) { return (size_t)gemm_ncb(B, prec) * gemm_row(B, prec); }
+
+bool gemm_pow2(int bw) { return bw >= 32 && bw <= 1024 && !(bw & (bw - 1)); }
 
 bool fast_supported(int bw, int cutoff, int prec) {
-    const int N = 2 * bw;
-    // N*N must hold at least one block tile of rows (N >= 64); any cutoff in [0, bw]
-    if (bw < 32 || (N & (N - 1)) || N > 2048 || cutoff < 0 || cutoff > bw) return false;
+    if (cutoff < 0 || cutoff > bw || bw > 1024) return false;
+    // power-of-two grids with N >= 64: the register FFTs (s2_blocks.h) and the
+    // GEMMs in FP64 or FP32; other bw >= 16: the per-field path's mixed-radix
+    // FFTs and DCTs with the GEMMs, FP64 only
+    if (!gemm_pow2(bw) && (prec != 0 || bw < 16)) return false;
     return gemm_ncb(bw, prec) >= 16;  // GEMM LDS tile of at least 16 columns
+}
+
+void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre, double* cim,
+                     int F, Fmt fmt, hipStream_t s) {
+    const int B = c.bw, ncb = gemm_ncb(B, g.prec), cols = 4 * F;
+    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
+    const size_t lds = gemm_lds(B, g.prec);
+    if (g.prec != 0)
+        launch_lds(k_gemm_fwd<float, false>, grid, kThreads, lds, s, (const float*)Cb, (const float2*)X,
+                   (const double*)c.w_naive, (const float*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
+                   (int)fmt);
+    else if (g.gen)
+        launch_lds(k_gemm_fwd<double, true>, grid, kThreads, lds, s, (const double*)Cb, (const double2*)X,
+                   (const double*)c.w_naive, (const double*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
+                   (int)fmt);
+    else
+        launch_lds(k_gemm_fwd<double, false>, grid, kThreads, lds, s, (const double*)Cb, (const double2*)X,
+                   (const double*)c.w_naive, (const double*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
+                   (int)fmt);
+}
+
+void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
+                     int F, Fmt fmt, hipStream_t s) {
+    const int B = c.bw, ncb = gemm_ncb(B, g.prec), cols = 4 * F;
+    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
+    const size_t lds = gemm_lds(B, g.prec);
+    if (g.prec != 0)
+        launch_lds(k_gemm_inv<float, false>, grid, kThreads, lds, s, cre, cim, (const float*)g.ai,
+                   (const GemmMeta*)g.meta, (float*)Cb, (float2*)X, B, g.cut, F, ncb, (int)fmt);
+    else if (g.gen)
+        launch_lds(k_gemm_inv<double, true>, grid, kThreads, lds, s, cre, cim, (const double*)g.ai,
+                   (const GemmMeta*)g.meta, (double*)Cb, (double2*)X, B, g.cut, F, ncb, (int)fmt);
+    else
+        launch_lds(k_gemm_inv<double, false>, grid, kThreads, lds, s, cre, cim, (const double*)g.ai,
+                   (const GemmMeta*)g.meta, (double*)Cb, (double2*)X, B, g.cut, F, ncb, (int)fmt);
 }
 
 template <typename R>
@@ -530,12 +636,8 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
         }
     }
     {
-        const int ncb = gemm_ncb(B, g.prec);
-        const int cols = 4 * F;
         Span sp(tm, ST_LEG_FWD, s);
-        launch_lds(k_gemm_fwd<R>, dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s, (const R*)Cb,
-                   (const T*)X, (const double*)c.w_naive, (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, lg2(B),
-                   cut, F, ncb, (int)fmt);
+        gemm_fwd_launch(c, g, Cb, X, cre, cim, F, fmt, s);
     }
 }
 
@@ -553,11 +655,8 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     const T* ph = (const T*)(sizeof(R) == 8 ? (const void*)c.ph : (const void*)c.ph32);
     const R* sv = (const R*)(sizeof(R) == 8 ? (const void*)c.sinv : (const void*)c.sinv32);
     {
-        const int ncb = gemm_ncb(B, g.prec);
-        const int cols = 4 * F;
         Span sp(tm, ST_LEG_INV, s);
-        launch_lds(k_gemm_inv<R>, dim3((cols + ncb - 1) / ncb, B), kThreads, gemm_lds(B, g.prec), s, cre, cim,
-                   (const R*)g.ai, (const GemmMeta*)g.meta, Cb, X, lg2(B), g.cut, F, ncb, (int)fmt);
+        gemm_inv_launch(c, g, cre, cim, Cb, X, F, fmt, s);
     }
     // DCT-III of the seminaive order rows (the naive rows are already in X)
     const int cut = g.cut, nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
@@ -587,7 +686,9 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
 
 void forward_fast(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
                   double* cim, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm) {
-    if (g.prec == 0)
+    if (g.gen)
+        forward_fp64_gemm(c, g, re, im, cre, cim, F, fmt, (double2*)X, (double2*)Cb, s, tm);
+    else if (g.prec == 0)
         fwd_impl<double>(c, g, re, im, cre, cim, F, fmt, X, Cb, s, tm);
     else
         fwd_impl<float>(c, g, re, im, cre, cim, F, fmt, X, Cb, s, tm);
@@ -595,7 +696,9 @@ void forward_fast(const DevConsts& c, const GemmTables& g, const double* re, con
 
 void inverse_fast(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
                   double* im, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm) {
-    if (g.prec == 0)
+    if (g.gen)
+        inverse_fp64_gemm(c, g, cre, cim, re, im, F, fmt, (double2*)X, (double2*)Cb, s, tm);
+    else if (g.prec == 0)
         inv_impl<double>(c, g, cre, cim, re, im, F, fmt, X, Cb, s, tm);
     else
         inv_impl<float>(c, g, cre, cim, re, im, F, fmt, X, Cb, s, tm);

@@ -90,6 +90,7 @@ struct GemmMeta {
 struct GemmTables {
     int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32 (seminaive); 2: FP32 naive (mixed form); 3: split BF16 naive
     int cut = 0;           // prec 0, 1: orders m >= cut are naive (sampled P_l^m tables)
+    bool gen = false;      // prec 0, bw not a power of two: natural-order Cb rows of the per-field path
     void* af = nullptr;
     void* ai = nullptr;
     GemmMeta* meta = nullptr;
@@ -178,8 +179,23 @@ void inv_dlt_semi_one(const double* coeffs, const double* ttab, const double* si
 void dlt_naive_one(const double* data, const double* w, const double* pml, double* out, int B, int m, hipStream_t s);
 void inv_dlt_naive_one(const double* coeffs, const double* pml, double* out, int B, int m, hipStream_t s);
 
-// ---- fast path (s2_fast.hip): power-of-two grids, cutoff = bw ----------
+// FP64 pipeline with the MFMA GEMMs of s2_fast.hip for both Legendre
+// branches: the per-field mixed-radix FFTs and DCTs above, for bandwidths
+// that are not powers of two (g.gen).
+void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
+                       double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm);
+void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
+                       double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm);
+
+// ---- GEMM path (s2_fast.hip): MFMA Legendre GEMMs at any cutoff --------
+// Power-of-two bw in [32, 1024]: register FFTs, FP64 or FP32.  Other bw in
+// [16, 1024]: FP64, on the per-field FFT/DCT kernels.
+bool gemm_pow2(int bw);
 bool fast_supported(int bw, int cutoff, int prec);
+void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre, double* cim,
+                     int F, Fmt fmt, hipStream_t s);
+void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
+                     int F, Fmt fmt, hipStream_t s);
 void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t s);
 void gemm_tables_destroy(GemmTables& g);
 size_t fast_x_bytes(int bw, int prec);   // scratch X per field

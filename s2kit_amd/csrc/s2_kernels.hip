@@ -108,14 +108,14 @@ __device__ __forceinline__ void mr_pass(const T* a, T* b, int N, int Ns, int nff
     }
 }
 
-// The same pass for a radix known only at run time (prime factors above 7):
-// one output per thread, sum_q a[j + q*N/R] * exp(SIGN 2 pi i q (k + p*Ns) / (Ns*R)),
-// the butterfly with its twiddle merged in.
+// The same pass for a radix known only at run time (prime factors above 3):
+// one output per thread, sum_q a[j + q*N/R] * w^q with
+// w = exp(SIGN 2 pi i (k + p*Ns) / (Ns*R)), the butterfly with its twiddle
+// merged in; w^q by repeated multiplication (relative error ~q ulp).
 template <typename T, int SIGN>
 __device__ __forceinline__ void mr_pass_any(const T* a, T* b, int N, int Ns, int R, int nfft,
                                             const T* __restrict__ tw) {
-    using F = typename Real<T>::type;
-    const int NR = N / R, NsR = Ns * R, step = N / NsR;
+    const int NR = N / R, step = N / (Ns * R);
     const unsigned long long mN = magic16(N), mNs = magic16(Ns), mR = magic16(R);
     const int total = nfft * N;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
@@ -123,19 +123,16 @@ __device__ __forceinline__ void mr_pass_any(const T* a, T* b, int N, int Ns, int
         const int t1 = div16(o, mNs), k = o - t1 * Ns;
         const int jq = div16(t1, mR), p = t1 - jq * R;
         const T* in = a + f * N + jq * Ns + k;
-        const int e0 = k + p * Ns;  // < Ns*R
-        F sr = 0, si = 0;
-        int e = 0;
-        for (int q = 0; q < R; ++q) {
-            T w = tw[e * step];
-            if (SIGN > 0) w.y = -w.y;
+        T w1 = tw[(k + p * Ns) * step];  // (k + p*Ns) * step < N
+        if (SIGN > 0) w1.y = -w1.y;
+        T acc = in[0], w = w1;
+        for (int q = 1; q < R; ++q) {
             const T v = in[q * NR];
-            sr += v.x * w.x - v.y * w.y;
-            si += v.x * w.y + v.y * w.x;
-            e += e0;
-            if (e >= NsR) e -= NsR;
+            acc.x += v.x * w.x - v.y * w.y;
+            acc.y += v.x * w.y + v.y * w.x;
+            w = cmul(w, w1);
         }
-        b[f * N + o] = mk<T>(sr, si);
+        b[f * N + o] = acc;
     }
 }
 
@@ -1399,6 +1396,52 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         Span sp(tm, ST_NAIVE_INV, s);
         k_naive_inv<<<dim3(B - cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
             cre, cim, t.inv, t.off, X, N, B, cut, F, fc, f);
+        S2_HIP(hipGetLastError());
+    }
+    Span sp(tm, ST_PHI_INV, s);
+    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, s,
+               X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
+    S2_HIP(hipGetLastError());
+}
+
+void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
+                       double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
+    check_launch_shape(c);
+    const int N = c.N, B = c.bw, cut = g.cut;
+    const int rpb = rings_per_block(N);
+    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    {
+        Span sp(tm, ST_PHI_FWD, s);
+        fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft,
+                   s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
+        S2_HIP(hipGetLastError());
+    }
+    if (cut > 0) {
+        const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+        Span sp(tm, ST_DCT_FWD, s);
+        fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
+                   lds_fft, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+        S2_HIP(hipGetLastError());
+    }
+    Span sp(tm, ST_LEG_FWD, s);
+    gemm_fwd_launch(c, g, Cb, X, cre, cim, F, fmt, s);
+}
+
+void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, double* re,
+                       double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
+    check_launch_shape(c);
+    const int N = c.N, B = c.bw, cut = g.cut, f = (int)fmt;
+    const int rpb = rings_per_block(N);
+    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    {
+        Span sp(tm, ST_LEG_INV, s);
+        gemm_inv_launch(c, g, cre, cim, Cb, X, F, fmt, s);
+    }
+    if (cut > 0) {
+        const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+        Span sp(tm, ST_DCT_INV, s);
+        fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
+                   lds_fft, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
