@@ -108,31 +108,49 @@ __device__ __forceinline__ void mr_pass(const T* a, T* b, int N, int Ns, int nff
     }
 }
 
-// The same pass for a radix known only at run time (prime factors above 3):
-// one output per thread, sum_q a[j + q*N/R] * w^q with
-// w = exp(SIGN 2 pi i (k + p*Ns) / (Ns*R)), the butterfly with its twiddle
-// merged in; w^q by repeated multiplication (relative error ~q ulp).
+// The same pass for an odd prime radix R >= 5 known only at run time.  Work
+// item (butterfly j, s), s <= H = (R-1)/2, writes output p = 0 (s = 0) or
+// the pair p = s, R-s, which share their inputs:
+//   y_s, y_{R-s} = v_0 + sum_{q=1..H} (v_q + v_{R-q}) Re(w^{sq}) +- i (v_q - v_{R-q}) Im(w^{sq}),
+// v_q = a[j + q*N/R] * wk^q (wk = exp(SIGN 2 pi i k / (Ns*R)), wk^{R-q} =
+// wk^R conj(wk^q)), w = exp(SIGN 2 pi i / R); powers by repeated
+// multiplication (relative error ~q ulp).
 template <typename T, int SIGN>
-__device__ __forceinline__ void mr_pass_any(const T* a, T* b, int N, int Ns, int R, int nfft,
+__device__ __forceinline__ void mr_pass_odd(const T* a, T* b, int N, int Ns, int R, int nfft,
                                             const T* __restrict__ tw) {
-    const int NR = N / R, step = N / (Ns * R);
-    const unsigned long long mN = magic16(N), mNs = magic16(Ns), mR = magic16(R);
-    const int total = nfft * N;
+    const int NR = N / R, step = N / (Ns * R), H = (R - 1) >> 1, H1 = H + 1;
+    const unsigned long long mH1 = magic16(H1), mNR = magic16(NR), mNs = magic16(Ns);
+    const int total = nfft * NR * H1;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int f = div16(idx, mN), o = idx - f * N;
-        const int t1 = div16(o, mNs), k = o - t1 * Ns;
-        const int jq = div16(t1, mR), p = t1 - jq * R;
-        const T* in = a + f * N + jq * Ns + k;
-        T w1 = tw[(k + p * Ns) * step];  // (k + p*Ns) * step < N
-        if (SIGN > 0) w1.y = -w1.y;
-        T acc = in[0], w = w1;
-        for (int q = 1; q < R; ++q) {
-            const T v = in[q * NR];
-            acc.x += v.x * w.x - v.y * w.y;
-            acc.y += v.x * w.y + v.y * w.x;
-            w = cmul(w, w1);
+        const int bfi = div16(idx, mH1), sp_ = idx - bfi * H1;
+        const int f = div16(bfi, mNR), j = bfi - f * NR;
+        const int jq = div16(j, mNs), k = j - jq * Ns;
+        const T* in = a + f * N + j;
+        T wk = tw[k * step], WR = tw[k * step * R], om = tw[sp_ * NR];  // indices < N
+        if (SIGN > 0) {
+            wk.y = -wk.y;
+            WR.y = -WR.y;
+            om.y = -om.y;
         }
-        b[f * N + o] = acc;
+        T acc1 = in[0], acc2 = in[0], wq = wk, oq = om;
+        for (int q = 1; q <= H; ++q) {
+            const T vq = cmul(in[q * NR], wq);
+            const T vr = cmul(in[(R - q) * NR], cmul(WR, cconj(wq)));
+            const T su = cadd(vq, vr), df = csub(vq, vr);
+            acc1.x += su.x * oq.x - df.y * oq.y;
+            acc1.y += su.y * oq.x + df.x * oq.y;
+            acc2.x += su.x * oq.x + df.y * oq.y;
+            acc2.y += su.y * oq.x - df.x * oq.y;
+            wq = cmul(wq, wk);
+            oq = cmul(oq, om);
+        }
+        T* out = b + f * N + jq * Ns * R + k;
+        if (sp_ == 0) {
+            out[0] = acc1;
+        } else {
+            out[sp_ * Ns] = acc1;
+            out[(R - sp_) * Ns] = acc2;
+        }
     }
 }
 
@@ -169,8 +187,8 @@ __device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T
     }
     // Other lengths: mixed-radix Stockham (autosort), one pass per factor R
     // of N (4 while 4 divides, then 2, then odd factors in ascending order).
-    // Radices 2, 3 and 4 run as butterflies; the rest one output per thread
-    // (more butterfly instances in one kernel made hipcc spill).
+    // Radices 2, 3 and 4 run as butterflies; larger primes as conjugate output
+    // pairs (more butterfly instances in one kernel made hipcc spill).
     int rem = N, Ns = 1;
     while (rem > 1) {
         int R = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : 3;
@@ -179,7 +197,7 @@ __device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T
             case 2: mr_pass<T, SIGN, 2>(a, b, N, Ns, nfft, tw); break;
             case 3: mr_pass<T, SIGN, 3>(a, b, N, Ns, nfft, tw); break;
             case 4: mr_pass<T, SIGN, 4>(a, b, N, Ns, nfft, tw); break;
-            default: mr_pass_any<T, SIGN>(a, b, N, Ns, R, nfft, tw); break;
+            default: mr_pass_odd<T, SIGN>(a, b, N, Ns, R, nfft, tw); break;
         }
         __syncthreads();
         T* t = a;
