@@ -18,6 +18,8 @@ Also reported on the same JSON line:
               MFMA GEMMs; split: BF16 MFMA with split operands) timed on the
               same fields, with their errors against FP64; and the FP64 form
               at cutoff = bw/4 (naive branch for orders >= 64);
+  non_pow2_bw255  FP64 pairs/s at bw=255 (mixed-radix FFTs) beside bw=256,
+              256 fields, and the bw=255 batch against its per-field path;
   cpu_baseline  the CPU FP64 oracle (a port of the reference path) timed on
               the host, one process per core, on a bounded sample.
 
@@ -67,11 +69,17 @@ KERNELS = {
               "phi_inv": "k_fphi_inv<float, 16, 512>"},
     "split": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "leg_fwd": "k_split_fwd", "leg_inv": "k_split_inv",
               "phi_inv": "k_fphi_inv<float, 16, 512>"},
-    "fp32": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "dct_fwd": "k_fdct_fwd<float, 16>", "leg_fwd": "k_gemm_fwd<float>",
-             "leg_inv": "k_gemm_inv<float>", "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16, 512>"},
-    "fp64": {"phi_fwd": "k_fphi_fwd<double, 8, 512>", "dct_fwd": "k_fdct_fwd<double, 8>", "leg_fwd": "k_gemm_fwd<double>",
-             "leg_inv": "k_gemm_inv<double>", "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8, 512>"},
+    "fp32": {"phi_fwd": "k_fphi_fwd<float, 16, 512>", "dct_fwd": "k_fdct_fwd<float, 16>",
+             "leg_fwd": "k_gemm_fwd<float, false>", "leg_inv": "k_gemm_inv<float, false>",
+             "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16, 512>"},
+    "fp64": {"phi_fwd": "k_fphi_fwd<double, 8, 512>", "dct_fwd": "k_fdct_fwd<double, 8>",
+             "leg_fwd": "k_gemm_fwd<double, false>", "leg_inv": "k_gemm_inv<double, false>",
+             "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8, 512>"},
 }
+# FP64 at a bandwidth that is not a power of two: mixed-radix FFT/DCT kernels
+KERNELS_GEN = {"phi_fwd": "k_phi_fwd<double2, true>", "dct_fwd": "k_dct_fwd<double2, true>",
+               "leg_fwd": "k_gemm_fwd<double, true>", "leg_inv": "k_gemm_inv<double, true>",
+               "dct_inv": "k_dct_inv<double2, true>", "phi_inv": "k_phi_inv<double2, true>"}
 
 
 def measured_traffic(form, bw, nfields, stage):
@@ -100,6 +108,47 @@ def table_doubles(bw):
             off += hl + 1
         tot += off
     return tot
+
+
+def non_pow2(dev, steps):
+    """FP64 plans at bw=255 (510 = 2*3*5*17 rings: mixed-radix FFTs and DCTs,
+    MFMA Legendre GEMMs) and bw=256, 256 fields each; the bw=255 GEMM path
+    is checked against the same plan's per-field path (one field per call)."""
+    import torch
+    import s2kit_amd as S
+    from s2kit_amd import synth
+    out = {"fields": 256}
+    for bw in (255, 256):
+        F, n = 256, 2 * bw
+        c0r, c0i = synth.power_law_coeffs(bw, F, 7, dev)
+        gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+        gim = torch.empty_like(gre)
+        cr, ci = torch.empty_like(c0r), torch.empty_like(c0i)
+        p = S.Plan(bw, F, precision=S.FP64)
+        for _ in range(2):
+            p.inverse(c0r, c0i, gre, gim)
+            p.forward(gre, gim, cr, ci)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            p.inverse(c0r, c0i, gre, gim)
+            p.forward(gre, gim, cr, ci)
+        torch.cuda.synchronize()
+        out[f"bw{bw}_pairs_per_s"] = F * steps / (time.perf_counter() - t0)
+        if bw == 255:  # field 0 through the per-field kernels
+            r1, i1 = torch.empty_like(cr[:1]), torch.empty_like(ci[:1])
+            g1r, g1i = torch.empty_like(gre[:1]), torch.empty_like(gim[:1])
+            p.forward(gre[:1], gim[:1], r1, i1, nfields=1)
+            p.inverse(c0r[:1], c0i[:1], g1r, g1i, nfields=1)
+            torch.cuda.synchronize()
+            out["bw255_forward_vs_per_field"] = float((torch.complex(cr[0], ci[0]) - torch.complex(r1[0], i1[0])).abs().max()
+                                                      / torch.complex(r1[0], i1[0]).abs().max())
+            out["bw255_inverse_vs_per_field"] = float((torch.complex(gre[0], gim[0]) - torch.complex(g1r[0], g1i[0])).abs().max()
+                                                      / torch.complex(g1r[0], g1i[0]).abs().max())
+        p.close()
+        del gre, gim
+        torch.cuda.empty_cache()
+    return out
 
 
 def single_field_bw512(dev, pairs):
@@ -345,6 +394,8 @@ def main():
                     help="calls timed per drop-in case (bw=64 pair, bw=128 conv); 0: skip")
     ap.add_argument("--single-pairs", type=int, default=10,
                     help="FP64 pairs timed on one bw=512 field (table streaming); 0: skip")
+    ap.add_argument("--non-pow2-steps", type=int, default=3,
+                    help="FP64 steps timed at bw=255 and bw=256, 256 fields (0: skip)")
     ap.add_argument("--workload", choices=["sht", "conv"], default="sht",
                     help="sht: batched fwd+inv SHT (default); conv: uneven-bandwidth convolution batch")
     ap.add_argument("--conv-requests", type=int, default=4096, help="requests in the conv workload")
@@ -498,6 +549,9 @@ def main():
     # ---- one field at bw=512: table streaming and per-form error -------------
     single = single_field_bw512(dev, args.single_pairs) if args.single_pairs > 0 else None
 
+    # ---- a grid that is not a power of two ------------------------------------
+    npow2 = non_pow2(dev, args.non_pow2_steps) if args.non_pow2_steps > 0 else None
+
     # ---- roofline of the dominant kernel ----------------------------------
     elem = 16 if prec == S.FP64 else 8  # bytes per complex intermediate value
     dom = max(stages, key=lambda k: stages[k][0])
@@ -506,7 +560,7 @@ def main():
     achieved = stage_bytes(dom, bw, F, elem, args.precision) / avg_s
     roofline = {
         "bound": "hbm",
-        "kernel": KERNELS[args.precision].get(dom, dom),
+        "kernel": (KERNELS[args.precision] if bw & (bw - 1) == 0 else KERNELS_GEN).get(dom, dom),
         "stage": dom,
         "achieved": achieved / 1e9,
         "peak": HBM_PEAK / 1e9,
@@ -540,6 +594,7 @@ def main():
         "fp64_batch1_pairs_per_s": plain,
         "speedup_vs_fp64_batch1": (value / plain) if plain else None,
         "single_field_bw512": single,
+        "non_pow2_bw255": npow2,
         "dropin_host_buffers": dropin,
         "roofline": roofline,
         "cpu_baseline": cpu,
