@@ -55,10 +55,11 @@ typedef struct s2amd_plan s2amd_plan;
 /* Plan for bandwidth bw (2..1024), up to max_fields fields per call,
  * seminaive for orders < cutoff and naive (naive.c) for orders >= cutoff
  * (cutoff = bw: all seminaive).  Generates the forward and transposed tables
- * on the current device.  FP64 and FP32 plans on power-of-two grids with bw
- * in [32, 1024] run both branches as MFMA GEMMs at any cutoff; the mixed and
- * split-BF16 forms use the naive transform for every order whatever the
- * cutoff. */
+ * on the current device.  FP64 plans with bw in [16, 1024] and FP32 plans on
+ * power-of-two grids with bw in [32, 1024] run both branches as MFMA GEMMs at
+ * any cutoff (FP64 grids that are not powers of two on mixed-radix FFTs);
+ * the mixed and split-BF16 forms use the naive transform for every order
+ * whatever the cutoff. */
 int s2amd_plan_create(s2amd_plan** plan, int bw, int cutoff, int max_fields, int precision);
 /* Waits for the device (work queued on any stream may use the plan), then frees. */
 int s2amd_plan_destroy(s2amd_plan* plan);
