@@ -113,6 +113,29 @@ def test_conv_batch_matches_oracle(env, O, prec):
     batch.close()
 
 
+@pytest.mark.parametrize("prec", [0, 2])
+def test_conv_batch_bandwidths_not_powers_of_two(env, O, prec):
+    # groups of 3 and 4 requests at bw 24 and 50 run the FP64 GEMM path on
+    # mixed-radix FFTs (48 = 16*3, 100 = 4*25 rings), the single bw=45 request
+    # the per-field path; mixed batches run these bandwidths in FP64
+    S, torch, dev = env
+    bws = [24, 50, 24, 45, 24, 50, 50, 50]
+    reqs, expect = make_requests(S, torch, dev, O, bws, seed=11 + prec, contiguous_bw=50)
+    batch = S.ConvBatch(reqs, precision=prec)
+    groups = batch.groups()
+    assert [g[0] for g in groups] == [24, 45, 50]
+    assert [g[1] for g in groups] == [3, 1, 4]
+    assert [g[2] for g in groups] == [0, 0, 0]
+    for _ in range(2):  # the recorded graph, twice
+        batch.run()
+        torch.cuda.synchronize()
+        check(reqs, expect, lambda bw: 1e-11)
+        for r in reqs:
+            r[5].zero_()
+        torch.cuda.synchronize()
+    batch.close()
+
+
 def test_conv_batch_argument_errors(env):
     S, torch, dev = env
     t = torch.zeros((8, 8), dtype=torch.float64, device=dev)
