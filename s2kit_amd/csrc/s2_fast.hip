@@ -135,6 +135,30 @@ __host__ __device__ inline GGeo ggeo(int B, bool gen) {
     return g;
 }
 
+#ifndef S2_GEMM_RUN
+#define S2_GEMM_RUN 8
+#endif
+// acc += sum over k in [k0, k1), step 4, of the 16x16x4 MFMA of the LDS
+// operand lds[k + kq] and the table operand glb[(k + kq) * stride], kq =
+// lane/16.  The table values of S steps are loaded before their MFMAs, so S
+// loads share one L2 round trip instead of each MFMA waiting for its own.
+template <typename R, int S = S2_GEMM_RUN>
+__device__ __forceinline__ void mma_run(typename Mma<R>::acc& acc, const R* lds, const R* __restrict__ glb,
+                                        size_t stride, int k0, int k1) {
+    const int kq = (threadIdx.x & 63) >> 4;
+    int k = k0;
+    for (; k + 4 * S <= k1; k += 4 * S) {
+        R g[S], l[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) g[u] = glb[(size_t)(k + 4 * u + kq) * stride];
+#pragma unroll
+        for (int u = 0; u < S; ++u) l[u] = lds[k + 4 * u + kq];
+#pragma unroll
+        for (int u = 0; u < S; ++u) acc = Mma<R>::mma(l[u], g[u], acc);
+    }
+    for (; k < k1; k += 4) acc = Mma<R>::mma(lds[k + kq], glb[(size_t)(k + kq) * stride], acc);
+}
+
 // Orders m of a GEMM launch: grid (column blocks, roundup8(bw)); blocks past
 // the last order return at once.
 __device__ __forceinline__ bool gemm_block(int B, int& bx, int& m) {
@@ -214,10 +238,7 @@ __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __res
             typename Mma<R>::acc acc = {0, 0, 0, 0};
             const R* bcol = Bd + (c0 + (lane & 15)) * rs + p * G.hE;
             const R* arow = AF + g.offF + i0 + (lane & 15);
-            for (int t0 = 0; t0 < kend; t0 += 4) {
-                const int tk = t0 + (lane >> 4);
-                acc = Mma<R>::mma(bcol[tk], arow[(size_t)tk * Li16], acc);
-            }
+            mma_run<R>(acc, bcol, arow, Li16, 0, kend);
             const int i = i0 + (lane & 15);
             if (i >= g.Lp) continue;
 #pragma unroll
@@ -274,10 +295,7 @@ __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typenam
             typename Mma<R>::acc acc = {0, 0, 0, 0};
             const R* bcol = Bd + (c0 + (lane & 15)) * rs;
             const R* arow = AF + g.offF + i0 + (lane & 15);
-            for (int t0 = 0; t0 < B4; t0 += 4) {
-                const int tk = t0 + (lane >> 4);
-                acc = Mma<R>::mma(bcol[tk], arow[(size_t)tk * Li16], acc);
-            }
+            mma_run<R>(acc, bcol, arow, Li16, 0, B4);
             const int i = i0 + (lane & 15);
             if (i >= g.Lp) continue;
 #pragma unroll
@@ -375,10 +393,7 @@ __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* 
             typename Mma<R>::acc acc = {0, 0, 0, 0};
             const R* acol = Cd + (c0 + (lane & 15)) * rs + (p ? Li4_0 : 0);
             const R* brow = AI + g.offI + t0 + (lane & 15);
-            for (int i0 = istart; i0 < Li4; i0 += 4) {
-                const int ik = i0 + (lane >> 4);
-                acc = Mma<R>::mma(acol[ik], brow[(size_t)ik * Tp16], acc);
-            }
+            mma_run<R>(acc, acol, brow, Tp16, istart, Li4);
             const int t = t0 + (lane & 15);
             if (t >= cnt) continue;
 #pragma unroll
@@ -424,15 +439,9 @@ __device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double*
         typename Mma<R>::acc e = {0, 0, 0, 0}, o = {0, 0, 0, 0};
         const R* acol = Cd + (c0 + (lane & 15)) * rs;
         const R* b0 = AI + g0.offI + t0 + (lane & 15);
-        for (int i0 = 0; i0 < Li4_0; i0 += 4) {
-            const int ik = i0 + (lane >> 4);
-            e = Mma<R>::mma(acol[ik], b0[(size_t)ik * B16], e);
-        }
+        mma_run<R>(e, acol, b0, B16, 0, Li4_0);
         const R* b1 = AI + g1.offI + t0 + (lane & 15);
-        for (int i0 = 0; i0 < Li4_1; i0 += 4) {
-            const int ik = i0 + (lane >> 4);
-            o = Mma<R>::mma(acol[Li4_0 + ik], b1[(size_t)ik * B16], o);
-        }
+        mma_run<R>(o, acol + Li4_0, b1, B16, 0, Li4_1);
         const int j = t0 + (lane & 15);
         if (j >= B) continue;
 #pragma unroll
@@ -603,7 +612,7 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
                      double* cim, int F, Fmt fmt, void* Xv, void* Cbv, hipStream_t s, StageTimer* tm) {
     using T = typename Cplx<R>::type;
     constexpr int V = sizeof(R) == 8 ? 8 : 16;
-    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int N = c.N, lgN = lg2(N);
     const int E = V * kThreads, rpb = E / N;
     const size_t lds = sizeof(T) * ((size_t)rpb * (N + 1) + N);
     T* X = (T*)Xv;
@@ -646,7 +655,7 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
                      double* im, int F, Fmt fmt, void* Xv, void* Cbv, hipStream_t s, StageTimer* tm) {
     using T = typename Cplx<R>::type;
     constexpr int V = sizeof(R) == 8 ? 8 : 16;
-    const int B = c.bw, N = c.N, lgN = lg2(N);
+    const int N = c.N, lgN = lg2(N);
     const int E = V * kThreads, rpb = E / N;
     const size_t lds = sizeof(T) * ((size_t)rpb * (N + 1) + N);
     T* X = (T*)Xv;
