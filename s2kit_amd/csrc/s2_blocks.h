@@ -310,32 +310,35 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
     xcd_block(bx, f);
     const int q0 = bx * rpb;
     {
-        R kr[V], ki[V], nr[V], ni[V];
+        // V[k] = conj(ph[k]) * (X[k] - i X[N-k]) with X zero from index B on:
+        // X[k] for k < B, -i X[N-k] for k > B, 0 at k = B -- one value each
+        R cr[V], ci[V];
 #pragma unroll
-        for (int u = 0; u < V; ++u) {  // X[k] for k < B, X[N-k] for 0 < N-k < B (others 0)
+        for (int u = 0; u < V; ++u) {
             const int idx = threadIdx.x + u * NTH;
             const int q = q0 + (idx >> lgN), k = idx & (N - 1);
             const R* row = Cb + ((size_t)f * N + frow_of(min(q, nrows - 1), N, cut)) * 2 * B;
-            const int kk = k < B ? k : 0, kn = (k > 0 && N - k < B) ? N - k : 0;
-            kr[u] = row[cpos(kk, half)];
-            ki[u] = row[B + cpos(kk, half)];
-            nr[u] = row[cpos(kn, half)];
-            ni[u] = row[B + cpos(kn, half)];
+            const int ks = k < B ? k : (k > B ? N - k : 0);
+            cr[u] = row[cpos(ks, half)];
+            ci[u] = row[B + cpos(ks, half)];
         }
 #pragma unroll
         for (int u = 0; u < V; ++u) {
             const int idx = threadIdx.x + u * NTH;
             const int q = q0 + (idx >> lgN), rl = idx >> lgN, k = idx & (N - 1);
-            const bool hk = q < nrows && k < B, hn = q < nrows && k > 0 && N - k < B;
-            const R xr = hk ? kr[u] : (R)0, xi = hk ? ki[u] : (R)0;
-            const R yr = hn ? nr[u] : (R)0, yi = hn ? ni[u] : (R)0;
-            a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(xr + yi, xi - yr));
+            const bool live = q < nrows && k != B;
+            const R tr = live ? (k < B ? cr[u] : ci[u]) : (R)0;
+            const R ti = live ? (k < B ? ci[u] : -cr[u]) : (R)0;
+            a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(tr, ti));
         }
     }
     __syncthreads();
     fft_block<T, +1, V, NTH>(a, lgN, S, twl);
     const R inv_s2pi = (R)0.39894228040143267794;
-#pragma unroll
+    // FP32 (V = 16) unrolled by 4 only: a full unroll keeps every element's
+    // value and store addresses live (162 VGPRs, one block per CU instead of two)
+    constexpr int UNR = sizeof(R) == 4 ? 4 : V;
+#pragma unroll UNR
     for (int u = 0; u < V; ++u) {
         const int idx = threadIdx.x + u * NTH;
         const int rl = idx >> lgN, j = idx & (N - 1);
