@@ -138,25 +138,34 @@ __host__ __device__ inline GGeo ggeo(int B, bool gen) {
 #ifndef S2_GEMM_RUN
 #define S2_GEMM_RUN 8
 #endif
-// acc += sum over k in [k0, k1), step 4, of the 16x16x4 MFMA of the LDS
-// operand lds[k + kq] and the table operand glb[(k + kq) * stride], kq =
-// lane/16.  The table values of S steps are loaded before their MFMAs, so S
-// loads share one L2 round trip instead of each MFMA waiting for its own.
-template <typename R, int S = S2_GEMM_RUN>
-__device__ __forceinline__ void mma_run(typename Mma<R>::acc& acc, const R* lds, const R* __restrict__ glb,
-                                        size_t stride, int k0, int k1) {
+// acc[c] += sum over k in [k0, k1), step 4, of the 16x16x4 MFMA of the LDS
+// operand lds[c*cs + k + kq] (column tile c < CT) and the table operand
+// glb[(k + kq) * stride], kq = lane/16.  Each table value feeds CT MFMAs, and
+// the table values of S steps are loaded before their MFMAs, so S loads share
+// one L2 round trip instead of each MFMA waiting for its own.
+template <typename R, int CT, int S = S2_GEMM_RUN>
+__device__ __forceinline__ void mma_run(typename Mma<R>::acc (&acc)[CT], const R* lds, int cs,
+                                        const R* __restrict__ glb, size_t stride, int k0, int k1) {
     const int kq = (threadIdx.x & 63) >> 4;
     int k = k0;
     for (; k + 4 * S <= k1; k += 4 * S) {
-        R g[S], l[S];
+        R g[S], l[CT][S];
 #pragma unroll
         for (int u = 0; u < S; ++u) g[u] = glb[(size_t)(k + 4 * u + kq) * stride];
 #pragma unroll
-        for (int u = 0; u < S; ++u) l[u] = lds[k + 4 * u + kq];
+        for (int c = 0; c < CT; ++c)
 #pragma unroll
-        for (int u = 0; u < S; ++u) acc = Mma<R>::mma(l[u], g[u], acc);
+            for (int u = 0; u < S; ++u) l[c][u] = lds[c * cs + k + 4 * u + kq];
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[c] = Mma<R>::mma(l[c][u], g[u], acc[c]);
     }
-    for (; k < k1; k += 4) acc = Mma<R>::mma(lds[k + kq], glb[(size_t)(k + kq) * stride], acc);
+    for (; k < k1; k += 4) {
+        const R g = glb[(size_t)(k + kq) * stride];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[c] = Mma<R>::mma(lds[c * cs + k + kq], g, acc[c]);
+    }
 }
 
 // Orders m of a GEMM launch: grid (column blocks, roundup8(bw)); blocks past
@@ -185,7 +194,7 @@ __device__ __forceinline__ void fwd_store(double v, int col, int m, int l, int B
 
 // Seminaive order m < cutoff.  LDS: Bd[c][k2], k2 = parity-split cosine
 // position (p*hE + t).
-template <typename R, bool GEN>
+template <typename R, bool GEN, int CT>
 __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __restrict__ Cb, const R* __restrict__ AF,
                                               const GemmMeta* __restrict__ meta, double* __restrict__ cre,
                                               double* __restrict__ cim, int m, int cbase, const GGeo& G, int F,
@@ -224,28 +233,30 @@ __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __res
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nct = ncb >> 4;
+    const int nctg = (ncb >> 4) / CT;  // groups of CT column tiles
     for (int p = 0; p < 2; ++p) {
         const GemmMeta g = meta[m * 2 + p];
         if (g.Lp == 0) continue;
         const int Li16 = (g.Lp + 15) & ~15;
-        const int ntiles = (Li16 >> 4) * nct;
+        const int ntiles = (Li16 >> 4) * nctg;
         for (int item = wave; item < ntiles; item += kThreads / 64) {
-            const int it = item / nct, ct = item - it * nct;
-            const int i0 = it * 16, c0 = ct * 16;
+            const int it = item / nctg, ct = item - it * nctg;
+            const int i0 = it * 16, c0 = ct * 16 * CT;
             const int ilast = min(i0 + 15, g.Lp - 1);
             const int kend = (row_size(m, m + p + 2 * ilast) + 3) & ~3;
-            typename Mma<R>::acc acc = {0, 0, 0, 0};
+            typename Mma<R>::acc acc[CT] = {};
             const R* bcol = Bd + (c0 + (lane & 15)) * rs + p * G.hE;
             const R* arow = AF + g.offF + i0 + (lane & 15);
-            mma_run<R>(acc, bcol, arow, Li16, 0, kend);
+            mma_run<R, CT>(acc, bcol, 16 * rs, arow, Li16, 0, kend);
             const int i = i0 + (lane & 15);
             if (i >= g.Lp) continue;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int col = cbase + c0 + Mma<R>::row(lane, r);
-                if (col < ncols) fwd_store((double)acc[r], col, m, m + p + 2 * i, B, cpf, fmt, cre, cim);
-            }
+            for (int c = 0; c < CT; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = cbase + c0 + 16 * c + Mma<R>::row(lane, r);
+                    if (col < ncols) fwd_store((double)acc[c][r], col, m, m + p + 2 * i, B, cpf, fmt, cre, cim);
+                }
         }
     }
 }
@@ -255,7 +266,7 @@ __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __res
 // P_l^m(x_j): degree l = m+p+2i is the sum over the bw rings j < bw of
 // (w_j X_j + (-1)^p w_{N-1-j} X_{N-1-j}) P_l^m(x_j).  One parity at a time,
 // LDS: Bd[c][j], zero for j >= bw.
-template <typename R>
+template <typename R, int CT>
 __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typename Cplx<R>::type* __restrict__ X,
                                                const double* __restrict__ wn, const R* __restrict__ AF,
                                                const GemmMeta* __restrict__ meta, double* __restrict__ cre,
@@ -267,7 +278,7 @@ __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typenam
     const int ncols = F * cpf;
     const int rs = G.rs;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nct = ncb >> 4;
+    const int nctg = (ncb >> 4) / CT;
     for (int p = 0; p < 2; ++p) {
         const GemmMeta g = meta[m * 2 + p];
         if (g.Lp == 0) break;
@@ -288,27 +299,29 @@ __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typenam
         }
         __syncthreads();
         const int Li16 = (g.Lp + 15) & ~15;
-        const int ntiles = (Li16 >> 4) * nct;
+        const int ntiles = (Li16 >> 4) * nctg;
         for (int item = wave; item < ntiles; item += kThreads / 64) {
-            const int it = item / nct, ct = item - it * nct;
-            const int i0 = it * 16, c0 = ct * 16;
-            typename Mma<R>::acc acc = {0, 0, 0, 0};
+            const int it = item / nctg, ct = item - it * nctg;
+            const int i0 = it * 16, c0 = ct * 16 * CT;
+            typename Mma<R>::acc acc[CT] = {};
             const R* bcol = Bd + (c0 + (lane & 15)) * rs;
             const R* arow = AF + g.offF + i0 + (lane & 15);
-            mma_run<R>(acc, bcol, arow, Li16, 0, B4);
+            mma_run<R, CT>(acc, bcol, 16 * rs, arow, Li16, 0, B4);
             const int i = i0 + (lane & 15);
             if (i >= g.Lp) continue;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int col = cbase + c0 + Mma<R>::row(lane, r);
-                if (col < ncols) fwd_store((double)acc[r], col, m, m + p + 2 * i, B, cpf, fmt, cre, cim);
-            }
+            for (int c = 0; c < CT; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = cbase + c0 + 16 * c + Mma<R>::row(lane, r);
+                    if (col < ncols) fwd_store((double)acc[c][r], col, m, m + p + 2 * i, B, cpf, fmt, cre, cim);
+                }
         }
     }
 }
 
 // Forward: D'[c][i] = sum_k data[c][k] * A(m,p)[i][k].  grid (ceil(cols/ncb), roundup8(bw)).
-template <typename R, bool GEN>
+template <typename R, bool GEN, int CT>
 __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
                                                        const typename Cplx<R>::type* __restrict__ X,
                                                        const double* __restrict__ wn, const R* __restrict__ AF,
@@ -323,9 +336,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
     if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
     const GGeo G = ggeo<R>(B, GEN);
     if (m < cut)
-        gemm_fwd_semi<R, GEN>(Bd, Cb, AF, meta, cre, cim, m, cbase, G, F, ncb, fmt);
+        gemm_fwd_semi<R, GEN, CT>(Bd, Cb, AF, meta, cre, cim, m, cbase, G, F, ncb, fmt);
     else
-        gemm_fwd_naive<R>(Bd, X, wn, AF, meta, cre, cim, m, cbase, G, F, ncb, fmt);
+        gemm_fwd_naive<R, CT>(Bd, X, wn, AF, meta, cre, cim, m, cbase, G, F, ncb, fmt);
 }
 
 // Coefficient columns of order m into LDS: Cd[c][q], q = i (p = 0) or
@@ -364,7 +377,7 @@ __device__ __forceinline__ void inv_fill(R* __restrict__ Cd, const double* __res
 
 // Seminaive order m < cutoff: D'[c][t] = sum_i coef[c][i] * AI(m,p)[i][t],
 // cosine coefficients k = 2t + p into the Cb row of (field, +-m).
-template <typename R, bool GEN>
+template <typename R, bool GEN, int CT>
 __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* __restrict__ cre,
                                               const double* __restrict__ cim, const R* __restrict__ AI,
                                               const GemmMeta* __restrict__ meta, R* __restrict__ Cb, int m, int cbase,
@@ -378,7 +391,7 @@ __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* 
     inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, G, ncb, cpf, ncols, false);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nct = ncb >> 4;
+    const int nctg = (ncb >> 4) / CT;
     const int Tp16 = G.Tp16;
     const int ntt = Tp16 >> 4;
     for (int p = 0; p < 2; ++p) {
@@ -386,27 +399,29 @@ __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* 
         const int Li4 = p ? Li4_1 : Li4_0;
         const int cnt = p ? G.hO : G.hE;
         const int r0 = g.Lp > 0 ? row_size(m, m + p) : 0;
-        for (int item = wave; item < ntt * nct; item += kThreads / 64) {
-            const int tt = item / nct, ct = item - tt * nct;
-            const int t0 = tt * 16, c0 = ct * 16;
+        for (int item = wave; item < ntt * nctg; item += kThreads / 64) {
+            const int tt = item / nctg, ct = item - tt * nctg;
+            const int t0 = tt * 16, c0 = ct * 16 * CT;
             const int istart = max(0, t0 - r0 + 1) & ~3;
-            typename Mma<R>::acc acc = {0, 0, 0, 0};
+            typename Mma<R>::acc acc[CT] = {};
             const R* acol = Cd + (c0 + (lane & 15)) * rs + (p ? Li4_0 : 0);
             const R* brow = AI + g.offI + t0 + (lane & 15);
-            mma_run<R>(acc, acol, brow, Tp16, istart, Li4);
+            mma_run<R, CT>(acc, acol, 16 * rs, brow, Tp16, istart, Li4);
             const int t = t0 + (lane & 15);
             if (t >= cnt) continue;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int col = cbase + c0 + Mma<R>::row(lane, r);
-                if (col >= ncols) continue;
-                const int f = col / cpf, rem = col - f * cpf;
-                const int row = (rem >> 1) ? N - m : m;
-                if constexpr (GEN)
-                    reinterpret_cast<R*>(Cb)[(((size_t)f * N + row) * B + 2 * t + p) * 2 + (rem & 1)] = acc[r];
-                else
-                    Cb[(((size_t)f * N + row) * 2 + (rem & 1)) * B + p * G.hE + t] = acc[r];
-            }
+            for (int c = 0; c < CT; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = cbase + c0 + 16 * c + Mma<R>::row(lane, r);
+                    if (col >= ncols) continue;
+                    const int f = col / cpf, rem = col - f * cpf;
+                    const int row = (rem >> 1) ? N - m : m;
+                    if constexpr (GEN)
+                        reinterpret_cast<R*>(Cb)[(((size_t)f * N + row) * B + 2 * t + p) * 2 + (rem & 1)] = acc[c][r];
+                    else
+                        Cb[(((size_t)f * N + row) * 2 + (rem & 1)) * B + p * G.hE + t] = acc[c][r];
+                }
         }
     }
 }
@@ -416,7 +431,7 @@ __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* 
 // rows (which carry 1/sqrt(2pi)) give ring j <- E+O and ring N-1-j <- E-O.
 // Row N-m of an odd order carries a minus sign (FST_semi_memo.c:301-348),
 // put on its coefficients; REAL data also fills the conjugate row N-m.
-template <typename R>
+template <typename R, int CT>
 __device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double* __restrict__ cre,
                                                const double* __restrict__ cim, const R* __restrict__ AI,
                                                const GemmMeta* __restrict__ meta, typename Cplx<R>::type* __restrict__ X,
@@ -430,40 +445,42 @@ __device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double*
     inv_fill<R>(Cd, cre, cim, g0, g1, m, cbase, G, ncb, cpf, ncols, true);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nct = ncb >> 4;
+    const int nctg = (ncb >> 4) / CT;
     const int ntt = B16 >> 4;
     const bool mirror = fmt == 1 && m > 0;
-    for (int item = wave; item < ntt * nct; item += kThreads / 64) {
-        const int tt = item / nct, ct = item - tt * nct;
-        const int t0 = tt * 16, c0 = ct * 16;
-        typename Mma<R>::acc e = {0, 0, 0, 0}, o = {0, 0, 0, 0};
+    for (int item = wave; item < ntt * nctg; item += kThreads / 64) {
+        const int tt = item / nctg, ct = item - tt * nctg;
+        const int t0 = tt * 16, c0 = ct * 16 * CT;
+        typename Mma<R>::acc e[CT] = {}, o[CT] = {};
         const R* acol = Cd + (c0 + (lane & 15)) * rs;
         const R* b0 = AI + g0.offI + t0 + (lane & 15);
-        mma_run<R>(e, acol, b0, B16, 0, Li4_0);
+        mma_run<R, CT>(e, acol, 16 * rs, b0, B16, 0, Li4_0);
         const R* b1 = AI + g1.offI + t0 + (lane & 15);
-        mma_run<R>(o, acol + Li4_0, b1, B16, 0, Li4_1);
+        mma_run<R, CT>(o, acol + Li4_0, 16 * rs, b1, B16, 0, Li4_1);
         const int j = t0 + (lane & 15);
         if (j >= B) continue;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int col = cbase + c0 + Mma<R>::row(lane, r);
-            if (col >= ncols) continue;
-            const int f = col / cpf, rem = col - f * cpf, comp = rem & 1;
-            R* xr = reinterpret_cast<R*>(X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N) + comp;
-            const R lo = e[r] + o[r], hi = e[r] - o[r];
-            xr[2 * j] = lo;
-            xr[2 * (N - 1 - j)] = hi;
-            if (mirror) {  // conj into row N-m
-                R* xm = reinterpret_cast<R*>(X + ((size_t)f * N + (N - m)) * N) + comp;
-                xm[2 * j] = comp ? -lo : lo;
-                xm[2 * (N - 1 - j)] = comp ? -hi : hi;
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = cbase + c0 + 16 * c + Mma<R>::row(lane, r);
+                if (col >= ncols) continue;
+                const int f = col / cpf, rem = col - f * cpf, comp = rem & 1;
+                R* xr = reinterpret_cast<R*>(X + ((size_t)f * N + ((rem >> 1) ? N - m : m)) * N) + comp;
+                const R lo = e[c][r] + o[c][r], hi = e[c][r] - o[c][r];
+                xr[2 * j] = lo;
+                xr[2 * (N - 1 - j)] = hi;
+                if (mirror) {  // conj into row N-m
+                    R* xm = reinterpret_cast<R*>(X + ((size_t)f * N + (N - m)) * N) + comp;
+                    xm[2 * j] = comp ? -lo : lo;
+                    xm[2 * (N - 1 - j)] = comp ? -hi : hi;
+                }
             }
-        }
     }
 }
 
 // Inverse.  grid (ceil(cols/ncb), roundup8(bw)).
-template <typename R, bool GEN>
+template <typename R, bool GEN, int CT>
 __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
                                                        const R* __restrict__ AI, const GemmMeta* __restrict__ meta,
                                                        R* __restrict__ Cb, typename Cplx<R>::type* __restrict__ X,
@@ -476,9 +493,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
     if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
     const GGeo G = ggeo<R>(B, GEN);
     if (m < cut)
-        gemm_inv_semi<R, GEN>(Cd, cre, cim, AI, meta, Cb, m, cbase, G, F, ncb, fmt);
+        gemm_inv_semi<R, GEN, CT>(Cd, cre, cim, AI, meta, Cb, m, cbase, G, F, ncb, fmt);
     else
-        gemm_inv_naive<R>(Cd, cre, cim, AI, meta, X, m, cbase, G, F, ncb, fmt);
+        gemm_inv_naive<R, CT>(Cd, cre, cim, AI, meta, X, m, cbase, G, F, ncb, fmt);
 }
 
 // ---------------------------------------------------------------------------
@@ -551,13 +568,20 @@ static int gemm_row(int B, int prec) {
     return prec == 0 ? 8 * ggeo<double>(B, gen).rs : 4 * ggeo<float>(B, gen).rs;
 }
 
-// Columns per GEMM block: the largest multiple of 16 (at most 64) whose
-// padded LDS tile fits in 64 KiB, or failing that in gfx950's 160 KiB.
+// Columns per GEMM block: the widest of 64, 32, 16 whose LDS tile lets four
+// blocks share a CU (40 KiB each of gfx950's 160 KiB; waves then run two
+// column tiles per table load when there are 32 or more), else 16 in up to
+// 160 KiB.  Occupancy outweighs table reuse: at bw=256 FP64, 32 columns in
+// 66 KiB (two blocks per CU) ran the GEMMs 1.2-1.3x slower than 16 in 33 KiB.
 static int gemm_ncb(int B, int prec) {
     const int row = gemm_row(B, prec);
-    const int small = std::min(64, 65536 / row) & ~15;
-    return small >= 16 ? small : (std::min(64, 163840 / row) & ~15);
+    for (int n = 64; n >= 16; n >>= 1)
+        if (n * row <= 40960) return n;
+    return 16 * row <= 163840 ? 16 : 0;
 }
+
+// Column tiles per wave item.
+static int gemm_ct(int ncb) { return ncb >= 32 ? 2 : 1; }
 
 static size_t gemm_lds(int B, int prec) { return (size_t)gemm_ncb(B, prec) * gemm_row(B, prec); }
 
@@ -572,39 +596,47 @@ bool fast_supported(int bw, int cutoff, int prec) {
     return gemm_ncb(bw, prec) >= 16;  // GEMM LDS tile of at least 16 columns
 }
 
+template <typename R, bool GEN, int CT>
+static void fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre,
+                       double* cim, int F, Fmt fmt, int ncb, hipStream_t s) {
+    using T = typename Cplx<R>::type;
+    const int B = c.bw, cols = 4 * F;
+    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
+    launch_lds(k_gemm_fwd<R, GEN, CT>, grid, kThreads, gemm_lds(B, g.prec), s, (const R*)Cb, (const T*)X,
+               (const double*)c.w_naive, (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
+               (int)fmt);
+}
+
+template <typename R, bool GEN, int CT>
+static void inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb,
+                       void* X, int F, Fmt fmt, int ncb, hipStream_t s) {
+    using T = typename Cplx<R>::type;
+    const int B = c.bw, cols = 4 * F;
+    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
+    launch_lds(k_gemm_inv<R, GEN, CT>, grid, kThreads, gemm_lds(B, g.prec), s, cre, cim, (const R*)g.ai,
+               (const GemmMeta*)g.meta, (R*)Cb, (T*)X, B, g.cut, F, ncb, (int)fmt);
+}
+
 void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre, double* cim,
                      int F, Fmt fmt, hipStream_t s) {
-    const int B = c.bw, ncb = gemm_ncb(B, g.prec), cols = 4 * F;
-    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
-    const size_t lds = gemm_lds(B, g.prec);
+    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb) == 2;
     if (g.prec != 0)
-        launch_lds(k_gemm_fwd<float, false>, grid, kThreads, lds, s, (const float*)Cb, (const float2*)X,
-                   (const double*)c.w_naive, (const float*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
-                   (int)fmt);
+        (two ? fwd_launch<float, false, 2> : fwd_launch<float, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
     else if (g.gen)
-        launch_lds(k_gemm_fwd<double, true>, grid, kThreads, lds, s, (const double*)Cb, (const double2*)X,
-                   (const double*)c.w_naive, (const double*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
-                   (int)fmt);
+        (two ? fwd_launch<double, true, 2> : fwd_launch<double, true, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
     else
-        launch_lds(k_gemm_fwd<double, false>, grid, kThreads, lds, s, (const double*)Cb, (const double2*)X,
-                   (const double*)c.w_naive, (const double*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
-                   (int)fmt);
+        (two ? fwd_launch<double, false, 2> : fwd_launch<double, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
 }
 
 void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
                      int F, Fmt fmt, hipStream_t s) {
-    const int B = c.bw, ncb = gemm_ncb(B, g.prec), cols = 4 * F;
-    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
-    const size_t lds = gemm_lds(B, g.prec);
+    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb) == 2;
     if (g.prec != 0)
-        launch_lds(k_gemm_inv<float, false>, grid, kThreads, lds, s, cre, cim, (const float*)g.ai,
-                   (const GemmMeta*)g.meta, (float*)Cb, (float2*)X, B, g.cut, F, ncb, (int)fmt);
+        (two ? inv_launch<float, false, 2> : inv_launch<float, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
     else if (g.gen)
-        launch_lds(k_gemm_inv<double, true>, grid, kThreads, lds, s, cre, cim, (const double*)g.ai,
-                   (const GemmMeta*)g.meta, (double*)Cb, (double2*)X, B, g.cut, F, ncb, (int)fmt);
+        (two ? inv_launch<double, true, 2> : inv_launch<double, true, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
     else
-        launch_lds(k_gemm_inv<double, false>, grid, kThreads, lds, s, cre, cim, (const double*)g.ai,
-                   (const GemmMeta*)g.meta, (double*)Cb, (double2*)X, B, g.cut, F, ncb, (int)fmt);
+        (two ? inv_launch<double, false, 2> : inv_launch<double, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
 }
 
 template <typename R>
