@@ -580,8 +580,9 @@ static int gemm_ncb(int B, int prec) {
     return 16 * row <= 163840 ? 16 : 0;
 }
 
-// Column tiles per wave item.
-static int gemm_ct(int ncb) { return ncb >= 32 ? 2 : 1; }
+// Column tiles per wave item: two from 32 columns, when an order has enough
+// 16-degree tiles (bw >= 64) that pairing still gives all four waves work.
+static int gemm_ct(int ncb, int B) { return ncb >= 32 && B >= 64 ? 2 : 1; }
 
 static size_t gemm_lds(int B, int prec) { return (size_t)gemm_ncb(B, prec) * gemm_row(B, prec); }
 
@@ -619,7 +620,7 @@ static void inv_launch(const DevConsts& c, const GemmTables& g, const double* cr
 
 void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre, double* cim,
                      int F, Fmt fmt, hipStream_t s) {
-    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb) == 2;
+    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb, c.bw) == 2;
     if (g.prec != 0)
         (two ? fwd_launch<float, false, 2> : fwd_launch<float, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
     else if (g.gen)
@@ -630,7 +631,7 @@ void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, co
 
 void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
                      int F, Fmt fmt, hipStream_t s) {
-    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb) == 2;
+    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb, c.bw) == 2;
     if (g.prec != 0)
         (two ? inv_launch<float, false, 2> : inv_launch<float, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
     else if (g.gen)

@@ -154,6 +154,14 @@ __device__ __forceinline__ void mr_pass_odd(const T* a, T* b, int N, int Ns, int
     }
 }
 
+// Threads of the mixed-radix instances: their passes are LDS-latency bound,
+// and two waves per SIMD (256 threads, 64 KiB of LDS per block) leave that
+// latency exposed.
+#ifndef S2_MR_THREADS
+#define S2_MR_THREADS 512
+#endif
+constexpr int kMrThreads = S2_MR_THREADS;
+
 // nfft independent length-N transforms stored back to back in a (b is
 // scratch of the same size).  SIGN = -1: forward, +1: inverse (unnormalised).
 // tw[t] = exp(-2 pi i t / N).  Returns the buffer holding the result.
@@ -220,7 +228,7 @@ __device__ __forceinline__ int row_of(int q, int N, int cut) { return q < cut ? 
 
 // grid (ceil(N/rpb), F): rpb rings of one field per block.
 template <typename T, bool MR>
-__global__ __launch_bounds__(256) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
+__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
                                                  T* __restrict__ X, int N, int logN, int pow2, int rpb,
                                                  const T* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(256) void k_phi_fwd(const double* __restrict__ re, 
 // grid (ceil(nrows/rpb), F).  w_semi holds the even then the odd weights,
 // premultiplied by sqrt(2 pi)/N (FFT scaling) and 1/sqrt(2N) (DCT scaling).
 template <typename T, bool MR>
-__global__ __launch_bounds__(256) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
+__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
                                                  const double* __restrict__ w_semi, const T* __restrict__ tw,
                                                  const T* __restrict__ ph, int N, int logN, int pow2, int B,
                                                  int cut, int nrows, int rpb) {
@@ -753,7 +761,7 @@ __global__ __launch_bounds__(256) void k_leg_inv(const double* __restrict__ cre,
 // x sin(theta) for odd orders, x (-1) on odd negative rows, x 1/sqrt(2pi).
 // REAL mirrors row N-m = conj(row m).  grid (ceil(nrows/rpb), F).
 template <typename T, bool MR>
-__global__ __launch_bounds__(256) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
+__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
                                                  const T* __restrict__ tw, const T* __restrict__ ph,
                                                  const double* __restrict__ sinv, int N, int logN, int pow2, int B,
                                                  int cut, int nrows, int rpb, int fmt) {
@@ -833,7 +841,7 @@ __global__ __launch_bounds__(256) void k_naive_inv(const double* __restrict__ cr
 // Inverse ring FFT (+i, unnormalised) per ring; row B (Nyquist) is zero.
 // grid (ceil(N/rpb), F).
 template <typename T, bool MR>
-__global__ __launch_bounds__(256) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
+__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
                                                  double* __restrict__ im, int N, int logN, int pow2, int B, int rpb,
                                                  const T* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1134,7 +1142,7 @@ static void fft_launch(bool pow2, KP kp, KM km, dim3 grid, size_t lds, hipStream
     if (pow2)
         kp<<<grid, 256, lds, s>>>(args...);
     else
-        km<<<grid, 256, lds, s>>>(args...);
+        km<<<grid, kMrThreads, lds, s>>>(args...);
 }
 
 static int rings_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
