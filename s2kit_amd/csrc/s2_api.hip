@@ -278,7 +278,7 @@ static void run_dropin(const char* fn, F&& body) {
 // ---------------------------------------------------------------------------
 struct Plan {
     int bw, cutoff, max_fields, precision, device;
-    bool fast = false;  // power-of-two grid path with MFMA GEMMs (s2_fast.hip)
+    bool fast = false;  // MFMA GEMM path (s2_fast.hip): FP64 bw >= 16, FP32 power-of-two bw >= 32
     DevConsts c;
     DevTables t;
     GemmTables g;

@@ -1,8 +1,12 @@
-// Batched fast path for power-of-two grids (N = 2*bw = 64 .. 2048), in FP64
-// or FP32 arithmetic (FP64 inputs and outputs either way):
+// Batched GEMM path, FP64 or FP32 arithmetic (FP64 inputs and outputs either
+// way).  Power-of-two grids (N = 2*bw = 64 .. 2048):
 //
 //   k_fphi_fwd -> k_fdct_fwd -> k_gemm_fwd     forward  (ring FFT, DCT: s2_blocks.h)
 //   k_gemm_inv -> k_fdct_inv -> k_fphi_inv     inverse
+//
+// Other bw in [16, 1024] (FP64): the per-field kernels' mixed-radix FFTs and
+// DCTs (s2_kernels.hip) around the same GEMMs, which then read and write the
+// natural-order complex DCT rows (GEN).
 //
 //   k_gemm_fwd   seminaive projections as MFMA GEMMs            (seminaive.c:183-197)
 //                naive projections (orders >= cutoff) likewise  (naive.c:34-59)
@@ -19,8 +23,8 @@
 // each (m,p) is a dense matrix A[i][t] (zero above the staircase).  The
 // stripped tables are repacked once per plan (k_pack_gemm) into
 //   AF(m,p)  [t][i]  (t < roundup4(Kf), i < roundup16(Lp))   forward
-//   AI(m,p)  [i][t]  (i < roundup4(Lp), t < roundup16(bw/2)) inverse, built
-//            from the transposed table with its scaling folded in
+//   AI(m,p)  [i][t]  (i < roundup4(Lp), t < roundup16(ceil(bw/2))) inverse,
+//            built from the transposed table with its scaling folded in
 // and the data columns are (field, +-m, re/im).  Per 16x16 tile the K loop
 // stops at the staircase, so the padded zeros cost at most one tile row.
 #include <algorithm>
