@@ -286,13 +286,15 @@ struct Plan {
     StageTimer timer;
 };
 
-// FP64 plans: fields per call from which the batched GEMM path beats the
-// per-field kernels (measured on MI355X: one table pass per call either way,
-// but the GEMM tiles only pay off once a few fields share them).
+// FP64 plans: the GEMM path from two fields per call, and for one field below
+// bw = 512; single fields at bw >= 512 go to the per-field kernels, whose row
+// GEMVs stream the tables faster than one field's GEMM tiles use them
+// (measured on MI355X, us per fwd+inv call, GEMM vs per-field: bw=256 F=1
+// 75 vs 88; bw=512 F=1 210 vs 205, F=2 266 vs 336; bw=1024 F=1 1389 vs 1239).
 static bool use_fast(const Plan* p, int F) {
     if (!p->fast) return false;
     if (p->precision != S2AMD_FP64) return true;
-    return F >= (p->bw >= 512 ? 6 : p->bw >= 256 ? 2 : 3);
+    return F >= 2 || p->bw < 512;
 }
 
 static void plan_forward(Plan* p, const double* re, const double* im, double* cre, double* cim, int F, Fmt fmt,
