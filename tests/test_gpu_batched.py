@@ -229,3 +229,26 @@ def test_cutoff_plans_agree(env, bw, prec, tol, fmt):
         ec = float((c - c_ref).abs().max() / c_ref.abs().max())
         print(f"cutoff={cut} prec={prec} fmt={fmt}: grid {eg:.3e}, coefficients {ec:.3e}", flush=True)
         assert eg < tol and ec < tol
+
+
+def test_large_bw_not_power_of_two_round_trip(env):
+    # bw=1000: 2000 = 4*4*5*5*5 rings (mixed-radix FFTs and DCTs), 1008-double
+    # LDS rows (GEMM tiles in up to 160 KiB); two fields take the GEMM path.
+    # Inverse then forward is the identity on band-limited coefficients (bw even).
+    S, torch, dev = env
+    from s2kit_amd import synth
+    bw, F = 1000, 2
+    plan = S.Plan(bw, F, precision=0)
+    cre, cim = synth.power_law_coeffs(bw, F, 78, dev)
+    n = 2 * bw
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    plan.inverse(cre, cim, gre, gim)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+    plan.forward(gre, gim, c2r, c2i)
+    torch.cuda.synchronize()
+    b = torch.complex(cre, cim)
+    err = float((torch.complex(c2r, c2i) - b).abs().max() / b.abs().max())
+    print(f"bw={bw} F={F}: round-trip rel err {err:.3e}", flush=True)
+    assert err < 1e-11
+    plan.close()
