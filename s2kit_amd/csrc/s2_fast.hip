@@ -142,29 +142,60 @@ __host__ __device__ inline GGeo ggeo(int B, bool gen) {
 #ifndef S2_GEMM_RUN
 #define S2_GEMM_RUN 8
 #endif
+#ifndef S2_GEMM_PIPE
+#define S2_GEMM_PIPE 1
+#endif
 // acc[c] += sum over k in [k0, k1), step 4, of the 16x16x4 MFMA of the LDS
 // operand lds[c*cs + k + kq] (column tile c < CT) and the table operand
 // glb[(k + kq) * stride], kq = lane/16.  Each table value feeds CT MFMAs, and
-// the table values of S steps are loaded before their MFMAs, so S loads share
-// one L2 round trip instead of each MFMA waiting for its own.
+// the table values of S steps are loaded together, one batch ahead of their
+// MFMAs (two register buffers), so the L2 round trips overlap the MFMAs
+// instead of each MFMA waiting for its own load.
+template <typename R, int CT, int S>
+__device__ __forceinline__ void mma_batch(typename Mma<R>::acc (&acc)[CT], const R* lds, int cs, const R (&g)[S],
+                                          int k, int kq) {
+    R l[CT][S];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int u = 0; u < S; ++u) l[c][u] = lds[c * cs + k + 4 * u + kq];
+#pragma unroll
+    for (int u = 0; u < S; ++u)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[c] = Mma<R>::mma(l[c][u], g[u], acc[c]);
+}
+
+template <typename R, int S>
+__device__ __forceinline__ void table_batch(R (&g)[S], const R* __restrict__ glb, size_t stride, int k, int kq) {
+#pragma unroll
+    for (int u = 0; u < S; ++u) g[u] = glb[(size_t)(k + 4 * u + kq) * stride];
+}
+
 template <typename R, int CT, int S = S2_GEMM_RUN>
 __device__ __forceinline__ void mma_run(typename Mma<R>::acc (&acc)[CT], const R* lds, int cs,
                                         const R* __restrict__ glb, size_t stride, int k0, int k1) {
     const int kq = (threadIdx.x & 63) >> 4;
     int k = k0;
-    for (; k + 4 * S <= k1; k += 4 * S) {
-        R g[S], l[CT][S];
-#pragma unroll
-        for (int u = 0; u < S; ++u) g[u] = glb[(size_t)(k + 4 * u + kq) * stride];
-#pragma unroll
-        for (int c = 0; c < CT; ++c)
-#pragma unroll
-            for (int u = 0; u < S; ++u) l[c][u] = lds[c * cs + k + 4 * u + kq];
-#pragma unroll
-        for (int u = 0; u < S; ++u)
-#pragma unroll
-            for (int c = 0; c < CT; ++c) acc[c] = Mma<R>::mma(l[c][u], g[u], acc[c]);
+#if S2_GEMM_PIPE
+    // two table buffers: batch k + 4S is in flight while batch k's MFMAs run
+    R ga[S], gb[S];
+    if (k + 4 * S <= k1) table_batch<R, S>(ga, glb, stride, k, kq);
+    while (k + 4 * S <= k1) {
+        if (k + 8 * S <= k1) table_batch<R, S>(gb, glb, stride, k + 4 * S, kq);
+        mma_batch<R, CT, S>(acc, lds, cs, ga, k, kq);
+        k += 4 * S;
+        if (k + 4 * S > k1) break;
+        if (k + 8 * S <= k1) table_batch<R, S>(ga, glb, stride, k + 4 * S, kq);
+        mma_batch<R, CT, S>(acc, lds, cs, gb, k, kq);
+        k += 4 * S;
     }
+#else
+    for (; k + 4 * S <= k1; k += 4 * S) {
+        R g[S];
+        table_batch<R, S>(g, glb, stride, k, kq);
+        mma_batch<R, CT, S>(acc, lds, cs, g, k, kq);
+    }
+#endif
     for (; k < k1; k += 4) {
         const R g = glb[(size_t)(k + kq) * stride];
 #pragma unroll
