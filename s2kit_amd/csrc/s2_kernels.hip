@@ -34,7 +34,10 @@ namespace s2 {
 // Division by a divisor 1 <= d < 2^16 of a value x < 2^16: floor(x/d) =
 // (x * ceil(2^32/d)) >> 32 (the rounding error x*(ceil(2^32/d)*d - 2^32)/2^32
 // stays below 1/d).
-__device__ __forceinline__ unsigned long long magic16(unsigned d) { return (0x100000000ull + d - 1) / d; }
+__device__ __forceinline__ unsigned long long magic16(unsigned d) {
+    // ceil(2^32/d) = floor((2^32-1)/d) + 1 for d >= 2 (a 32-bit division)
+    return d == 1 ? 0x100000000ull : (unsigned long long)(0xFFFFFFFFu / d) + 1;
+}
 __device__ __forceinline__ int div16(int x, unsigned long long mg) { return (int)(((unsigned long long)x * mg) >> 32); }
 
 // One Stockham pass of radix R over nfft length-N transforms, a -> b, Ns the
@@ -239,8 +242,14 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double*
     const size_t base = ((size_t)f * N + j0) * N;
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x)
         a[i] = mk<T>((typename Real<T>::type)re[base + i], (typename Real<T>::type)im[base + i]);
+    const T* twp = tw;
+    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = b + rpb * N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+        twp = twl;
+    }
     __syncthreads();
-    T* r = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, tw);
+    T* r = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, twp);
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         const int row = i / nr, jl = i - row * nr;
         X[((size_t)f * N + row) * N + j0 + jl] = r[jl * N + row];
@@ -269,8 +278,14 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __re
         const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
         a[rl * N + p] = x;
     }
+    const T* twp = tw;
+    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = b + rpb * N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+        twp = twl;
+    }
     __syncthreads();
-    T* V = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, tw);
+    T* V = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, twp);
     for (int i = threadIdx.x; i < nr * B; i += blockDim.x) {
         const int rl = i / B, k = i - rl * B;
         const int r = row_of(q0 + rl, N, cut);
@@ -781,8 +796,14 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __re
         const T t = mk<T>(xk.x + xn.y, xk.y - xn.x);
         a[rl * N + k] = cmul(cconj(ph[k]), t);
     }
+    const T* twp = tw;
+    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = b + rpb * N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+        twp = twl;
+    }
     __syncthreads();
-    T* u = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, tw);
+    T* u = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, twp);
     const R inv_s2pi = (R)0.39894228040143267794;  // 1/sqrt(2 pi)
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         const int rl = i / N, j = i - rl * N;
@@ -853,8 +874,14 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_inv(const T* __re
         const int r = i / nr, jl = i - r * nr;
         a[jl * N + r] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + jl];
     }
+    const T* twp = tw;
+    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = b + rpb * N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+        twp = twl;
+    }
     __syncthreads();
-    T* y = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, tw);
+    T* y = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, twp);
     const size_t base = ((size_t)f * N + j0) * N;
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         re[base + i] = (double)y[i].x;
@@ -1137,12 +1164,16 @@ static int ilog2_exact(int n) {
 // Launch the power-of-two (kp) or the mixed-radix (km) instance of a
 // generic FFT kernel: separate instances keep the mixed-radix passes'
 // registers out of the power-of-two kernels.
+// The mixed-radix instance also holds the N twiddles in LDS.
 template <typename KP, typename KM, typename... A>
-static void fft_launch(bool pow2, KP kp, KM km, dim3 grid, size_t lds, hipStream_t s, A... args) {
-    if (pow2)
+static void fft_launch(bool pow2, KP kp, KM km, dim3 grid, size_t lds, int N, hipStream_t s, A... args) {
+    if (pow2) {
         kp<<<grid, 256, lds, s>>>(args...);
-    else
-        km<<<grid, kMrThreads, lds, s>>>(args...);
+    } else {
+        const size_t lm = lds + sizeof(double2) * N;
+        if (lm > 65536) S2_HIP(hipFuncSetAttribute((const void*)km, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lm));
+        km<<<grid, kMrThreads, lm, s>>>(args...);
+    }
 }
 
 static int rings_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
@@ -1349,7 +1380,7 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     {
     Span sp(tm, ST_PHI_FWD, s);
-    fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, s,
+    fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N, s,
                re, im, X, N, c.logN, c.pow2, rpb, c.tw);
     S2_HIP(hipGetLastError());
     }
@@ -1358,7 +1389,7 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
         {
         Span sp(tm, ST_DCT_FWD, s);
         fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+                   lds_fft, N, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
         }
         Span sp(tm, ST_LEG_FWD, s);
@@ -1414,7 +1445,7 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_INV, s);
         fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+                   lds_fft, N, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
@@ -1425,7 +1456,7 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
-    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, s,
+    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N, s,
                X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
     S2_HIP(hipGetLastError());
 }
@@ -1438,7 +1469,7 @@ void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     {
         Span sp(tm, ST_PHI_FWD, s);
-        fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft,
+        fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N,
                    s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
         S2_HIP(hipGetLastError());
     }
@@ -1446,7 +1477,7 @@ void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_FWD, s);
         fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+                   lds_fft, N, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_LEG_FWD, s);
@@ -1467,11 +1498,11 @@ void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cr
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_INV, s);
         fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+                   lds_fft, N, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
-    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, s,
+    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N, s,
                X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
     S2_HIP(hipGetLastError());
 }
@@ -1486,7 +1517,7 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
     S2_HIP(hipGetLastError());
     const int rpb = rings_per_block(N);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
-    fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3(1, F), lds_fft, s, X, Cb, c.w_semi,
+    fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3(1, F), lds_fft, N, s, X, Cb, c.w_semi,
                c.tw, c.ph, N, c.logN, c.pow2, B, t.cutoff, 1, 1);
     S2_HIP(hipGetLastError());
     const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
