@@ -1176,7 +1176,17 @@ static void fft_launch(bool pow2, KP kp, KM km, dim3 grid, size_t lds, int N, hi
     }
 }
 
-static int rings_per_block(int N) { return N >= 2048 ? 1 : 2048 / N; }
+// Rings per block of the per-field FFT/DCT kernels: 2048 values in LDS for
+// power-of-two N; S2_MR_ELEMS for the mixed-radix instances, capped so that
+// both buffers and the twiddles fit gfx950's 160 KiB.
+#ifndef S2_MR_ELEMS
+#define S2_MR_ELEMS 2048
+#endif
+static int rings_per_block(int N, bool pow2) {
+    if (pow2) return N >= 2048 ? 1 : 2048 / N;
+    const int cap = (163840 / (int)sizeof(double2) - N) / (2 * N);
+    return std::max(1, std::min(S2_MR_ELEMS / N, cap));
+}
 
 void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s) {
     c.bw = bw;
@@ -1376,7 +1386,7 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
                   double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
-    const int rpb = rings_per_block(N);
+    const int rpb = rings_per_block(N, c.pow2);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     {
     Span sp(tm, ST_PHI_FWD, s);
@@ -1421,7 +1431,7 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
                   double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
-    const int rpb = rings_per_block(N);
+    const int rpb = rings_per_block(N, c.pow2);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     if (cut > 0) {
         const int head = ((B + 2) * (int)sizeof(int) + 15) / 16 * 16;  // row offsets
@@ -1465,7 +1475,7 @@ void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re
                        double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = g.cut;
-    const int rpb = rings_per_block(N);
+    const int rpb = rings_per_block(N, c.pow2);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     {
         Span sp(tm, ST_PHI_FWD, s);
@@ -1488,7 +1498,7 @@ void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cr
                        double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = g.cut, f = (int)fmt;
-    const int rpb = rings_per_block(N);
+    const int rpb = rings_per_block(N, c.pow2);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     {
         Span sp(tm, ST_LEG_INV, s);
@@ -1515,7 +1525,7 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
     const int N = c.N, B = c.bw;
     k_ring_sum<<<dim3((N + 3) / 4, F), 256, 0, s>>>(re, im, X, N);
     S2_HIP(hipGetLastError());
-    const int rpb = rings_per_block(N);
+    const int rpb = rings_per_block(N, c.pow2);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3(1, F), lds_fft, N, s, X, Cb, c.w_semi,
                c.tw, c.ph, N, c.logN, c.pow2, B, t.cutoff, 1, 1);
