@@ -112,8 +112,8 @@ def table_doubles(bw):
 
 def non_pow2(dev, steps):
     """FP64 plans at bw=255 (510 = 2*3*5*17 rings: mixed-radix FFTs and DCTs,
-    MFMA Legendre GEMMs) and bw=256, 256 fields each; the bw=255 GEMM path
-    is checked against the same plan's per-field path (one field per call)."""
+    MFMA Legendre GEMMs) and bw=256, 256 fields each; the bw=255 batch is
+    checked against the drop-ins (per-field kernels) on its first field."""
     import torch
     import s2kit_amd as S
     from s2kit_amd import synth
@@ -135,16 +135,14 @@ def non_pow2(dev, steps):
             p.forward(gre, gim, cr, ci)
         torch.cuda.synchronize()
         out[f"bw{bw}_pairs_per_s"] = F * steps / (time.perf_counter() - t0)
-        if bw == 255:  # field 0 through the per-field kernels
-            r1, i1 = torch.empty_like(cr[:1]), torch.empty_like(ci[:1])
-            g1r, g1i = torch.empty_like(gre[:1]), torch.empty_like(gim[:1])
-            p.forward(gre[:1], gim[:1], r1, i1, nfields=1)
-            p.inverse(c0r[:1], c0i[:1], g1r, g1i, nfields=1)
-            torch.cuda.synchronize()
-            out["bw255_forward_vs_per_field"] = float((torch.complex(cr[0], ci[0]) - torch.complex(r1[0], i1[0])).abs().max()
-                                                      / torch.complex(r1[0], i1[0]).abs().max())
-            out["bw255_inverse_vs_per_field"] = float((torch.complex(gre[0], gim[0]) - torch.complex(g1r[0], g1i[0])).abs().max()
-                                                      / torch.complex(g1r[0], g1i[0]).abs().max())
+        if bw == 255:  # field 0 through the drop-ins (per-field kernels, host buffers)
+            tabs = S.HostTables(bw)
+            r1, i1 = S.fst(gre[0].cpu().numpy(), gim[0].cpu().numpy(), bw, tabs)
+            g1r, g1i = S.inv_fst(c0r[0].cpu().numpy(), c0i[0].cpu().numpy(), bw, tabs)
+            a = cr[0].cpu().numpy() + 1j * ci[0].cpu().numpy()
+            out["bw255_forward_vs_per_field"] = float(np.abs(a - (r1 + 1j * i1)).max() / np.abs(r1 + 1j * i1).max())
+            g = (gre[0] + 1j * gim[0]).cpu().numpy().ravel()
+            out["bw255_inverse_vs_per_field"] = float(np.abs(g - (g1r + 1j * g1i)).max() / np.abs(g1r + 1j * g1i).max())
         p.close()
         del gre, gim
         torch.cuda.empty_cache()
