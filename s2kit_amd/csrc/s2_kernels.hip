@@ -24,6 +24,7 @@
 #include "s2_complex.h"
 #include "s2_internal.h"
 #include "s2_recurrence.h"
+#include "s2_blocks.h"
 
 namespace s2 {
 
@@ -237,7 +238,9 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double*
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
-    const int f = blockIdx.y, j0 = blockIdx.x * rpb;
+    int bx = blockIdx.x, f = blockIdx.y;
+    if constexpr (MR) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
+    const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
     const size_t base = ((size_t)f * N + j0) * N;
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x)
@@ -267,7 +270,9 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __re
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
-    const int f = blockIdx.y, q0 = blockIdx.x * rpb;
+    int bx = blockIdx.x, f = blockIdx.y;
+    if constexpr (MR) xcd_block(bx, f);
+    const int q0 = bx * rpb;
     const int nr = min(rpb, nrows - q0);
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         const int rl = i / N, j = i - rl * N;
@@ -783,7 +788,9 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __re
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
-    const int f = blockIdx.y, q0 = blockIdx.x * rpb;
+    int bx = blockIdx.x, f = blockIdx.y;
+    if constexpr (MR) xcd_block(bx, f);
+    const int q0 = bx * rpb;
     const int nr = min(rpb, nrows - q0);
     using R = typename Real<T>::type;
     // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]), X[N] = 0, X[k >= B] = 0
@@ -868,7 +875,9 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_inv(const T* __re
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
-    const int f = blockIdx.y, j0 = blockIdx.x * rpb;
+    int bx = blockIdx.x, f = blockIdx.y;
+    if constexpr (MR) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
+    const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
     for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
         const int r = i / nr, jl = i - r * nr;
