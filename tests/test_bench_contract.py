@@ -1,0 +1,39 @@
+"""bench.py pieces that run without a GPU: the one-GPU refusal, the
+algorithmic byte counts behind roofline.achieved, and the HBM traffic read
+from the committed rocprofv3 summaries."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("bw", [64, 256, 255])
+def test_stage_bytes(bw):
+    n, rows = 2 * bw, 2 * bw - 1
+    # mixed form: FP64 complex grids (16 B) in and out, FP32 complex intermediate (8 B)
+    assert bench.stage_bytes("phi_fwd", bw, 1, 8, "mixed") == 16 * n * n + 8 * n * n
+    assert bench.stage_bytes("leg_fwd", bw, 3, 8, "mixed") == 3 * (8 * n * rows + 16 * bw * bw)
+    assert bench.stage_bytes("phi_inv", bw, 1, 8, "mixed") == 8 * n * rows + 16 * n * n
+    # FP64 seminaive form: the DCT rows hold bw complex values
+    assert bench.stage_bytes("dct_fwd", bw, 1, 16, "fp64") == 16 * n * rows + 16 * bw * rows
+    assert bench.stage_bytes("leg_inv", bw, 1, 16, "fp64") == 16 * bw * bw + 16 * bw * rows
+
+
+def test_traffic_from_committed_profiles():
+    # the default workload's dominant stages, within 10% of their algorithmic bytes
+    for stage in ("phi_fwd", "phi_inv"):
+        t = bench.measured_traffic("mixed", 256, 1024, stage)
+        a = bench.stage_bytes(stage, 256, 1024, 8, "mixed")
+        assert t is not None and abs(t / a - 1) < 0.1, (stage, t, a)
+    assert bench.measured_traffic("mixed", 256, 7, "phi_fwd") is None  # no profile for that workload
