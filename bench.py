@@ -152,8 +152,9 @@ def non_pow2(dev, steps):
 
 def single_field_bw512(dev, pairs):
     """One power-law field at bw=512 (1024x1024 grid): the FP64 form called
-    per field streams both FP64 tables from HBM on every pair; the FP32 and
-    mixed forms are compared with it on the same field."""
+    per field streams both FP64 tables from HBM on every pair; the FP32, mixed
+    and split-BF16 forms, and split BF16 with one FP64 residual refinement, are
+    compared with it on the same field."""
     import torch
     import s2kit_amd as S
     from s2kit_amd import synth
@@ -191,6 +192,19 @@ def single_field_bw512(dev, pairs):
                    torch.complex(ref_r, ref_i).abs().max())
         ie = float((torch.complex(g2r, g2i) - torch.complex(gre, gim)).abs().max() / torch.complex(gre, gim).abs().max())
         out["errors_vs_fp64"][name] = {"forward": fe, "inverse": ie}
+        if name == "split":
+            # split BF16 x3 forward with one FP64 residual refinement
+            # (forward_refined): FP64 accuracy, for more than an FP64 forward costs
+            S.forward_refined(p, p64, gre, gim, r, i)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(pairs):
+                S.forward_refined(p, p64, gre, gim, r, i)
+            torch.cuda.synchronize()
+            tr = (time.perf_counter() - t0) / pairs
+            fe = float((torch.complex(r, i) - torch.complex(ref_r, ref_i)).abs().max() /
+                       torch.complex(ref_r, ref_i).abs().max())
+            out["errors_vs_fp64"]["split_fp64_refined"] = {"forward": fe, "forward_ms": tr * 1e3}
         p.close()
     p64.close()
     return out

@@ -371,6 +371,36 @@ class Plan:
                                      self._stream(stream)))
 
 
+def _on(stream):
+    """torch's elementwise work on the plan calls' stream."""
+    import contextlib
+    import torch
+    return torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
+
+
+def forward_refined(lo, p64, re, im, cre, cim, nfields=None, data_format=COMPLEX, stream=None):
+    """Forward transform by plan `lo` (mixed or split BF16) with one FP64
+    residual refinement through plan `p64` (same bw, FP64):
+        c0 = lo.forward(x);  c = c0 + lo.forward(x - p64.inverse(c0)).
+    The residual is ~1e-7 of x, so its low-precision transform adds ~1e-14:
+    FP64 accuracy, at the cost of an FP64 inverse plus two low-precision
+    forwards (more than one FP64 forward).  Only the forward (analysis)
+    admits this: the FP32 errors of a low-precision synthesis lie partly
+    outside the band-limited space, where a coefficient-space residual
+    cannot see them."""
+    import torch
+    with _on(stream):
+        lo.forward(re, im, cre, cim, nfields, data_format, stream)
+        gr, gi = torch.empty_like(re), torch.empty_like(im)
+        p64.inverse(cre, cim, gr, gi, nfields, data_format, stream)
+        gr.neg_().add_(re)
+        gi.neg_().add_(im)
+        dr, di = torch.empty_like(cre), torch.empty_like(cim)
+        lo.forward(gr, gi, dr, di, nfields, data_format, stream)
+        cre.add_(dr)
+        cim.add_(di)
+
+
 class ConvRequest(ctypes.Structure):
     """s2amd_conv_request: one ConvOn2SphereSemiMemo on device grids [2bw, 2bw]."""
     _fields_ = [("bw", ctypes.c_int), ("sre", ctypes.c_void_p), ("sim", ctypes.c_void_p),

@@ -252,3 +252,36 @@ def test_large_bw_not_power_of_two_round_trip(env):
     print(f"bw={bw} F={F}: round-trip rel err {err:.3e}", flush=True)
     assert err < 1e-11
     plan.close()
+
+
+@pytest.mark.parametrize("prec", [2, 3])
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_one_fp64_refinement_reaches_fp64(env, O, prec, fmt):
+    # mixed / split-BF16 forward with one FP64 residual refinement
+    # (s2kit_amd.forward_refined) against the oracle at the FP64 bound, where
+    # the unrefined forward sits at ~3e-7
+    S, torch, dev = env
+    bw, F = 128, 2
+    lo = S.Plan(bw, F, precision=prec)
+    p64 = S.Plan(bw, F, precision=0)
+    kind = 2 if fmt == 1 else 0
+    coeffs = [O.make_coeffs(300 + f, bw, kind) for f in range(F)]
+    cre = to_dev(torch, dev, [c[0] for c in coeffs])
+    cim = to_dev(torch, dev, [c[1] for c in coeffs])
+    n = 2 * bw
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    p64.inverse(cre, cim, gre, gim, data_format=fmt)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+    S.forward_refined(lo, p64, gre, gim, c2r, c2i, data_format=fmt)
+    u2r, u2i = torch.empty_like(cre), torch.empty_like(cim)
+    lo.forward(gre, gim, u2r, u2i, data_format=fmt)
+    torch.cuda.synchronize()
+    for f in range(F):
+        orc, oic = O.fst(gre[f].cpu().numpy().ravel(), gim[f].cpu().numpy().ravel(), bw, data_format=fmt)
+        ef = crel(c2r[f].cpu().numpy(), c2i[f].cpu().numpy(), orc, oic)
+        eu = crel(u2r[f].cpu().numpy(), u2i[f].cpu().numpy(), orc, oic)
+        print(f"prec={prec} fmt={fmt} field {f}: refined forward {ef:.2e}, unrefined {eu:.2e}", flush=True)
+        assert ef < 1e-11 and eu > 1e-9
+    lo.close()
+    p64.close()
