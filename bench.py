@@ -14,6 +14,9 @@ Also reported on the same JSON line:
   roofline    the dominant kernel's algorithmic HBM bytes per launch over its
               average duration (HIP events on the launch stream, recorded
               during the timed steps), against 8 TB/s;
+  mfma        the Legendre GEMMs' useful flops per launch over their average
+              duration, and the MFMA flops issued against the dense peak of
+              their operand type;
   other_forms the FP64 form and the other FP32-accurate form (mixed: FP32
               MFMA GEMMs; split: BF16 MFMA with split operands) timed on the
               same fields, with their errors against FP64; and the FP64 form
@@ -109,6 +112,23 @@ def table_doubles(bw):
             off += hl + 1
         tot += off
     return tot
+
+
+def legendre_flops(bw, nfields, form):
+    """Useful flops of one Legendre GEMM launch (either direction) over
+    nfields fields: 2 flops per table MAC, 4 data columns per order m > 0
+    (re/im x +-m) and 2 at m = 0.  Seminaive forms: TableSize(m) MACs per
+    column (4 (2T - TableSize(0)) per field); naive forms (mixed, split): bw
+    folded rings x (bw - m) degrees per column (4 bw^3 per field)."""
+    if form in ("mixed", "split"):
+        return 4 * bw ** 3 * nfields
+    ts0 = bw // 2 * (bw // 2 + 1) + (bw // 2 + 1 if bw & 1 else 0)  # TableSize(0, bw)
+    return 4 * (2 * table_doubles(bw) - ts0) * nfields
+
+
+# dense MFMA peaks of MI355X (TFLOP/s) by the GEMM's operand type, and the MFMA
+# flops issued per useful flop (split BF16: six piece products)
+MFMA_PEAK = {"fp64": (78.6, 1), "fp32": (157.3, 1), "mixed": (157.3, 1), "split": (2500.0, 6)}
 
 
 def non_pow2(dev, steps):
@@ -585,6 +605,17 @@ def main():
         "stages_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
     }
 
+    # ---- MFMA utilisation of the Legendre GEMMs ---------------------------
+    peak, issue = MFMA_PEAK[args.precision]
+    kmap = KERNELS[args.precision] if bw & (bw - 1) == 0 else KERNELS_GEN
+    mfma = {}
+    for st_ in ("leg_fwd", "leg_inv"):
+        if st_ in stages:
+            ms, cnt = stages[st_]
+            useful = legendre_flops(bw, F, args.precision) / (ms / cnt / 1e3) / 1e12
+            mfma[st_] = {"kernel": kmap.get(st_, st_), "useful_tflops": useful, "issued_tflops": useful * issue,
+                         "peak_tflops": peak, "frac": useful * issue / peak}
+
     out = {
         "metric": "fwd+inv SHT pairs/s, bw=256 x1024 batch: mixed vs FP64, rel err<=1e-6",
         "value": value,
@@ -610,6 +641,7 @@ def main():
         "non_pow2_bw255": npow2,
         "dropin_host_buffers": dropin,
         "roofline": roofline,
+        "mfma": mfma,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -37,3 +37,12 @@ def test_traffic_from_committed_profiles():
         a = bench.stage_bytes(stage, 256, 1024, 8, "mixed")
         assert t is not None and abs(t / a - 1) < 0.1, (stage, t, a)
     assert bench.measured_traffic("mixed", 256, 7, "phi_fwd") is None  # no profile for that workload
+
+
+@pytest.mark.parametrize("bw,pair_flops", [(8, 1504), (64, 707328), (128, 5625344), (256, 44870656),
+                                           (512, 358438912)])
+def test_legendre_flops_per_pair(bw, pair_flops):
+    # seminaive Legendre flops per fwd+inv pair: 8 (2 T(B) - TableSize(0, B)),
+    # tabulated from the TableSize formula (cospml.c:39-59) independently
+    assert 2 * bench.legendre_flops(bw, 1, "fp64") == pair_flops
+    assert bench.legendre_flops(bw, 3, "mixed") == 3 * 4 * bw ** 3
