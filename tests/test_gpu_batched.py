@@ -285,3 +285,48 @@ def test_one_fp64_refinement_reaches_fp64(env, O, prec, fmt):
         assert ef < 1e-11 and eu > 1e-9
     lo.close()
     p64.close()
+
+
+@pytest.mark.parametrize("bw,prec", [(256, 2), (64, 0), (100, 0)])
+def test_plan_calls_replay_from_a_hip_graph(env, bw, prec):
+    # forward + inverse recorded once in a HIP graph (torch.cuda.CUDAGraph) and
+    # replayed three times, the inputs rescaled eagerly between replays; every
+    # replay matches the eager calls on the same inputs
+    S, torch, dev = env
+    from s2kit_amd import synth
+    F, n = 4, 2 * bw
+    plan = S.Plan(bw, F, precision=prec)
+    cre, cim = synth.power_law_coeffs(bw, F, 17, dev)
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    plan.inverse(cre, cim, gre, gim)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+    ore, oim = torch.empty_like(gre), torch.empty_like(gim)
+
+    def step():
+        plan.forward(gre, gim, c2r, c2i)
+        plan.inverse(c2r, c2i, ore, oim)
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step()  # warm-up outside the capture (tables, LDS attributes)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    for k in range(3):
+        gre.mul_(1.5)
+        gim.mul_(-0.5)
+        ore.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        gr_c, gi_c = c2r.clone(), c2i.clone()
+        go_r, go_i = ore.clone(), oim.clone()
+        step()  # eager on the same inputs
+        torch.cuda.synchronize()
+        assert torch.equal(gr_c, c2r) and torch.equal(gi_c, c2i), k
+        assert torch.equal(go_r, ore) and torch.equal(go_i, oim), k
+    plan.close()
