@@ -63,6 +63,17 @@ typedef struct s2amd_plan s2amd_plan;
 int s2amd_plan_create(s2amd_plan** plan, int bw, int cutoff, int max_fields, int precision);
 /* Waits for the device (work queued on any stream may use the plan), then frees. */
 int s2amd_plan_destroy(s2amd_plan* plan);
+/* Autotuning (no counterpart in the reference): times forward+inverse of
+ * nfields fields on stream for each tile shape the plan's GEMMs accept
+ * (FP64/FP32: data columns per block and column tiles per table load; mixed:
+ * 64-ring groups per inverse block) and keeps the fastest for later calls.
+ * Every shape computes the same sums in the same order: results do not
+ * change.  Allocates and frees temporary device fields; synchronises the
+ * stream.  Plans whose calls of nfields fields take no GEMM (or split-BF16
+ * plans) keep their defaults.  s2amd_plan_tuning reports the choice (0: the
+ * built-in heuristic). */
+int s2amd_plan_autotune(s2amd_plan* plan, int nfields, void* stream);
+int s2amd_plan_tuning(const s2amd_plan* plan, int* cols, int* col_tiles, int* ring_groups);
 /* Device bytes held by the plan (tables + scratch). */
 size_t s2amd_plan_bytes(const s2amd_plan* plan);
 /* Device pointers to the plan's tables (layout of SemiNaive_Naive_Pml_Table /
@@ -106,6 +117,8 @@ typedef struct {
 } s2amd_conv_request;
 
 typedef struct s2amd_conv_batch s2amd_conv_batch;
+/* Groups the requests by bandwidth, creates and autotunes (s2amd_plan_autotune)
+ * one plan per group, and records the whole batch as one HIP graph. */
 int s2amd_conv_batch_create(s2amd_conv_batch** batch, const s2amd_conv_request* req, int n, int precision,
                             void* stream);
 int s2amd_conv_batch_run(s2amd_conv_batch* batch, void* stream);

@@ -81,6 +81,8 @@ SIGNATURES = {
                                    ctypes.POINTER(_i)]),
     "s2amd_conv_batch_destroy": (_i, [_vp]),
     "s2amd_plan_timing": (_i, [_vp, _i]),
+    "s2amd_plan_autotune": (_i, [_vp, _i, _vp]),
+    "s2amd_plan_tuning": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "s2amd_plan_stage_times": (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long)]),
     "s2amd_stage_name": (ctypes.c_char_p, [_i]),
     "s2amd_set_abort_on_error": (None, [_i]),
@@ -328,6 +330,16 @@ class Plan:
 
     def nbytes(self):
         return lib().s2amd_plan_bytes(self.h)
+
+    def autotune(self, nfields, stream=None):
+        """Time the GEMM tile shapes at nfields fields and keep the fastest."""
+        self._check(lib().s2amd_plan_autotune(self.h, nfields, self._stream(stream)))
+
+    def tuning(self):
+        """(columns per block, column tiles per table load, ring groups); 0: heuristic."""
+        v = [ctypes.c_int() for _ in range(3)]
+        self._check(lib().s2amd_plan_tuning(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     @staticmethod
     def _stream(stream):

@@ -981,6 +981,94 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
     });
 }
 
+// Time plan_forward + plan_inverse of nfields zero fields on stream s:
+// best of two runs after a warm-up, in ms.
+static float time_pair(Plan* p, int F, hipStream_t s, double* re, double* im, double* cr, double* ci) {
+    hipEvent_t e0, e1;
+    S2_HIP(hipEventCreate(&e0));
+    S2_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+        S2_HIP(hipEventRecord(e0, s));
+        plan_forward(p, re, im, cr, ci, F, Fmt::Complex, s);
+        plan_inverse(p, cr, ci, re, im, F, Fmt::Complex, s);
+        S2_HIP(hipEventRecord(e1, s));
+        S2_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        S2_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0) best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+extern "C" int s2amd_plan_autotune(s2amd_plan* p, int nfields, void* stream) {
+    return run_plan("s2amd_plan_autotune", [&] {
+        if (!p) throw ArgError{"plan is NULL"};
+        if (nfields < 1 || nfields > p->max_fields) throw ArgError{"nfields must be in [1, max_fields]"};
+        S2_HIP(hipSetDevice(p->device));
+        const hipStream_t s = (hipStream_t)stream;
+        const bool gemm = (p->precision == S2AMD_FP64 || p->precision == S2AMD_FP32) && use_fast(p, nfields);
+        const bool mixed = p->precision == S2AMD_MIXED;
+        if (!gemm && !mixed) return;  // nothing to choose for this plan and field count
+        const size_t N = 2 * (size_t)p->bw, F = nfields;
+        struct Scratch {  // temporary fields, freed on every exit
+            DBuf re, im, cr, ci;
+            ~Scratch() {
+                for (DBuf* b : {&re, &im, &cr, &ci}) b->release();
+            }
+        } t;
+        struct Restore {  // the previous choice stays if a trial fails
+            GemmTables& g;
+            int ncb, ct, gj;
+            bool done = false;
+            ~Restore() {
+                if (!done) g.t_ncb = ncb, g.t_ct = ct, g.t_gj = gj;
+            }
+        } keep{p->g, p->g.t_ncb, p->g.t_ct, p->g.t_gj};
+        t.re.reserve(sizeof(double) * N * N * F);
+        t.im.reserve(sizeof(double) * N * N * F);
+        t.cr.reserve(sizeof(double) * p->bw * p->bw * F);
+        t.ci.reserve(sizeof(double) * p->bw * p->bw * F);
+        for (DBuf* b : {&t.re, &t.im, &t.cr, &t.ci}) S2_HIP(hipMemsetAsync(b->p, 0, b->n, s));
+        auto run = [&] {
+            return time_pair(p, nfields, s, t.re.as<double>(), t.im.as<double>(), t.cr.as<double>(), t.ci.as<double>());
+        };
+        float best = 1e30f;
+        if (gemm) {
+            int bn = 0, bc = 0;
+            for (const auto& tc : gemm_tile_choices(p->bw, p->precision)) {
+                p->g.t_ncb = tc.first;
+                p->g.t_ct = tc.second;
+                const float ms = run();
+                if (ms < best) best = ms, bn = tc.first, bc = tc.second;
+            }
+            p->g.t_ncb = bn;
+            p->g.t_ct = bc;
+        } else {
+            int bg = 0;
+            for (int gj : mix_gj_choices(p->bw)) {
+                p->g.t_gj = gj;
+                const float ms = run();
+                if (ms < best) best = ms, bg = gj;
+            }
+            p->g.t_gj = bg;
+        }
+        S2_HIP(hipStreamSynchronize(s));
+        keep.done = true;
+    });
+}
+
+extern "C" int s2amd_plan_tuning(const s2amd_plan* p, int* cols, int* col_tiles, int* ring_groups) {
+    return run_plan("s2amd_plan_tuning", [&] {
+        if (!p) throw ArgError{"plan is NULL"};
+        if (cols) *cols = p->g.t_ncb;
+        if (col_tiles) *col_tiles = p->g.t_ct;
+        if (ring_groups) *ring_groups = p->g.t_gj;
+    });
+}
+
 extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
     return run_plan("s2amd_plan_destroy", [&] {
         if (!p) return;

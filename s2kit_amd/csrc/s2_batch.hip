@@ -132,6 +132,8 @@ extern "C" int s2amd_conv_batch_create(s2amd_conv_batch** out, const s2amd_conv_
             g.n = (int)idx.size();
             g.precision = (precision != S2AMD_FP64 && mixed_ok(g.bw)) ? precision : S2AMD_FP64;
             check_plan(s2amd_plan_create(&g.plan, g.bw, g.bw, g.n, g.precision));
+            // tile shapes timed at this group's size before the graph fixes them
+            check_plan(s2amd_plan_autotune(g.plan, g.n, b->stream));
             const long nn = 4L * g.bw * g.bw;
             const size_t bytes = sizeof(double) * nn * g.n;
             auto ptr_in = [&](int a, int i) -> const double* {

@@ -619,7 +619,6 @@ static int gemm_ncb(int B, int prec) {
 // 16-degree tiles (bw >= 64) that pairing still gives all four waves work.
 static int gemm_ct(int ncb, int B) { return ncb >= 32 && B >= 64 ? 2 : 1; }
 
-static size_t gemm_lds(int B, int prec) { return (size_t)gemm_ncb(B, prec) * gemm_row(B, prec); }
 
 bool gemm_pow2(int bw) { return bw >= 32 && bw <= 1024 && !(bw & (bw - 1)); }
 
@@ -638,7 +637,7 @@ static void fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, 
     using T = typename Cplx<R>::type;
     const int B = c.bw, cols = 4 * F;
     const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
-    launch_lds(k_gemm_fwd<R, GEN, CT>, grid, kThreads, gemm_lds(B, g.prec), s, (const R*)Cb, (const T*)X,
+    launch_lds(k_gemm_fwd<R, GEN, CT>, grid, kThreads, (size_t)ncb * gemm_row(B, g.prec), s, (const R*)Cb, (const T*)X,
                (const double*)c.w_naive, (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
                (int)fmt);
 }
@@ -649,13 +648,32 @@ static void inv_launch(const DevConsts& c, const GemmTables& g, const double* cr
     using T = typename Cplx<R>::type;
     const int B = c.bw, cols = 4 * F;
     const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
-    launch_lds(k_gemm_inv<R, GEN, CT>, grid, kThreads, gemm_lds(B, g.prec), s, cre, cim, (const R*)g.ai,
+    launch_lds(k_gemm_inv<R, GEN, CT>, grid, kThreads, (size_t)ncb * gemm_row(B, g.prec), s, cre, cim, (const R*)g.ai,
                (const GemmMeta*)g.meta, (R*)Cb, (T*)X, B, g.cut, F, ncb, (int)fmt);
+}
+
+std::vector<std::pair<int, int>> gemm_tile_choices(int bw, int prec) {
+    const int row = gemm_row(bw, prec), n0 = gemm_ncb(bw, prec);
+    std::vector<std::pair<int, int>> v{{n0, gemm_ct(n0, bw)}};
+    for (int n = 16; n <= 64; n <<= 1) {
+        if (n * row > 163840) continue;
+        for (int ct = 1; ct <= (n >= 32 ? 2 : 1); ++ct)
+            if (n != v[0].first || ct != v[0].second) v.push_back({n, ct});
+    }
+    return v;
+}
+
+// The plan's tuned tile shape, or the heuristic's.
+static void gemm_tiles(const DevConsts& c, const GemmTables& g, int& ncb, bool& two) {
+    ncb = g.t_ncb ? g.t_ncb : gemm_ncb(c.bw, g.prec);
+    two = (g.t_ncb ? g.t_ct : gemm_ct(ncb, c.bw)) == 2;
 }
 
 void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre, double* cim,
                      int F, Fmt fmt, hipStream_t s) {
-    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb, c.bw) == 2;
+    int ncb;
+    bool two;
+    gemm_tiles(c, g, ncb, two);
     if (g.prec != 0)
         (two ? fwd_launch<float, false, 2> : fwd_launch<float, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
     else if (g.gen)
@@ -666,7 +684,9 @@ void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, co
 
 void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
                      int F, Fmt fmt, hipStream_t s) {
-    const int ncb = gemm_ncb(c.bw, g.prec), two = gemm_ct(ncb, c.bw) == 2;
+    int ncb;
+    bool two;
+    gemm_tiles(c, g, ncb, two);
     if (g.prec != 0)
         (two ? inv_launch<float, false, 2> : inv_launch<float, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
     else if (g.gen)

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <exception>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "s2_layout.h"
@@ -91,6 +92,9 @@ struct GemmTables {
     int bw = 0, prec = 0;  // prec 0: FP64, 1: FP32 (seminaive); 2: FP32 naive (mixed form); 3: split BF16 naive
     int cut = 0;           // prec 0, 1: orders m >= cut are naive (sampled P_l^m tables)
     bool gen = false;      // prec 0, bw not a power of two: natural-order Cb rows of the per-field path
+    // tile choices from s2amd_plan_autotune (0: the built-in heuristic)
+    int t_ncb = 0, t_ct = 0;  // prec 0, 1: data columns per GEMM block, column tiles per table load
+    int t_gj = 0;             // prec 2: 64-ring groups per inverse GEMM block
     void* af = nullptr;
     void* ai = nullptr;
     GemmMeta* meta = nullptr;
@@ -196,6 +200,11 @@ void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, co
                      int F, Fmt fmt, hipStream_t s);
 void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
                      int F, Fmt fmt, hipStream_t s);
+// Tile shapes the GEMMs accept at bw: (columns per block, column tiles per
+// table load), the heuristic's choice first.
+std::vector<std::pair<int, int>> gemm_tile_choices(int bw, int prec);
+// 64-ring groups per mixed inverse GEMM block that bw admits, the default first.
+std::vector<int> mix_gj_choices(int bw);
 void gemm_tables_create(GemmTables& g, const DevTables& t, int prec, hipStream_t s);
 void gemm_tables_destroy(GemmTables& g);
 size_t fast_x_bytes(int bw, int prec);   // scratch X per field

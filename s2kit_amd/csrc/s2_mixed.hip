@@ -377,6 +377,13 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+std::vector<int> mix_gj_choices(int bw) {
+    std::vector<int> v{std::min(S2_MIX_GJMAX, bw / 64)};
+    for (int gj = 1; gj <= 4 && gj <= bw / 64; gj <<= 1)
+        if (gj != v[0]) v.push_back(gj);
+    return v;
+}
+
 bool mixed_supported(int bw) {
     const int N = 2 * bw;
     return bw >= 64 && bw <= 512 && !(N & (N - 1));
@@ -444,7 +451,7 @@ static void mix_fwd_gemm(const DevConsts& c, const GemmTables& g, const float2* 
 static void mix_inv_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, float2* X,
                          int F, Fmt fmt, hipStream_t s) {
     const int B = c.bw;
-    const int gj = std::min(S2_MIX_GJMAX, B / 64);
+    const int gj = g.t_gj ? g.t_gj : std::min(S2_MIX_GJMAX, B / 64);
     const int nb = (4 * F + kMixCols - 1) / kMixCols;
     const dim3 grid(B / 2 * nb * (B / (64 * gj)));  // order pairs x column blocks x ring ranges
     const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * 64 * gj);

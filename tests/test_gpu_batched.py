@@ -330,3 +330,40 @@ def test_plan_calls_replay_from_a_hip_graph(env, bw, prec):
         assert torch.equal(gr_c, c2r) and torch.equal(gi_c, c2i), k
         assert torch.equal(go_r, ore) and torch.equal(go_i, oim), k
     plan.close()
+
+
+@pytest.mark.parametrize("bw,prec", [(256, 0), (128, 1), (256, 2), (100, 0), (64, 3)])
+def test_autotune_keeps_results(env, bw, prec):
+    # s2amd_plan_autotune times the tile shapes and keeps the fastest; every
+    # shape computes the same sums in the same order, so the results are
+    # bit-identical before and after
+    S, torch, dev = env
+    from s2kit_amd import synth
+    F, n = 8, 2 * bw
+    plan = S.Plan(bw, F, precision=prec)
+    cre, cim = synth.power_law_coeffs(bw, F, 23, dev)
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+
+    def pair():
+        plan.inverse(cre, cim, gre, gim)
+        plan.forward(gre, gim, c2r, c2i)
+        torch.cuda.synchronize()
+        return gre.clone(), gim.clone(), c2r.clone(), c2i.clone()
+
+    before = pair()
+    assert plan.tuning() == (0, 0, 0)
+    plan.autotune(F)
+    cols, tiles, rings = plan.tuning()
+    print(f"bw={bw} prec={prec}: tuned cols={cols} col_tiles={tiles} ring_groups={rings}", flush=True)
+    if prec in (0, 1):
+        assert cols in (16, 32, 64) and tiles in (1, 2) and rings == 0
+    elif prec == 2:
+        assert cols == 0 and rings in (1, 2, 4)
+    else:
+        assert (cols, tiles, rings) == (0, 0, 0)  # split BF16: nothing to choose
+    after = pair()
+    for a, b in zip(before, after):
+        assert torch.equal(a, b)
+    plan.close()
