@@ -145,12 +145,7 @@ __host__ __device__ inline GGeo ggeo(int B, bool gen) {
 #ifndef S2_GEMM_PIPE
 #define S2_GEMM_PIPE 1
 #endif
-// acc[c] += sum over k in [k0, k1), step 4, of the 16x16x4 MFMA of the LDS
-// operand lds[c*cs + k + kq] (column tile c < CT) and the table operand
-// glb[(k + kq) * stride], kq = lane/16.  Each table value feeds CT MFMAs, and
-// the table values of S steps are loaded together, one batch ahead of their
-// MFMAs (two register buffers), so the L2 round trips overlap the MFMAs
-// instead of each MFMA waiting for its own load.
+// S K steps of MFMAs for CT column tiles on table values g already loaded.
 template <typename R, int CT, int S>
 __device__ __forceinline__ void mma_batch(typename Mma<R>::acc (&acc)[CT], const R* lds, int cs, const R (&g)[S],
                                           int k, int kq) {
@@ -165,12 +160,20 @@ __device__ __forceinline__ void mma_batch(typename Mma<R>::acc (&acc)[CT], const
         for (int c = 0; c < CT; ++c) acc[c] = Mma<R>::mma(l[c][u], g[u], acc[c]);
 }
 
+// The table values of S K steps (one per lane and step).
 template <typename R, int S>
 __device__ __forceinline__ void table_batch(R (&g)[S], const R* __restrict__ glb, size_t stride, int k, int kq) {
 #pragma unroll
     for (int u = 0; u < S; ++u) g[u] = glb[(size_t)(k + 4 * u + kq) * stride];
 }
 
+// acc[c] += sum over k in [k0, k1), step 4, of the 16x16x4 MFMA of the LDS
+// operand lds[c*cs + k + kq] (column tile c < CT) and the table operand
+// glb[(k + kq) * stride], kq = lane/16.  Each table value feeds CT MFMAs, and
+// the table values of S steps are loaded together, one batch ahead of their
+// MFMAs (two register buffers), so the L2 round trips overlap the MFMAs
+// instead of each MFMA waiting for its own load.  The K order of every
+// accumulator is ascending whatever S, CT or the column blocking.
 template <typename R, int CT, int S = S2_GEMM_RUN>
 __device__ __forceinline__ void mma_run(typename Mma<R>::acc (&acc)[CT], const R* lds, int cs,
                                         const R* __restrict__ glb, size_t stride, int k0, int k1) {
