@@ -143,6 +143,32 @@ __device__ __forceinline__ void fft_block(T* a, int lgN, int S, const T* tw) {
 
 __device__ __forceinline__ int frow_of(int q, int N, int cut) { return q < cut ? q : N - 2 * cut + 1 + q; }
 
+// Streaming accesses of the ring FFTs, non-temporal with NT (no cache
+// retention for data read or written once per kernel).  Measured at bw=256
+// x1024 (mixed form): in the forward FFT the grid loads and the transposed
+// stores both gain (k_fphi_fwd 1.19 -> ~1.08 ms) when its row segments are
+// whole 128-byte lines; with 64-byte segments (bw=512 FP64, 4 rings x 16 B)
+// it lost 5-13%, and in the inverse FFT either kind lost (1.19 -> 1.23 ms).
+// k_fphi_fwd_nt() says which forward instance to launch.
+inline bool k_fphi_fwd_nt(int rings_per_block, int value_bytes) { return rings_per_block * value_bytes >= 128; }
+
+template <bool NT, typename E>
+__device__ __forceinline__ E ld_stream(const E* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream_c(T* p, T v) {  // complex
+    if constexpr (NT) {
+        using E = typename Real<T>::type;
+        typedef E v2 __attribute__((ext_vector_type(2)));
+        const v2 w = {v.x, v.y};
+        __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+    } else {
+        *p = v;
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8), and
 // each XCD has its own L2.  Renumber blocks so that every XCD works through a
 // contiguous range of logical blocks (x fastest): neighbours that share cache
@@ -186,7 +212,7 @@ __device__ __forceinline__ T* stage_twiddles(unsigned char* smem, const T* __res
 // FFT-type kernels.  grid.y = field, block = NTH (default kThreads) threads,
 // LDS = N twiddles + rows*(N+1) values
 // ---------------------------------------------------------------------------
-template <typename R, int V, int NTH = kThreads>
+template <typename R, int V, int NTH = kThreads, bool NTF = false>
 __global__ __launch_bounds__(NTH) void k_fphi_fwd(const double* __restrict__ re, const double* __restrict__ im,
                                                        typename Cplx<R>::type* __restrict__ X, int lgN,
                                                        const typename Cplx<R>::type* __restrict__ tw) {
@@ -205,8 +231,8 @@ __global__ __launch_bounds__(NTH) void k_fphi_fwd(const double* __restrict__ re,
         double vr[V], vi[V];
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-            vr[u] = re[base + threadIdx.x + u * NTH];
-            vi[u] = im[base + threadIdx.x + u * NTH];
+            vr[u] = ld_stream<NTF>(re + base + threadIdx.x + u * NTH);
+            vi[u] = ld_stream<NTF>(im + base + threadIdx.x + u * NTH);
         }
 #pragma unroll
         for (int u = 0; u < V; ++u) {
@@ -226,7 +252,7 @@ __global__ __launch_bounds__(NTH) void k_fphi_fwd(const double* __restrict__ re,
 #pragma unroll
         for (int u = 0; u < V; ++u) {
             const int idx = threadIdx.x + u * NTH;
-            X[((size_t)f * N + (idx >> lgrpb)) * N + j0 + (idx & (rpb - 1))] = v[u];
+            st_stream_c<NTF>(X + ((size_t)f * N + (idx >> lgrpb)) * N + j0 + (idx & (rpb - 1)), v[u]);
         }
     }
 }

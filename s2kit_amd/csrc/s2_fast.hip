@@ -716,8 +716,11 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
         // 512-thread blocks (twice the rings, full 128-byte row segments) when a block still holds whole rows
         if ((long)N * N >= 512L * V) {
             const int rp2 = V * 512 / N;
-            launch_lds(k_fphi_fwd<R, V, 512>, dim3(N / rp2, F), 512, sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, re,
-                       im, X, lgN, tw);
+            const size_t l2 = sizeof(T) * ((size_t)rp2 * (N + 1) + N);
+            if (k_fphi_fwd_nt(rp2, sizeof(T)))
+                launch_lds(k_fphi_fwd<R, V, 512, true>, dim3(N / rp2, F), 512, l2, s, re, im, X, lgN, tw);
+            else
+                launch_lds(k_fphi_fwd<R, V, 512>, dim3(N / rp2, F), 512, l2, s, re, im, X, lgN, tw);
         } else {
             launch_lds(k_fphi_fwd<R, V>, dim3(N / rpb, F), kThreads, lds, s, re, im, X, lgN, tw);
         }

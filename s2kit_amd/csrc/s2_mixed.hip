@@ -472,9 +472,13 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
     float2* X = (float2*)Xv;
     {
         Span sp(tm, ST_PHI_FWD, s);
-        launch_lds(k_fphi_fwd<float, V, kMixFftThreads>, dim3(N / rpb, F), kMixFftThreads,
-                   sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
-                   re, im, X, lgN, (const float2*)c.tw32);
+        const size_t lds = sizeof(float2) * ((size_t)rpb * (N + 1) + N);
+        if (k_fphi_fwd_nt(rpb, sizeof(float2)))
+            launch_lds(k_fphi_fwd<float, V, kMixFftThreads, true>, dim3(N / rpb, F), kMixFftThreads, lds, s, re, im,
+                       X, lgN, (const float2*)c.tw32);
+        else
+            launch_lds(k_fphi_fwd<float, V, kMixFftThreads>, dim3(N / rpb, F), kMixFftThreads, lds, s, re, im, X,
+                       lgN, (const float2*)c.tw32);
     }
     {
         Span sp(tm, ST_LEG_FWD, s);
