@@ -165,6 +165,10 @@ __device__ __forceinline__ void mr_pass_odd(const T* a, T* b, int N, int Ns, int
 #define S2_MR_THREADS 512
 #endif
 constexpr int kMrThreads = S2_MR_THREADS;
+// Loads per thread that the load phases of the per-field FFT/DCT kernels keep
+// in flight before storing to LDS (4 covers a 2048-value block at 512
+// threads in one batch).
+constexpr int kFillBatch = 4;
 
 // nfft independent length-N transforms stored back to back in a (b is
 // scratch of the same size).  SIGN = -1: forward, +1: inverse (unnormalised).
@@ -243,8 +247,24 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double*
     const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
     const size_t base = ((size_t)f * N + j0) * N;
-    for (int i = threadIdx.x; i < nr * N; i += blockDim.x)
-        a[i] = mk<T>((typename Real<T>::type)re[base + i], (typename Real<T>::type)im[base + i]);
+    const int tot = nr * N;  // <= 2048
+    using F = typename Real<T>::type;
+    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
+        double vr[kFillBatch], vi[kFillBatch];
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) {
+                vr[u] = re[base + i];
+                vi[u] = im[base + i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) a[i] = mk<T>((F)vr[u], (F)vi[u]);
+        }
+    }
     const T* twp = tw;
     if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
         T* twl = b + rpb * N;
@@ -253,8 +273,9 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double*
     }
     __syncthreads();
     T* r = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, twp);
-    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
-        const int row = i / nr, jl = i - row * nr;
+    const unsigned long long mnr = magic16(nr);
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int row = div16(i, mnr), jl = i - row * nr;
         X[((size_t)f * N + row) * N + j0 + jl] = r[jl * N + row];
     }
 }
@@ -274,14 +295,31 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __re
     if constexpr (MR) xcd_block(bx, f);
     const int q0 = bx * rpb;
     const int nr = min(rpb, nrows - q0);
-    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
-        const int rl = i / N, j = i - rl * N;
-        const int r = row_of(q0 + rl, N, cut);
-        const int am = r < B ? r : N - r;
-        const typename Real<T>::type w = (typename Real<T>::type)w_semi[(am & 1) * N + j];
-        const T x = cscale(X[((size_t)f * N + r) * N + j], w);
-        const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
-        a[rl * N + p] = x;
+    const int tot = nr * N;  // <= 2048
+    const unsigned long long mN = magic16(N);
+    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
+        T xv[kFillBatch];
+        double wv[kFillBatch];
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) {
+                const int rl = div16(i, mN), j = i - rl * N;
+                const int r = row_of(q0 + rl, N, cut);
+                const int am = r < B ? r : N - r;
+                wv[u] = w_semi[(am & 1) * N + j];
+                xv[u] = X[((size_t)f * N + r) * N + j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) {
+                const int rl = div16(i, mN), j = i - rl * N;
+                const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+                a[rl * N + p] = cscale(xv[u], (typename Real<T>::type)wv[u]);
+            }
+        }
     }
     const T* twp = tw;
     if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
@@ -291,8 +329,9 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __re
     }
     __syncthreads();
     T* V = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, twp);
+    const unsigned long long mB = magic16(B);
     for (int i = threadIdx.x; i < nr * B; i += blockDim.x) {
-        const int rl = i / B, k = i - rl * B;
+        const int rl = div16(i, mB), k = i - rl * B;
         const int r = row_of(q0 + rl, N, cut);
         const T* v = V + rl * N;
         const T e = ph[k];
@@ -793,15 +832,32 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __re
     const int q0 = bx * rpb;
     const int nr = min(rpb, nrows - q0);
     using R = typename Real<T>::type;
-    // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]), X[N] = 0, X[k >= B] = 0
-    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
-        const int rl = i / N, k = i - rl * N;
-        const int r = row_of(q0 + rl, N, cut);
-        const T* crow = Cb + ((size_t)f * N + r) * B;
-        const T xk = (k < B) ? crow[k] : mk<T>(0, 0);
-        const T xn = (k > 0 && N - k < B) ? crow[N - k] : mk<T>(0, 0);
-        const T t = mk<T>(xk.x + xn.y, xk.y - xn.x);
-        a[rl * N + k] = cmul(cconj(ph[k]), t);
+    // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]), X[N] = 0, X[k >= B] = 0: one
+    // of the two terms is nonzero, X[k] for k < B, X[N-k] for k > B.
+    const int tot = nr * N;  // <= 2048
+    const unsigned long long mN = magic16(N);
+    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
+        T xv[kFillBatch], e[kFillBatch];
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) {
+                const int rl = div16(i, mN), k = i - rl * N;
+                const int r = row_of(q0 + rl, N, cut);
+                const T* crow = Cb + ((size_t)f * N + r) * B;
+                xv[u] = (k != B) ? crow[k < B ? k : N - k] : mk<T>(0, 0);
+                e[u] = ph[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) {
+                const int k = i - div16(i, mN) * N;
+                const T x = xv[u];
+                a[i] = cmul(cconj(e[u]), (k <= B) ? x : mk<T>(x.y, -x.x));  // k = B: x is zero
+            }
+        }
     }
     const T* twp = tw;
     if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
@@ -812,8 +868,8 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __re
     __syncthreads();
     T* u = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, twp);
     const R inv_s2pi = (R)0.39894228040143267794;  // 1/sqrt(2 pi)
-    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
-        const int rl = i / N, j = i - rl * N;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int rl = div16(i, mN), j = i - rl * N;
         const int r = row_of(q0 + rl, N, cut);
         const int am = r < B ? r : N - r;
         const int n = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
@@ -879,9 +935,26 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_inv(const T* __re
     if constexpr (MR) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
     const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
-    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
-        const int r = i / nr, jl = i - r * nr;
-        a[jl * N + r] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + jl];
+    const int tot = nr * N;  // <= 2048
+    const unsigned long long mnr = magic16(nr);
+    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
+        T v[kFillBatch];
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) {
+                const int r = div16(i, mnr), jl = i - r * nr;
+                v[u] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + jl];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < tot) {
+                const int r = div16(i, mnr), jl = i - r * nr;
+                a[jl * N + r] = v[u];
+            }
+        }
     }
     const T* twp = tw;
     if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
@@ -892,7 +965,7 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_inv(const T* __re
     __syncthreads();
     T* y = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, twp);
     const size_t base = ((size_t)f * N + j0) * N;
-    for (int i = threadIdx.x; i < nr * N; i += blockDim.x) {
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         re[base + i] = (double)y[i].x;
         im[base + i] = (double)y[i].y;
     }
@@ -1192,6 +1265,7 @@ static void fft_launch(bool pow2, KP kp, KM km, dim3 grid, size_t lds, int N, hi
 #define S2_MR_ELEMS 2048
 #endif
 static int rings_per_block(int N, bool pow2) {
+    if (N > 2048) throw ArgError{"ring length above 2048"};  // block loops index below 2^16
     if (pow2) return N >= 2048 ? 1 : 2048 / N;
     const int cap = (163840 / (int)sizeof(double2) - N) / (2 * N);
     return std::max(1, std::min(S2_MR_ELEMS / N, cap));
