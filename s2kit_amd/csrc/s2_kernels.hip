@@ -1465,6 +1465,14 @@ static void rows_launch(int B, int nf, L&& launch, K f1a, K f2a, K f4a, K f1b, K
     else launch(nf == 1 ? f1c : f2c);  // bw > 512: at most 2 fields fit the LDS
 }
 
+// Power-of-two N in [64, 2048]: the per-field pipeline's ring FFTs run on
+// the register kernels of the batched path (radix-8 passes in registers,
+// 256-thread blocks of 2048 values, the same order-major X), which beat the
+// radix-2 LDS passes at the same block count.
+static bool reg_rings(const DevConsts& c) { return c.pow2 && c.N >= 64 && c.N <= 2048; }
+static int reg_rpb(int N) { return 8 * kThreads / N; }
+static size_t reg_lds(int N) { return sizeof(double2) * ((size_t)reg_rpb(N) * (N + 1) + N); }
+
 void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* cre,
                   double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
@@ -1473,8 +1481,11 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
     {
     Span sp(tm, ST_PHI_FWD, s);
-    fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N, s,
-               re, im, X, N, c.logN, c.pow2, rpb, c.tw);
+    if (reg_rings(c))
+        launch_lds(k_fphi_fwd<double, 8>, dim3(N / reg_rpb(N), F), kThreads, reg_lds(N), s, re, im, X, c.logN, c.tw);
+    else
+        fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft,
+                   N, s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
     S2_HIP(hipGetLastError());
     }
     if (cut > 0) {
@@ -1549,8 +1560,12 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
-    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N, s,
-               X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
+    if (reg_rings(c))
+        launch_lds(k_fphi_inv<double, 8>, dim3(N / reg_rpb(N), F), kThreads, reg_lds(N), s, (const double2*)X, re, im,
+                   c.logN, c.tw);
+    else
+        fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft,
+                   N, s, X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
     S2_HIP(hipGetLastError());
 }
 
