@@ -613,7 +613,8 @@ __device__ __forceinline__ double sum16(double v) {
 template <int NF, int MAXI>
 __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict__ Cb, const double* __restrict__ T,
                                                       const long* __restrict__ toff, double* __restrict__ cre,
-                                                      double* __restrict__ cim, int N, int B, int F, int fmt, int rpb) {
+                                                      double* __restrict__ cim, int N, int B, int F, int fmt, int rpb,
+                                                      int planar) {
     extern __shared__ __align__(16) unsigned char smem[];
     double2* cs = reinterpret_cast<double2*>(smem);
     const int m = blockIdx.y, f0 = blockIdx.z * NF;
@@ -624,7 +625,14 @@ __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict_
     const int nsign = (m > 0 && fmt == 0) ? 2 : 1;
     for (int i = threadIdx.x; i < nf * nsign * B; i += blockDim.x) {
         const int k = i % B, rest = i / B, s = rest % nsign, fl = rest / nsign;
-        cs[i] = Cb[((size_t)(f0 + fl) * N + (s ? N - m : m)) * B + k];
+        const size_t row = ((size_t)(f0 + fl) * N + (s ? N - m : m)) * B;
+        if (planar) {  // re then im plane, even then odd k (the register DCT's rows)
+            const double* pr = reinterpret_cast<const double*>(Cb + row);
+            const int pos = cpos(k, B >> 1);
+            cs[i] = make_double2(pr[pos], pr[B + pos]);
+        } else {
+            cs[i] = Cb[row + k];
+        }
     }
     __syncthreads();
     const double* tm = T + toff[m];
@@ -696,7 +704,7 @@ template <int NF, int MAXI>
 __global__ __launch_bounds__(256) void k_leg_inv_rows(const double* __restrict__ cre, const double* __restrict__ cim,
                                                       const double* __restrict__ TT, const long* __restrict__ toff,
                                                       const int* __restrict__ trow, double2* __restrict__ Cb, int N,
-                                                      int B, int F, int fmt, int rpb) {
+                                                      int B, int F, int fmt, int rpb, int planar) {
     extern __shared__ __align__(16) unsigned char smem[];
     double2* co = reinterpret_cast<double2*>(smem);
     const int m = blockIdx.y, f0 = blockIdx.z * NF;
@@ -762,8 +770,17 @@ __global__ __launch_bounds__(256) void k_leg_inv_rows(const double* __restrict__
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (f < nf && s < nsign)
-                    Cb[((size_t)(f0 + f) * N + (s ? N - m : m)) * B + i] = make_double2(vr[f][s], vi[f][s]);
+                if (f < nf && s < nsign) {
+                    const size_t row = ((size_t)(f0 + f) * N + (s ? N - m : m)) * B;
+                    if (planar) {  // re then im plane, even then odd index (the register DCT's rows)
+                        double* pr = reinterpret_cast<double*>(Cb + row);
+                        const int pos = cpos(i, B >> 1);
+                        pr[pos] = vr[f][s];
+                        pr[B + pos] = vi[f][s];
+                    } else {
+                        Cb[row + i] = make_double2(vr[f][s], vi[f][s]);
+                    }
+                }
     }
 }
 
@@ -1490,10 +1507,16 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
     }
     if (cut > 0) {
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
+        // the row GEMVs also read the register DCT's planar rows
+        const int planar = (reg_rings(c) && F <= kRowsMaxFields) ? 1 : 0;
         {
         Span sp(tm, ST_DCT_FWD, s);
-        fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, N, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+        if (planar)
+            launch_lds(k_fdct_fwd<double, 8>, dim3((nrows + reg_rpb(N) - 1) / reg_rpb(N), F), kThreads, reg_lds(N), s,
+                       (const double2*)X, (double*)Cb, c.w_semi, c.logN, cut, nrows, c.tw, c.ph);
+        else
+            fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
+                       lds_fft, N, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
         }
         Span sp(tm, ST_LEG_FWD, s);
@@ -1502,7 +1525,7 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
             const int rpb = rows_per_block(B);
             const dim3 grid((B + rpb - 1) / rpb, cut, (F + nf - 1) / nf);
             const size_t lds = sizeof(double2) * 2 * B * nf;
-            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb); },
+            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb, planar); },
                         k_leg_fwd_rows<1, 5>, k_leg_fwd_rows<2, 5>, k_leg_fwd_rows<4, 5>, k_leg_fwd_rows<1, 9>,
                         k_leg_fwd_rows<2, 9>, k_leg_fwd_rows<4, 9>, k_leg_fwd_rows<1, 17>, k_leg_fwd_rows<2, 17>);
         } else {
@@ -1527,6 +1550,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
     const int rpb = rings_per_block(N, c.pow2);
     const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    // the register DCT reads the row GEMVs' planar rows
+    const int planar = (reg_rings(c) && F <= kRowsMaxFields) ? 1 : 0;
     if (cut > 0) {
         const int head = ((B + 2) * (int)sizeof(int) + 15) / 16 * 16;  // row offsets
         const int fc = std::max(1, (65536 - head) / (2 * B * (int)sizeof(double2)));
@@ -1538,7 +1563,7 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
             const int rpb = rows_per_block(B);
             const dim3 grid((B + rpb - 1) / rpb, cut, (F + nf - 1) / nf);
             const size_t lr = sizeof(double2) * 2 * B * nf;
-            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lr, s>>>(cre, cim, t.inv, t.off, c.trow, Cb, N, B, F, f, rpb); },
+            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lr, s>>>(cre, cim, t.inv, t.off, c.trow, Cb, N, B, F, f, rpb, planar); },
                         k_leg_inv_rows<1, 5>, k_leg_inv_rows<2, 5>, k_leg_inv_rows<4, 5>, k_leg_inv_rows<1, 9>,
                         k_leg_inv_rows<2, 9>, k_leg_inv_rows<4, 9>, k_leg_inv_rows<1, 17>, k_leg_inv_rows<2, 17>);
         } else {
@@ -1548,8 +1573,12 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         }
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_INV, s);
-        fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, N, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+        if (planar)
+            launch_lds(k_fdct_inv<double, 8>, dim3((nrows + reg_rpb(N) - 1) / reg_rpb(N), F), kThreads, reg_lds(N), s,
+                       (const double*)Cb, X, c.sinv, c.logN, cut, nrows, f, c.tw, c.ph);
+        else
+            fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
+                       lds_fft, N, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
