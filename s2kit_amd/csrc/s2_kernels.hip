@@ -19,6 +19,7 @@
 // so one complex row carries the real and imaginary transforms together.
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "s2_complex.h"
@@ -579,20 +580,21 @@ void inv_dlt_naive_one(const double* coeffs, const double* pml, double* out, int
 // ---- projections for a few fields: the tables streamed as coalesced GEMVs ----
 // With one to four fields a call reads every table entry once and uses it
 // for at most 16 products, so the projections are bound by the table stream.
-// 16 lanes take one table row (128 contiguous bytes per load instead of one
-// row per lane; every load of a row issued at once), a block takes rpb rows
-// of one order, and the order's
-// data (DCT output, or coefficients) sits in LDS.
+// LPR = 16 or 32 lanes take one table row (16 or 32 contiguous 16-byte
+// loads instead of one row per lane; every load of a row issued at once), a
+// block takes rpb rows of one order, and the order's data (DCT output, or
+// coefficients) sits in LDS.
 
 typedef double dpair8 __attribute__((ext_vector_type(2))) __attribute__((aligned(8)));
 
-// A table row of rs entries, 16 lanes: lane j holds entries 32it + 2j and
-// 32it + 2j + 1 (zero past the row), all MAXI loads issued at once.
-template <int MAXI>
+// A table row of rs entries over LPR lanes: lane j holds entries
+// 2*LPR*it + 2j and 2*LPR*it + 2j + 1 (zero past the row), all MAXI loads
+// issued at once.
+template <int MAXI, int LPR>
 __device__ __forceinline__ void row_load(const double* __restrict__ tp, int rs, int j, dpair8 (&tv)[MAXI]) {
 #pragma unroll
     for (int it = 0; it < MAXI; ++it) {
-        const int t = 32 * it + 2 * j;
+        const int t = 2 * LPR * it + 2 * j;
         if (t + 1 < rs)
             tv[it] = *reinterpret_cast<const dpair8*>(tp + t);
         else if (t < rs)
@@ -602,15 +604,17 @@ __device__ __forceinline__ void row_load(const double* __restrict__ tp, int rs, 
     }
 }
 
-__device__ __forceinline__ double sum16(double v) {
+// Sum over the LPR lanes (16 or 32) that share a table row.
+template <int LPR>
+__device__ __forceinline__ double sum_row(double v) {
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+    for (int o = LPR / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPR);
     return v;
 }
 
 // Forward (DLTSemi's projections, seminaive.c:183-197) for orders m < cutoff:
 // row l of order m against the order's cosine coefficients Cb[f][+-m][k].
-template <int NF, int MAXI>
+template <int NF, int MAXI, int LPR>
 __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict__ Cb, const double* __restrict__ T,
                                                       const long* __restrict__ toff, double* __restrict__ cre,
                                                       double* __restrict__ cim, int N, int B, int F, int fmt, int rpb,
@@ -636,11 +640,11 @@ __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict_
     }
     __syncthreads();
     const double* tm = T + toff[m];
-    const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+    const int g = threadIdx.x / LPR, j = threadIdx.x % LPR;
     const size_t B2 = (size_t)B * B;
     const double sg = (m & 1) ? -1.0 : 1.0;
     const int lend = min(L, l0 + rpb);
-    for (int li = l0 + g; li < lend; li += 16) {
+    for (int li = l0 + g; li < lend; li += 256 / LPR) {
         const int l = m + li;
         const int rs = row_size(m, l);
         const double* tp = tm + table_offset(m, l);
@@ -651,12 +655,12 @@ __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict_
 #pragma unroll
             for (int s = 0; s < 2; ++s) ar[f][s] = ai[f][s] = 0.0;
         dpair8 tv[MAXI];
-        row_load<MAXI>(tp, rs, j, tv);
+        row_load<MAXI, LPR>(tp, rs, j, tv);
 #pragma unroll
         for (int it = 0; it < MAXI; ++it)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                const int t = 32 * it + 2 * j + e;
+                const int t = 2 * LPR * it + 2 * j + e;
                 if (t < rs) {
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
@@ -673,8 +677,8 @@ __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict_
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                ar[f][s] = sum16(ar[f][s]);
-                ai[f][s] = sum16(ai[f][s]);
+                ar[f][s] = sum_row<LPR>(ar[f][s]);
+                ai[f][s] = sum_row<LPR>(ai[f][s]);
             }
         if (j != 0) continue;
 #pragma unroll
@@ -700,7 +704,7 @@ __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict_
 // Inverse (InvDLTSemi's transposed projections, seminaive.c:74-95) for orders
 // m < cutoff: row i of order m's transposed table against the coefficients
 // m+first, m+first+2, ...; trow[m][i] is where row i starts in the order.
-template <int NF, int MAXI>
+template <int NF, int MAXI, int LPR>
 __global__ __launch_bounds__(256) void k_leg_inv_rows(const double* __restrict__ cre, const double* __restrict__ cim,
                                                       const double* __restrict__ TT, const long* __restrict__ toff,
                                                       const int* __restrict__ trow, double2* __restrict__ Cb, int N,
@@ -724,9 +728,9 @@ __global__ __launch_bounds__(256) void k_leg_inv_rows(const double* __restrict__
     const int* rp = trow + (size_t)m * (B + 1);
     const double scale = 0.5 / sqrt((double)B);
     const double scale0 = 1.0 / sqrt((double)N);
-    const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+    const int g = threadIdx.x / LPR, j = threadIdx.x % LPR;
     const int iend = min(B, i0 + rpb);
-    for (int i = i0 + g; i < iend; i += 16) {
+    for (int i = i0 + g; i < iend; i += 256 / LPR) {
         double vr[NF][2], vi[NF][2];
 #pragma unroll
         for (int f = 0; f < NF; ++f)
@@ -738,12 +742,12 @@ __global__ __launch_bounds__(256) void k_leg_inv_rows(const double* __restrict__
             const double* tp = tt + r0;
             const double2* a = co + first;
             dpair8 tv[MAXI];
-            row_load<MAXI>(tp, rs, j, tv);
+            row_load<MAXI, LPR>(tp, rs, j, tv);
 #pragma unroll
             for (int it = 0; it < MAXI; ++it)
 #pragma unroll
                 for (int e = 0; e < 2; ++e) {
-                    const int k = 32 * it + 2 * j + e;
+                    const int k = 2 * LPR * it + 2 * j + e;
                     if (k < rs) {
 #pragma unroll
                         for (int f = 0; f < NF; ++f)
@@ -761,8 +765,8 @@ __global__ __launch_bounds__(256) void k_leg_inv_rows(const double* __restrict__
             for (int f = 0; f < NF; ++f)
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
-                    vr[f][s] = sum16(vr[f][s]) * sc;
-                    vi[f][s] = sum16(vi[f][s]) * sc;
+                    vr[f][s] = sum_row<LPR>(vr[f][s]) * sc;
+                    vi[f][s] = sum_row<LPR>(vi[f][s]) * sc;
                 }
         }
         if (j != 0) continue;
@@ -1474,12 +1478,30 @@ static int rows_fields(int F, int B) {
 static int rows_per_block(int B) { return std::max(64, B / 2); }
 
 // Row loads per lane: a row holds at most bw/2 + 1 entries (32 per load round).
-template <typename L, typename K>
-static void rows_launch(int B, int nf, L&& launch, K f1a, K f2a, K f4a, K f1b, K f2b, K f4b, K f1c, K f2c) {
-    const int maxi = (B / 2 + 1 + 31) / 32;
-    if (maxi <= 5) launch(nf == 1 ? f1a : nf == 2 ? f2a : f4a);
-    else if (maxi <= 9) launch(nf == 1 ? f1b : nf == 2 ? f2b : f4b);
-    else launch(nf == 1 ? f1c : f2c);  // bw > 512: at most 2 fields fit the LDS
+// Row-GEMV instance for bw and nf fields: 16 lanes per row, a lane holding up
+// to 5, 9 or 17 pair loads of the longest row (bw/2 + 1 entries).  With
+// `wide`, bandwidths above 542 take 32 lanes per row instead (9 pairs per
+// lane: 112 VGPRs and 4 waves per SIMD against 190 and 2).  Measured at
+// bw=1024, one field: forward 455 -> 407 us with 32 lanes, inverse 444 ->
+// 497 us (and both slower with 32 lanes at bw=512), so only the forward
+// passes `wide`.  pick(NF, MAXI, LPR) returns the kernel.
+template <typename L, typename P>
+static void rows_launch(int B, int nf, bool wide, L&& launch, P&& pick) {
+    using std::integral_constant;
+    auto fields = [&](auto mx, auto lp) {
+        if (nf == 1) launch(pick(integral_constant<int, 1>{}, mx, lp));
+        else if (nf == 2) launch(pick(integral_constant<int, 2>{}, mx, lp));
+        else launch(pick(integral_constant<int, 4>{}, mx, lp));
+    };
+    const int row = B / 2 + 1;
+    if (B <= 542 || !wide) {
+        if ((row + 31) / 32 <= 5) fields(integral_constant<int, 5>{}, integral_constant<int, 16>{});
+        else if ((row + 31) / 32 <= 9) fields(integral_constant<int, 9>{}, integral_constant<int, 16>{});
+        else fields(integral_constant<int, 17>{}, integral_constant<int, 16>{});
+    } else {
+        if ((row + 63) / 64 <= 5) fields(integral_constant<int, 5>{}, integral_constant<int, 32>{});
+        else fields(integral_constant<int, 9>{}, integral_constant<int, 32>{});
+    }
 }
 
 // Power-of-two N in [64, 2048]: the per-field pipeline's ring FFTs run on
@@ -1525,9 +1547,10 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
             const int rpb = rows_per_block(B);
             const dim3 grid((B + rpb - 1) / rpb, cut, (F + nf - 1) / nf);
             const size_t lds = sizeof(double2) * 2 * B * nf;
-            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb, planar); },
-                        k_leg_fwd_rows<1, 5>, k_leg_fwd_rows<2, 5>, k_leg_fwd_rows<4, 5>, k_leg_fwd_rows<1, 9>,
-                        k_leg_fwd_rows<2, 9>, k_leg_fwd_rows<4, 9>, k_leg_fwd_rows<1, 17>, k_leg_fwd_rows<2, 17>);
+            rows_launch(B, nf, true, [&](auto kern) { kern<<<grid, 256, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb, planar); },
+                        [](auto nf_, auto mx, auto lp) {
+                            return k_leg_fwd_rows<decltype(nf_)::value, decltype(mx)::value, decltype(lp)::value>;
+                        });
         } else {
             const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
             k_leg_fwd<<<dim3(cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
@@ -1563,9 +1586,10 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
             const int rpb = rows_per_block(B);
             const dim3 grid((B + rpb - 1) / rpb, cut, (F + nf - 1) / nf);
             const size_t lr = sizeof(double2) * 2 * B * nf;
-            rows_launch(B, nf, [&](auto kern) { kern<<<grid, 256, lr, s>>>(cre, cim, t.inv, t.off, c.trow, Cb, N, B, F, f, rpb, planar); },
-                        k_leg_inv_rows<1, 5>, k_leg_inv_rows<2, 5>, k_leg_inv_rows<4, 5>, k_leg_inv_rows<1, 9>,
-                        k_leg_inv_rows<2, 9>, k_leg_inv_rows<4, 9>, k_leg_inv_rows<1, 17>, k_leg_inv_rows<2, 17>);
+            rows_launch(B, nf, false, [&](auto kern) { kern<<<grid, 256, lr, s>>>(cre, cim, t.inv, t.off, c.trow, Cb, N, B, F, f, rpb, planar); },
+                        [](auto nf_, auto mx, auto lp) {
+                            return k_leg_inv_rows<decltype(nf_)::value, decltype(mx)::value, decltype(lp)::value>;
+                        });
         } else {
             k_leg_inv<<<dim3(cut, (F + fc - 1) / fc), 256, lds, s>>>(cre, cim, t.inv, t.off, Cb, N, B, F, fc, f);
         }
