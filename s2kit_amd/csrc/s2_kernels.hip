@@ -113,40 +113,61 @@ __device__ __forceinline__ void mr_pass(const T* a, T* b, int N, int Ns, int nff
     }
 }
 
-// The same pass for an odd prime radix R >= 5 known only at run time.  Work
-// item (butterfly j, s), s <= H = (R-1)/2, writes output p = 0 (s = 0) or
-// the pair p = s, R-s, which share their inputs:
+// The same pass for an odd prime radix R >= 5 known only at run time.
+// Work item (butterfly j, s), s <= H = (R-1)/2, writes output p = 0 (s = 0)
+// or the pair p = s, R-s, which share their inputs:
 //   y_s, y_{R-s} = v_0 + sum_{q=1..H} (v_q + v_{R-q}) Re(w^{sq}) +- i (v_q - v_{R-q}) Im(w^{sq}),
-// v_q = a[j + q*N/R] * wk^q (wk = exp(SIGN 2 pi i k / (Ns*R)), wk^{R-q} =
-// wk^R conj(wk^q)), w = exp(SIGN 2 pi i / R); powers by repeated
+// v_q = a[j + q*N/R] exp(SIGN 2 pi i q k / (Ns*R)), k = j mod Ns,
+// w = exp(SIGN 2 pi i / R).  For R >= 11 the
+// twiddles are applied first, in place, once per element (each pair item
+// would otherwise twiddle all R inputs: at R = 17 the sweep saves 4-6% of a
+// B=255 FFT/DCT launch; at R = 5 its extra LDS pass and barrier cost 8-12%);
+// smaller R twiddle inside the items.  Powers of wk and w by repeated
 // multiplication (relative error ~q ulp).
 template <typename T, int SIGN>
-__device__ __forceinline__ void mr_pass_odd(const T* a, T* b, int N, int Ns, int R, int nfft,
-                                            const T* __restrict__ tw) {
+__device__ __forceinline__ void mr_pass_odd(T* a, T* b, int N, int Ns, int R, int nfft, const T* __restrict__ tw) {
     const int NR = N / R, step = N / (Ns * R), H = (R - 1) >> 1, H1 = H + 1;
-    const unsigned long long mH1 = magic16(H1), mNR = magic16(NR), mNs = magic16(Ns);
+    const unsigned long long mH1 = magic16(H1), mNR = magic16(NR), mNs = magic16(Ns), mN = magic16(N);
+    const bool sweep = R >= 11 && Ns > 1;
+    if (sweep) {
+        for (int idx = threadIdx.x; idx < nfft * N; idx += blockDim.x) {
+            const int f = div16(idx, mN), e = idx - f * N;
+            const int q = div16(e, mNR), j = e - q * NR;
+            const int k = j - div16(j, mNs) * Ns;
+            if (q > 0 && k > 0) {
+                T w = tw[q * k * step];  // q*k*step < R*Ns*step = N
+                if (SIGN > 0) w.y = -w.y;
+                a[idx] = cmul(a[idx], w);
+            }
+        }
+        __syncthreads();
+    }
     const int total = nfft * NR * H1;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int bfi = div16(idx, mH1), sp_ = idx - bfi * H1;
         const int f = div16(bfi, mNR), j = bfi - f * NR;
         const int jq = div16(j, mNs), k = j - jq * Ns;
         const T* in = a + f * N + j;
+        T acc1 = in[0], acc2 = in[0];
         T wk = tw[k * step], WR = tw[k * step * R], om = tw[sp_ * NR];  // indices < N
         if (SIGN > 0) {
             wk.y = -wk.y;
             WR.y = -WR.y;
             om.y = -om.y;
         }
-        T acc1 = in[0], acc2 = in[0], wq = wk, oq = om;
+        T wq = wk, oq = om;  // oq = w^{sq}
         for (int q = 1; q <= H; ++q) {
-            const T vq = cmul(in[q * NR], wq);
-            const T vr = cmul(in[(R - q) * NR], cmul(WR, cconj(wq)));
+            T vq = in[q * NR], vr = in[(R - q) * NR];
+            if (!sweep) {  // wk^q and wk^{R-q} = wk^R conj(wk^q)
+                vq = cmul(vq, wq);
+                vr = cmul(vr, cmul(WR, cconj(wq)));
+                wq = cmul(wq, wk);
+            }
             const T su = cadd(vq, vr), df = csub(vq, vr);
             acc1.x += su.x * oq.x - df.y * oq.y;
             acc1.y += su.y * oq.x + df.x * oq.y;
             acc2.x += su.x * oq.x + df.y * oq.y;
             acc2.y += su.y * oq.x - df.x * oq.y;
-            wq = cmul(wq, wk);
             oq = cmul(oq, om);
         }
         T* out = b + f * N + jq * Ns * R + k;
