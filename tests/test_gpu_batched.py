@@ -190,6 +190,37 @@ def test_large_bw_round_trip(env, bw, prec, tol):
     plan.close()
 
 
+@pytest.mark.parametrize("bw", [600, 1024])
+def test_single_field_path_agrees_with_batched(env, bw):
+    # An FP64 plan of one field at bw >= 512 runs the per-field kernels (row
+    # GEMVs, 32 lanes per row in the forward above bw=542; mixed radix at
+    # bw=600), a plan of two fields the MFMA GEMMs: the same transforms, FP64
+    # both ways, so each direction pins the other path independently.
+    S, torch, dev = env
+    from s2kit_amd import synth
+    n = 2 * bw
+    cre, cim = synth.power_law_coeffs(bw, 2, 31, dev)
+    p1, p2 = S.Plan(bw, 1, precision=0), S.Plan(bw, 2, precision=0)
+    g2r = torch.empty((2, n, n), dtype=torch.float64, device=dev)
+    g2i = torch.empty_like(g2r)
+    p2.inverse(cre, cim, g2r, g2i)
+    g1r, g1i = torch.empty_like(g2r[:1]), torch.empty_like(g2i[:1])
+    p1.inverse(cre[:1].contiguous(), cim[:1].contiguous(), g1r, g1i)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+    p2.forward(g2r, g2i, c2r, c2i)
+    c1r, c1i = torch.empty_like(cre[:1]), torch.empty_like(cim[:1])
+    p1.forward(g2r[:1].contiguous(), g2i[:1].contiguous(), c1r, c1i)
+    torch.cuda.synchronize()
+    ginv = float(torch.maximum((g1r - g2r[:1]).abs().max(), (g1i - g2i[:1]).abs().max())
+                 / torch.maximum(g2r[:1].abs().max(), g2i[:1].abs().max()))
+    cfwd = float(torch.maximum((c1r - c2r[:1]).abs().max(), (c1i - c2i[:1]).abs().max())
+                 / torch.maximum(c2r[:1].abs().max(), c2i[:1].abs().max()))
+    print(f"bw={bw}: one field vs two, inverse {ginv:.2e}, forward {cfwd:.2e}", flush=True)
+    assert ginv < 1e-11 and cfwd < 1e-11
+    p1.close()
+    p2.close()
+
+
 @pytest.mark.parametrize("bw,prec,tol", [(256, 0, 1e-11), (256, 1, 1e-4), (200, 0, 1e-11)])
 @pytest.mark.parametrize("fmt", [0, 1])
 def test_cutoff_plans_agree(env, bw, prec, tol, fmt):
