@@ -120,8 +120,8 @@ __device__ __forceinline__ void mr_pass(const T* a, T* b, int N, int Ns, int nff
 // v_q = a[j + q*N/R] exp(SIGN 2 pi i q k / (Ns*R)), k = j mod Ns,
 // w = exp(SIGN 2 pi i / R).  For R >= 11 the
 // twiddles are applied first, in place, once per element (each pair item
-// would otherwise twiddle all R inputs: at R = 17 the sweep saves 4-6% of a
-// B=255 FFT/DCT launch; at R = 5 its extra LDS pass and barrier cost 8-12%);
+// would otherwise twiddle all R inputs: at R = 17 the sweep saves about 8% of
+// a B=255 FFT/DCT launch; at R = 5 its extra LDS pass and barrier cost 8-12%);
 // smaller R twiddle inside the items.  Powers of wk and w by repeated
 // multiplication (relative error ~q ulp).
 template <typename T, int SIGN>
