@@ -358,6 +358,11 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
             a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(tr, ti));
         }
     }
+    // sin(theta_j) of the output phase, loaded while the FFT runs (FP64 B=256
+    // x 1024: 1.85 -> 1.75 ms; the same for the forward's phase factors lost 3-6%)
+    R sv[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) sv[u] = sinv[(threadIdx.x + u * NTH) & (N - 1)];
     __syncthreads();
     fft_block<T, +1, V, NTH>(a, lgN, S, twl);
     const R inv_s2pi = (R)0.39894228040143267794;
@@ -375,7 +380,7 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
         const int r = frow_of(q, N, cut);
         const int am = r < B ? r : N - r;
         R sc = inv_s2pi;
-        if (am & 1) sc *= sinv[j];
+        if (am & 1) sc *= sv[u];
         if (r > B && (r & 1)) sc = -sc;
         const T y = cscale(yv, sc);
         X[((size_t)f * N + r) * N + j] = y;
