@@ -388,7 +388,9 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
     }
 }
 
-template <typename R, int V, int NTH = kThreads>
+// W16: each thread stores two adjacent samples as one 16-byte store per
+// plane (re and im must be 16-byte aligned).
+template <typename R, int V, int NTH = kThreads, bool W16 = false>
 __global__ __launch_bounds__(NTH) void k_fphi_inv(const typename Cplx<R>::type* __restrict__ X,
                                                        double* __restrict__ re, double* __restrict__ im, int lgN,
                                                        const typename Cplx<R>::type* __restrict__ tw) {
@@ -419,7 +421,22 @@ __global__ __launch_bounds__(NTH) void k_fphi_inv(const typename Cplx<R>::type* 
     __syncthreads();
     fft_block<T, +1, V, NTH>(a, lgN, S, twl);
     const size_t base = ((size_t)f * N + j0) * N;
-    {
+    if constexpr (W16) {  // sample pairs (2p, 2p+1) of one ring (N even)
+        T v[V];
+#pragma unroll
+        for (int u = 0; u < V / 2; ++u) {
+            const int idx = 2 * (threadIdx.x + u * NTH);
+            const T* row = a + (idx >> lgN) * S + (idx & (N - 1));
+            v[2 * u] = row[0];
+            v[2 * u + 1] = row[1];
+        }
+#pragma unroll
+        for (int u = 0; u < V / 2; ++u) {
+            const size_t o = base + 2 * (threadIdx.x + u * NTH);
+            *reinterpret_cast<double2*>(re + o) = make_double2((double)v[2 * u].x, (double)v[2 * u + 1].x);
+            *reinterpret_cast<double2*>(im + o) = make_double2((double)v[2 * u].y, (double)v[2 * u + 1].y);
+        }
+    } else {
         T v[V];
 #pragma unroll
         for (int u = 0; u < V; ++u) {

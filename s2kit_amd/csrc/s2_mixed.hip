@@ -504,9 +504,13 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
     }
     {
         Span sp(tm, ST_PHI_INV, s);
-        launch_lds(k_fphi_inv<float, V, kMixFftThreads>, dim3(N / rpb, F), kMixFftThreads,
-                   sizeof(float2) * ((size_t)rpb * (N + 1) + N), s,
-                   (const float2*)X, re, im, lgN, (const float2*)c.tw32);
+        const size_t lds = sizeof(float2) * ((size_t)rpb * (N + 1) + N);
+        if ((((uintptr_t)re | (uintptr_t)im) & 15) == 0)
+            launch_lds(k_fphi_inv<float, V, kMixFftThreads, true>, dim3(N / rpb, F), kMixFftThreads, lds, s,
+                       (const float2*)X, re, im, lgN, (const float2*)c.tw32);
+        else
+            launch_lds(k_fphi_inv<float, V, kMixFftThreads>, dim3(N / rpb, F), kMixFftThreads, lds, s,
+                       (const float2*)X, re, im, lgN, (const float2*)c.tw32);
     }
 }
 
