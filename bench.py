@@ -70,15 +70,15 @@ def stage_bytes(stage, bw, nfields, elem, form):
 # CT, the GEMMs' column tiles per table load, is 1 or 2 depending on bw)
 KERNELS = {
     "mixed": {"phi_fwd": "k_fphi_fwd<float, 16, 512, true>", "leg_fwd": "k_mix_fwd", "leg_inv": "k_mix_inv",
-              "phi_inv": "k_fphi_inv<float, 16, 512>"},
+              "phi_inv": "k_fphi_inv<float, 16, 512, true>"},
     "split": {"phi_fwd": "k_fphi_fwd<float, 16, 512, true>", "leg_fwd": "k_split_fwd", "leg_inv": "k_split_inv",
-              "phi_inv": "k_fphi_inv<float, 16, 512>"},
+              "phi_inv": "k_fphi_inv<float, 16, 512, true>"},
     "fp32": {"phi_fwd": "k_fphi_fwd<float, 16, 512, true>", "dct_fwd": "k_fdct_fwd<float, 16>",
              "leg_fwd": "k_gemm_fwd<float, false, CT>", "leg_inv": "k_gemm_inv<float, false, CT>",
-             "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16, 512>"},
+             "dct_inv": "k_fdct_inv<float, 16>", "phi_inv": "k_fphi_inv<float, 16, 512, false>"},
     "fp64": {"phi_fwd": "k_fphi_fwd<double, 8, 512, true>", "dct_fwd": "k_fdct_fwd<double, 8>",
              "leg_fwd": "k_gemm_fwd<double, false, CT>", "leg_inv": "k_gemm_inv<double, false, CT>",
-             "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8, 512>"},
+             "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8, 512, false>"},
 }
 # FP64 at a bandwidth that is not a power of two: mixed-radix FFT/DCT kernels
 KERNELS_GEN = {"phi_fwd": "k_phi_fwd<double2, true>", "dct_fwd": "k_dct_fwd<double2, true>",
