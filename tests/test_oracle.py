@@ -66,9 +66,19 @@ def test_transpose_row_size_counts_nonzeros_even_bw(O, bw):
 
 def test_transpose_row_size_odd_bw_quirk(O):
     # For odd B the reference's count is one short for even cosine indices
-    # <= m of even orders, and order B-1 (even) keeps no entries at all.
+    # <= m of even orders and odd indices < m of odd orders, and order B-1
+    # (even) keeps no entries at all.  True counts by brute force below.
+    def true_count(i, m, bw):
+        return len([l for l in range(m, bw)
+                    if (l if m % 2 == 0 else l - 1) >= i and ((l if m % 2 == 0 else l - 1) - i) % 2 == 0])
+
     assert O.transpose_row_size(0, 0, 17) == 8  # true count is 9
     assert sum(O.transpose_row_size(i, 16, 17) for i in range(17)) == 0
+    for bw in (9, 17, 23):
+        for m in range(bw):
+            for i in range(bw - (m % 2)):
+                short = (i % 2 == m % 2) and (i <= m if m % 2 == 0 else i < m)
+                assert O.transpose_row_size(i, m, bw) == true_count(i, m, bw) - short, (bw, m, i)
 
 
 # ---------------------------------------------------------------------------
@@ -297,6 +307,22 @@ def test_known_answer_inverse_even_bw(O, name, bw, expect):
     assert np.abs(rd - re).max() < 1e-14 and np.abs(idd - im).max() < 1e-14
 
 
+@pytest.mark.parametrize("name,bw,expect", KNOWN[2:])
+def test_known_answer_inverse_odd_bw(O, name, bw, expect):
+    # The odd-bandwidth inverse against the reference's own y43_bw23 and
+    # yMix_bw17 grids: none of their coefficients has degree bw-1, the only
+    # degree the Transpose_RowSize quirk (cospml.c:270-288) touches.
+    re, im = golden(f"y_{name}.npy")
+    rc, ic = np.zeros(bw * bw), np.zeros(bw * bw)
+    for (m, l), v in expect.items():
+        assert l < bw - 1
+        rc[O.index_of_coeff(m, l, bw)] = np.real(v)
+        ic[O.index_of_coeff(m, l, bw)] = np.imag(v)
+    rd, idd = O.inv_fst(rc, ic, bw)
+    scale = max(np.abs(re).max(), np.abs(im).max())
+    assert max(np.abs(rd - re).max(), np.abs(idd - im).max()) < 1e-14 * scale
+
+
 @pytest.mark.parametrize("bw", [64, 128])
 def test_conv_golden(O, bw):
     s = golden(f"conv{bw}_signal.npy")
@@ -343,6 +369,89 @@ def test_round_trip(O, bw, cutoff, fmt):
         assert np.abs(idd).max() < 1e-12  # Hermitian coefficients -> real grid
     r2, i2 = O.fst(rd, idd, bw, data_format=fmt, cutoff=cutoff)
     assert np.abs(r2 + 1j * i2 - (rc + 1j * ic)).max() < 1e-12
+
+
+@pytest.mark.parametrize("bw", [8, 16])
+def test_cutoff_zero_is_the_exact_transform(O, bw):
+    # At cutoff 0 the reference's order-0 naive table is only partly written
+    # (cospml.c:456,464), so its output there is undefined; the oracle (like
+    # the GPU path) computes order 0 naively on a full Pml table and returns
+    # the exact transform, checked here against the brute-force synthesis.
+    rc, ic = O.make_coeffs(77 + bw, bw, 0)
+    f = _synthesise(bw, rc + 1j * ic)
+    rd, idd = O.inv_fst(rc, ic, bw, cutoff=0)
+    assert np.abs(rd + 1j * idd - f.ravel()).max() < 1e-13
+    r2, i2 = O.fst(f.real, f.imag, bw, cutoff=0)
+    assert np.abs(r2 + 1j * i2 - (rc + 1j * ic)).max() < 1e-13
+    assert len(O.tables(bw, 0).order(0)) == 2 * bw * bw
+
+
+@pytest.mark.parametrize("bw", [9, 11, 17])
+def test_brute_force_inverse_odd_bw(O, bw):
+    # Odd bandwidths, every degree below bw-1: the inverse is the exact
+    # synthesis (scipy's orthonormal Y_l^m summed on the grid).
+    rc, ic = O.make_coeffs(31 + bw, bw, 0)
+    deg = O.degrees(bw)
+    rc[deg == bw - 1] = 0.0
+    ic[deg == bw - 1] = 0.0
+    f = _synthesise(bw, rc + 1j * ic)
+    rd, idd = O.inv_fst(rc, ic, bw)
+    assert np.abs(rd + 1j * idd - f.ravel()).max() < 1e-13
+
+
+def _top_degree_quirk_model(bw, c):
+    """Synthesis of degree-(bw-1) coefficients c[m] (|m| < bw) as the
+    reference's odd-bw inverse performs it, built from the mathematics: per
+    order, the cosine series (in theta) of P~_{bw-1}^{|m|} (divided by
+    sin(theta) for odd |m|) with the entries at cosine indices i = |m|
+    (mod 2), i <= |m| for even |m| and i < |m| for odd |m|, removed -- the
+    rows whose length Transpose_RowSize (cospml.c:270-288) counts one short,
+    losing their last term, degree bw-1 -- summed at the 2bw rings, times
+    sin(theta) for odd
+    |m|, times e^{i m phi}/sqrt(2 pi) and the Condon-Shortley sign of
+    Y_l^m for m < 0."""
+    n, l = 2 * bw, bw - 1
+    th = np.pi * (2 * np.arange(n) + 1) / (4 * bw)
+    ph = 2 * np.pi * np.arange(n) / n
+    nodes = (2 * np.arange(bw) + 1) * np.pi / (2 * bw)  # bw Chebyshev nodes: exact for cos degree < bw
+    k = np.arange(bw)
+    out = np.zeros((n, n), complex)
+    for m, v in c.items():
+        am = abs(m)
+        g = _normalised_plm(am, l, np.cos(nodes))
+        if am % 2:
+            g = g / np.sin(nodes)
+        a = (2.0 / bw) * np.cos(np.outer(k, nodes)) @ g  # cos-series coefficients, a[0] doubled
+        a[0] *= 0.5
+        drop = ((k < am) | ((k == am) & (am % 2 == 0))) & ((k - am) % 2 == 0)
+        a[drop] = 0.0
+        theta_part = np.cos(np.outer(th, k)) @ a
+        if am % 2:
+            theta_part *= np.sin(th)
+        sign = (-1) ** am if m < 0 else 1.0
+        out += v * sign * np.outer(theta_part, np.exp(1j * m * ph)) / math.sqrt(2 * math.pi)
+    return out
+
+
+@pytest.mark.parametrize("bw", [9, 11, 17])
+def test_odd_bw_top_degree_matches_quirk_model(O, bw):
+    # Degree bw-1 alone at every order: the oracle's inverse equals the model
+    # above, so the terms it drops are exactly the counted-out ones and no
+    # other (a check of the oracle against the mathematics, not against a
+    # run of the reference: the dropped set itself is read from cospml.c).
+    rng = np.random.default_rng(bw)
+    rc, ic = np.zeros(bw * bw), np.zeros(bw * bw)
+    c = {}
+    for m in range(-(bw - 1), bw):
+        v = complex(rng.uniform(-1, 1), rng.uniform(-1, 1))
+        c[m] = v
+        rc[O.index_of_coeff(m, bw - 1, bw)] = v.real
+        ic[O.index_of_coeff(m, bw - 1, bw)] = v.imag
+    rd, idd = O.inv_fst(rc, ic, bw)
+    model = _top_degree_quirk_model(bw, c)
+    assert np.abs(rd + 1j * idd - model.ravel()).max() < 1e-13
+    # and the dropped part is not empty: the exact synthesis differs
+    assert np.abs(rd + 1j * idd - _synthesise(bw, rc + 1j * ic).ravel()).max() > 1e-3
 
 
 def test_odd_bw_inverse_drops_terms_like_the_reference(O):
