@@ -240,40 +240,14 @@ def conv_uneven(args):
     against FP64, normwise per request."""
     import torch
     import s2kit_amd as S
+    from s2kit_amd import synth
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
-    rng = np.random.default_rng(2024)
     nreq = args.conv_requests
-    bws = rng.choice([32, 64, 128, 256, 512], size=nreq, p=[0.3, 0.3, 0.2, 0.15, 0.05])
-    data = {}  # bw -> dict of [count, N, N] tensors (requests of one bw back to back, in request order)
-    g = torch.Generator(device=dev)
-    g.manual_seed(7)
-    for bw in sorted(set(bws.tolist())):
-        cnt = int((bws == bw).sum())
-        n = 2 * bw
-        t = {k: torch.empty((cnt, n, n), dtype=torch.float64, device=dev)
-             for k in ("sre", "sim", "hre", "him", "o64r", "o64i", "omxr", "omxi")}
-        cr = torch.rand((cnt, bw * bw), generator=g, dtype=torch.float64, device=dev) * 2 - 1
-        ci = torch.rand((cnt, bw * bw), generator=g, dtype=torch.float64, device=dev) * 2 - 1
-        p = S.Plan(bw, cnt, precision=S.FP64)
-        p.inverse(cr, ci, t["sre"], t["sim"])
-        sig = torch.tensor(rng.uniform(5, bw / 2, cnt), dtype=torch.float64, device=dev)
-        l = torch.arange(bw, dtype=torch.float64, device=dev)
-        hc = torch.zeros((cnt, bw * bw), dtype=torch.float64, device=dev)
-        hc[:, :bw] = torch.exp(-l[None, :] * (l[None, :] + 1) / (2 * sig[:, None] ** 2))
-        p.inverse(hc, torch.zeros_like(hc), t["hre"], t["him"])
-        torch.cuda.synchronize()
-        p.close()
-        data[bw] = t
-    slot = {bw: 0 for bw in data}
-    req64, reqmx = [], []
-    for bw in bws.tolist():
-        j = slot[bw]
-        slot[bw] += 1
-        t = data[bw]
-        req64.append((bw, t["sre"][j], t["sim"][j], t["hre"][j], t["him"][j], t["o64r"][j], t["o64i"][j]))
-        reqmx.append((bw, t["sre"][j], t["sim"][j], t["hre"][j], t["him"][j], t["omxr"][j], t["omxi"][j]))
+    bws, data = synth.conv_uneven_requests(nreq, 2024, dev)
+    req64 = synth.conv_requests_of(bws, data, "o64")
+    reqmx = synth.conv_requests_of(bws, data, "omx")
     out = {}
     for name, reqs, prec in (("fp64", req64, S.FP64), ("mixed", reqmx, S.MIXED)):
         b = S.ConvBatch(reqs, precision=prec)

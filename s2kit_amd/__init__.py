@@ -308,13 +308,35 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _check_tensors(args, device):
+    """Reject what the C ABI cannot see through a raw pointer.  `args` holds
+    (name, tensor, values needed) triples: every tensor must be a float64,
+    contiguous torch tensor holding at least the values needed (checked for
+    all of them first), and live on CUDA device index `device`."""
+    import torch
+    for name, t, need in args:
+        if not isinstance(t, torch.Tensor):
+            raise S2Error(f"{name}: expected a torch tensor, got {type(t).__name__}")
+        if t.dtype != torch.float64:
+            raise S2Error(f"{name}: dtype must be torch.float64, got {t.dtype}")
+        if not t.is_contiguous():
+            raise S2Error(f"{name}: tensor must be contiguous")
+        if t.numel() < need:
+            raise S2Error(f"{name}: holds {t.numel()} values, the call needs {need}")
+    for name, t, _ in args:
+        if not t.is_cuda or t.device.index != device:
+            raise S2Error(f"{name}: must be a CUDA tensor on cuda:{device}, got {t.device}")
+
+
 class Plan:
     """Batched transforms of up to ``max_fields`` fields at bandwidth ``bw``
     on device memory.  Arguments are torch float64 CUDA tensors laid out per
     field like the drop-ins: grids [F, 2bw, 2bw], coefficients [F, bw*bw]."""
 
     def __init__(self, bw, max_fields, precision=FP64, cutoff=None):
+        import torch
         cutoff = bw if cutoff is None else cutoff
+        self.device = torch.cuda.current_device()
         h = ctypes.c_void_p()
         rc = lib().s2amd_plan_create(ctypes.byref(h), bw, cutoff, max_fields, precision)
         if rc != 0:
@@ -362,23 +384,43 @@ class Plan:
         if rc != 0:
             raise S2Error(last_error())
 
+    def _fields(self, nfields, args):
+        """Validate (name, tensor, values per field) triples for a call of
+        nfields fields (default: the first tensor's leading dimension)."""
+        first = args[0][1]
+        if nfields is not None:
+            n = int(nfields)
+        elif hasattr(first, "dim") and first.dim() > 0:
+            n = first.shape[0]
+        else:
+            n = 1
+        if n < 0 or n > self.max_fields:
+            raise S2Error(f"nfields must be in [0, {self.max_fields}], got {n}")
+        _check_tensors([(name, t, n * per) for name, t, per in args], self.device)
+        return n
+
     def forward(self, re, im, cre, cim, nfields=None, data_format=COMPLEX, stream=None):
-        n = re.shape[0] if nfields is None else nfields
+        g, c = 4 * self.bw * self.bw, self.bw * self.bw
+        n = self._fields(nfields, [("re", re, g), ("im", im, g), ("cre", cre, c), ("cim", cim, c)])
         self._check(lib().s2amd_forward(self.h, _ptr(re), _ptr(im), _ptr(cre), _ptr(cim), n, data_format,
                                         self._stream(stream)))
 
     def inverse(self, cre, cim, re, im, nfields=None, data_format=COMPLEX, stream=None):
-        n = cre.shape[0] if nfields is None else nfields
+        g, c = 4 * self.bw * self.bw, self.bw * self.bw
+        n = self._fields(nfields, [("cre", cre, c), ("cim", cim, c), ("re", re, g), ("im", im, g)])
         self._check(lib().s2amd_inverse(self.h, _ptr(cre), _ptr(cim), _ptr(re), _ptr(im), n, data_format,
                                         self._stream(stream)))
 
     def zonal(self, re, im, zr, zi, nfields=None, data_format=COMPLEX, stream=None):
-        n = re.shape[0] if nfields is None else nfields
+        g = 4 * self.bw * self.bw
+        n = self._fields(nfields, [("re", re, g), ("im", im, g), ("zr", zr, self.bw), ("zi", zi, self.bw)])
         self._check(lib().s2amd_zonal(self.h, _ptr(re), _ptr(im), _ptr(zr), _ptr(zi), n, data_format,
                                       self._stream(stream)))
 
     def conv(self, sre, sim, hre, him, ore, oim, nfields=None, stream=None):
-        n = sre.shape[0] if nfields is None else nfields
+        g = 4 * self.bw * self.bw
+        n = self._fields(nfields, [("sre", sre, g), ("sim", sim, g), ("hre", hre, g), ("him", him, g),
+                                   ("ore", ore, g), ("oim", oim, g)])
         self._check(lib().s2amd_conv(self.h, _ptr(sre), _ptr(sim), _ptr(hre), _ptr(him), _ptr(ore), _ptr(oim), n,
                                      self._stream(stream)))
 
@@ -429,8 +471,14 @@ class ConvBatch:
     contents are read at every run()."""
 
     def __init__(self, requests, precision=MIXED):
+        import torch
+        dev = torch.cuda.current_device()
         arr = (ConvRequest * len(requests))()
         for k, (bw, *ts) in enumerate(requests):
+            if len(ts) != 6:
+                raise S2Error(f"request {k}: expected (bw, sre, sim, hre, him, ore, oim)")
+            _check_tensors([(f"request {k} {name}", t, 4 * bw * bw)
+                            for name, t in zip(("sre", "sim", "hre", "him", "ore", "oim"), ts)], dev)
             arr[k] = ConvRequest(bw, *[t.data_ptr() for t in ts])
         h = ctypes.c_void_p()
         rc = lib().s2amd_conv_batch_create(ctypes.byref(h), ctypes.cast(arr, ctypes.c_void_p), len(requests),

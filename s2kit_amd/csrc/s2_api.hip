@@ -4,9 +4,10 @@
 // Drop-in calls run on the current HIP device, one at a time (a process-wide
 // mutex), on a stream owned by the library.  Caller tables are copied to the
 // device once and cached: the cache key is (bw, cutoff, direction, the
-// caller's per-order pointers) plus a fingerprint of 64 sampled values per
-// order, re-checked on every call, so a table rebuilt in the same memory for
-// other parameters is uploaded again.
+// caller's per-order pointers) plus a hash of every entry of every order,
+// recomputed on every call, so a table edited in place (any entry) or rebuilt
+// in the same memory is uploaded again, as the reference reads the caller's
+// table on every call.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,6 +19,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "s2_internal.h"
@@ -113,45 +115,111 @@ static void ensure_device(Ctx& c) {
     S2_HIP(hipGetDevice(&dev));
     if (c.stream && c.device == dev) return;
     if (c.stream) {
-        // the current device changed: drop per-device state
+        // the current device changed: drop the previous device's state and
+        // stream, with that device current, then come back
+        DeviceGuard g(c.device);
         for (auto& e : c.tables) tables_destroy(e.t);
         for (auto& e : c.consts) consts_destroy(e.c);
         c.tables.clear();
         c.consts.clear();
         for (DBuf* b : {&c.re, &c.im, &c.cre, &c.cim, &c.X, &c.Cb, &c.aux1, &c.aux2, &c.aux3, &c.aux4})
             b->release();
+        (void)hipStreamDestroy(c.stream);
+        c.stream = nullptr;
     }
     c.device = dev;
     S2_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
 }
 
-static unsigned long long fingerprint(const double* p, long n) {
-    unsigned long long h = 1469598103934665603ULL ^ (unsigned long long)n;
-    if (n <= 0) return h;
-    const long step = std::max(1L, n / 64);
-    for (long i = 0; i < n; i += step) {
+// Hash of all n entries of a table.  Four interleaved lanes, each a chain of
+// s <- (s ^ word) * odd constant: every step is a bijection of the lane state,
+// so changing any single entry always changes its lane's final state, and the
+// lanes are folded the same way, so it always changes the hash.
+static unsigned long long table_hash(const double* p, long n) {
+    const unsigned long long K = 0x9E3779B97F4A7C15ULL;  // odd: multiplication is a bijection
+    unsigned long long h[4] = {0x243F6A8885A308D3ULL, 0x13198A2E03707344ULL, 0xA4093822299F31D0ULL,
+                               0x082EFA98EC4E6C89ULL};
+    long i = 0;
+    for (; i + 4 <= n; i += 4) {
+        unsigned long long v[4];
+        std::memcpy(v, p + i, 32);
+        for (int k = 0; k < 4; ++k) h[k] = (h[k] ^ v[k]) * K;
+    }
+    for (; i < n; ++i) {
         unsigned long long v;
         std::memcpy(&v, p + i, 8);
-        h = (h ^ v) * 1099511628211ULL;
+        h[0] = (h[0] ^ v) * K;
     }
-    unsigned long long v;
-    std::memcpy(&v, p + n - 1, 8);
-    return (h ^ v) * 1099511628211ULL;
+    unsigned long long r = (unsigned long long)n * K;
+    for (int k = 0; k < 4; ++k) r = (r ^ h[k]) * K;
+    return r ^ (r >> 29);
 }
 
 static long order_len(int m, int bw, int cutoff) {
     return (m < cutoff) ? table_size(m, bw) : 2L * bw * (bw - m);
 }
 
+// Hashes of orders 0..norders-1 of a caller table set; large sets are hashed
+// on several host threads (orders split into contiguous ranges).
+static std::vector<unsigned long long> table_hashes(const double* const* ptrs, int norders, int bw, int cutoff) {
+    std::vector<unsigned long long> h(norders);
+    long total = 0;
+    for (int m = 0; m < norders; ++m) total += order_len(m, bw, cutoff);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<long>({(long)std::min(hw, 8u), (long)norders, std::max(1L, total >> 21)});
+    auto work = [&](int t) {
+        for (int m = t; m < norders; m += nt) h[m] = table_hash(ptrs[m], order_len(m, bw, cutoff));
+    };
+    if (nt <= 1) {
+        work(0);
+        return h;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    return h;
+}
+
+// Device bytes an entry holds (direction 3 entries hold both directions).
+static size_t entry_bytes(const CachedTables& e) {
+    return sizeof(double) * (size_t)e.t.total * (e.dir == 3 ? 2 : 1);
+}
+
+// Every insertion into the table cache: drops an entry of the same key (bw,
+// cutoff, direction, device, caller pointers), then evicts least recently
+// used entries until the new one fits the budget.  Callers insert only with
+// the library stream idle (every drop-in ends with a synchronisation), so no
+// queued work reads an evicted table.
+static const DevTables& cache_insert(Ctx& c, CachedTables&& e) {
+    for (auto it = c.tables.begin(); it != c.tables.end();) {
+        if (it->bw == e.bw && it->cutoff == e.cutoff && it->dir == e.dir && it->device == e.device &&
+            it->ptrs == e.ptrs) {
+            tables_destroy(it->t);
+            it = c.tables.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    size_t used = entry_bytes(e);
+    for (const auto& x : c.tables) used += entry_bytes(x);
+    while (!c.tables.empty() && used > c.table_budget) {
+        used -= entry_bytes(c.tables.back());
+        tables_destroy(c.tables.back().t);
+        c.tables.pop_back();
+    }
+    c.tables.push_front(std::move(e));
+    return c.tables.front().t;
+}
+
 // Device copy of a caller's table set (one direction).  norders = bw, or 1
 // for the order-0 table of FZTSemiMemo.
 static const DevTables& device_tables(Ctx& c, int bw, int cutoff, int dir, const double* const* ptrs,
                                       int norders) {
-    std::vector<unsigned long long> fp(norders);
-    for (int m = 0; m < norders; ++m) {
+    for (int m = 0; m < norders; ++m)
         if (!ptrs[m]) throw ArgError{"table pointer for order " + std::to_string(m) + " is NULL"};
-        fp[m] = fingerprint(ptrs[m], order_len(m, bw, cutoff));
-    }
+    const std::vector<unsigned long long> fp = table_hashes(ptrs, norders, bw, cutoff);
     for (auto it = c.tables.begin(); it != c.tables.end(); ++it) {
         if (it->bw != bw || it->cutoff != cutoff || it->dir != dir || it->device != c.device) continue;
         if ((int)it->ptrs.size() != norders || !std::equal(it->ptrs.begin(), it->ptrs.end(), ptrs)) continue;
@@ -188,16 +256,7 @@ static const DevTables& device_tables(Ctx& c, int bw, int cutoff, int dir, const
     else
         e.t.fwd = d;
     e.t.owned = true;
-    // evict least recently used entries above the budget
-    size_t used = sizeof(double) * pos;
-    for (auto& x : c.tables) used += sizeof(double) * x.t.total;
-    while (!c.tables.empty() && used > c.table_budget) {
-        used -= sizeof(double) * c.tables.back().t.total;
-        tables_destroy(c.tables.back().t);
-        c.tables.pop_back();
-    }
-    c.tables.push_front(std::move(e));
-    return c.tables.front().t;
+    return cache_insert(c, std::move(e));
 }
 
 // Register a device table set that this library produced for the caller's
@@ -209,8 +268,7 @@ static void register_tables(Ctx& c, DevTables&& t, int dir, double** ptrs, int n
     e.dir = dir;
     e.device = c.device;
     e.ptrs.assign(ptrs, ptrs + norders);
-    e.fp.resize(norders);
-    for (int m = 0; m < norders; ++m) e.fp[m] = fingerprint(ptrs[m], order_len(m, t.bw, t.cutoff));
+    e.fp = table_hashes(ptrs, norders, t.bw, t.cutoff);
     if (dir == 1 && t.fwd) {
         (void)hipFree(t.fwd);
         t.fwd = nullptr;
@@ -220,7 +278,7 @@ static void register_tables(Ctx& c, DevTables&& t, int dir, double** ptrs, int n
         t.inv = nullptr;
     }
     e.t = std::move(t);
-    c.tables.push_front(std::move(e));
+    (void)cache_insert(c, std::move(e));
 }
 
 // Prescaled weights and twiddles for bw; `weights` (4bw, host) if the caller
@@ -697,8 +755,8 @@ static const DevTables& gen_tables(Ctx& c, int bw, int cutoff) {
     e.dir = 3;
     e.device = c.device;
     tables_generate(e.t, bw, cutoff, true, true, c.stream);
-    c.tables.push_front(std::move(e));
-    return c.tables.front().t;
+    S2_HIP(hipStreamSynchronize(c.stream));
+    return cache_insert(c, std::move(e));
 }
 
 // ConvOn2SphereSemiMemo / ConvOn2SphereSemiFly: REAL data, cutoff = bw
@@ -1007,7 +1065,7 @@ extern "C" int s2amd_plan_autotune(s2amd_plan* p, int nfields, void* stream) {
     return run_plan("s2amd_plan_autotune", [&] {
         if (!p) throw ArgError{"plan is NULL"};
         if (nfields < 1 || nfields > p->max_fields) throw ArgError{"nfields must be in [1, max_fields]"};
-        S2_HIP(hipSetDevice(p->device));
+        DeviceGuard dg(p->device);
         const hipStream_t s = (hipStream_t)stream;
         const bool gemm = (p->precision == S2AMD_FP64 || p->precision == S2AMD_FP32) && use_fast(p, nfields);
         const bool mixed = p->precision == S2AMD_MIXED;
@@ -1073,7 +1131,7 @@ extern "C" int s2amd_plan_destroy(s2amd_plan* p) {
     return run_plan("s2amd_plan_destroy", [&] {
         if (!p) return;
         // work queued on the caller's streams may still read the plan's buffers
-        S2_HIP(hipSetDevice(p->device));
+        DeviceGuard dg(p->device);
         S2_HIP(hipDeviceSynchronize());
         consts_destroy(p->c);
         tables_destroy(p->t);

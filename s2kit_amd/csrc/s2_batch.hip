@@ -71,6 +71,8 @@ struct s2amd_conv_batch {
 
 static void conv_batch_free(s2amd_conv_batch* b) {
     if (!b) return;
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
     (void)hipSetDevice(b->device);
     (void)hipDeviceSynchronize();
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -83,6 +85,7 @@ static void conv_batch_free(s2amd_conv_batch* b) {
     }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+    if (prev >= 0) (void)hipSetDevice(prev);
 }
 
 static void check_plan(int rc) {

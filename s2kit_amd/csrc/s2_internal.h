@@ -33,6 +33,20 @@ struct ArgError {
     std::string msg;
 };
 
+// Makes `dev` current for a scope and restores the caller's device on exit.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        S2_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+    DeviceGuard(const DeviceGuard&) = delete;
+    DeviceGuard& operator=(const DeviceGuard&) = delete;
+};
+
 // Map the exception escaping an API body to an S2AMD_* code, recording the
 // message for s2amd_last_error().
 int api_status(const char* fn, const std::exception_ptr& ep);

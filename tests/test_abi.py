@@ -134,3 +134,49 @@ def test_conv_batch_rejects_bad_arguments_before_touching_the_device(S):
     assert "aligned" in S.last_error()
     assert L.s2amd_conv_batch_run(None, None) == -1
     assert L.s2amd_conv_batch_destroy(None) == 0
+
+
+def _unbound_plan(S, bw=8, max_fields=4):
+    """A Plan object with no library handle: enough to run the wrappers'
+    argument checks, which must all fire before any library call."""
+    p = S.Plan.__new__(S.Plan)
+    p.h, p.bw, p.cutoff, p.max_fields, p.precision, p.device = None, bw, bw, max_fields, S.FP64, 0
+    return p
+
+
+def test_plan_wrappers_reject_tensors_the_abi_cannot_check(S):
+    import torch
+    p = _unbound_plan(S)
+    g = lambda *shape, **kw: torch.zeros(*shape, dtype=kw.get("dtype", torch.float64))
+    grid, coef = g(2, 16, 16), g(2, 64)
+    cases = [
+        ((grid.float(), grid, coef, coef), "dtype must be torch.float64"),
+        ((g(2, 16, 32)[:, :, ::2], grid, coef, coef), "contiguous"),
+        ((grid, grid, g(1, 64), coef), "holds 64 values, the call needs 128"),
+        ((grid, grid, coef, coef), "must be a CUDA tensor"),  # CPU tensors: never passed as pointers
+        (([1.0], grid, coef, coef), "expected a torch tensor"),
+    ]
+    for args, msg in cases:
+        with pytest.raises(S.S2Error, match=msg):
+            p.forward(*args)
+    with pytest.raises(S.S2Error, match=r"nfields must be in \[0, 4\]"):
+        p.forward(grid, grid, coef, coef, nfields=5)
+    with pytest.raises(S.S2Error, match="cre: holds 64 values"):
+        p.inverse(g(1, 64), coef, grid, grid, nfields=2)
+    with pytest.raises(S.S2Error, match="zr: holds 8 values"):
+        p.zonal(grid, grid, g(8), g(16))
+    with pytest.raises(S.S2Error, match="oim: dtype"):
+        p.conv(grid, grid, grid, grid, grid, grid.float())
+    S.Plan.close(p)  # no handle: nothing to free
+
+
+def test_conv_batch_wrapper_rejects_bad_tensors(S, monkeypatch):
+    import torch
+    monkeypatch.setattr(torch.cuda, "current_device", lambda: 0)
+    t = torch.zeros(16, 16, dtype=torch.float64)
+    with pytest.raises(S.S2Error, match="request 0 sre: must be a CUDA tensor"):
+        S.ConvBatch([(8, t, t, t, t, t, t)])
+    with pytest.raises(S.S2Error, match="request 0 hre: holds 64 values, the call needs 256"):
+        S.ConvBatch([(8, t, t, torch.zeros(64, dtype=torch.float64), t, t, t)])
+    with pytest.raises(S.S2Error, match="expected"):
+        S.ConvBatch([(8, t, t, t)])

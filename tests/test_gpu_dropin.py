@@ -236,6 +236,35 @@ def test_table_cache_sees_in_place_rewrite(S, O):
     assert crel(b[0], b[1], 2 * a[0], 2 * a[1]) < 1e-14
 
 
+@pytest.mark.parametrize("m", [0, 3, 40])
+def test_table_cache_sees_one_entry_edited_in_place(S, O, m):
+    # One entry of one order changed in place, away from any sparse sample
+    # grid (not a multiple of len/64, not the last): both directions must
+    # read the edited value, as the reference reads the caller's table on
+    # every call.  The oracle transforms with the same edited table.
+    print(f"table cache one-entry edit m={m}", flush=True)
+    bw, fmt = 64, 0
+    ot = O.Tables(bw)
+    n = 2 * bw
+    rng = np.random.default_rng(10 + m)
+    rd, idd = rng.uniform(-1, 1, n * n), rng.uniform(-1, 1, n * n)
+    rc, ic = rng.uniform(-1, 1, bw * bw), rng.uniform(-1, 1, bw * bw)
+    w = O.weights(bw)
+    length = O.table_size(m, bw)
+    step = max(1, length // 64)
+    k = next(k for k in range(1, length - 1) if k % step)
+    fst_with(S, rd, idd, bw, ot, fmt, bw, w)  # cache both directions' device copies
+    inv_with(S, rc, ic, bw, ot, fmt, bw)
+    ot.fwd[ot.off[m] + k] += 0.5
+    ot.inv[ot.off[m] + k] -= 0.25
+    a = fst_with(S, rd, idd, bw, ot, fmt, bw, w)
+    b = O.fst(rd, idd, bw, data_format=fmt, tabs=ot)
+    assert crel(a[0], a[1], b[0], b[1]) < 1e-11
+    c = inv_with(S, rc, ic, bw, ot, fmt, bw)
+    d = O.inv_fst(rc, ic, bw, data_format=fmt, tabs=ot)
+    assert crel(c[0], c[1], d[0], d[1]) < 1e-11
+
+
 # ---- FST_semi_fly.h: the same transforms with library-made tables -----------
 FLY_CASES = [(8, 8, 0), (17, 17, 1), (64, 22, 0), (64, 22, 1), (128, 54, 0), (64, 0, 0)]
 
