@@ -86,6 +86,27 @@ int s2amd_forward(s2amd_plan* plan, const double* re, const double* im, double* 
 /* nfields coefficient sets -> grids (InvFSTSemiMemo per field). */
 int s2amd_inverse(s2amd_plan* plan, const double* cre, const double* cim, double* re, double* im, int nfields,
                   int data_format, void* stream);
+/* s2amd_forward for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, kernels and results
+ * (bit for bit) as s2amd_forward on the same plan; the transform's scratch
+ * (the order-major intermediate and, for FP64/FP32 plans, the cosine-row
+ * buffer) lives in one static device array of S2AMD_NOALLOC_SCRATCH_BYTES
+ * instead of the plan's buffers.  The library holds that array on every
+ * device it runs on until the process ends.  A call needs per field
+ *   mixed / split-BF16 plans:  8 * (2bw)^2 bytes
+ *   FP64 / FP32 plans:        16 * (2bw)^2 + 16 * 2bw * bw bytes (+256)
+ * (bw=256: 2 MiB / 6 MiB per field, so up to 64 / 21 fields) and returns
+ * S2AMD_EINVAL, having launched nothing, when nfields fields need more.  It
+ * calls no allocating routine (the first call on a device only looks up the
+ * array's address).  Every call on a device shares the array, so calls that
+ * overlap on one device (different streams, or host threads) race: serialise
+ * them.  Nothing in the array needs to start at zero: each call writes every
+ * scratch value before reading it.  The plan's stage timing is not recorded
+ * for these calls. */
+#define S2AMD_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int s2amd_forward_noalloc(s2amd_plan* plan, const double* re, const double* im, double* cre, double* cim,
+                          int nfields, int data_format, void* stream);
 /* Zonal coefficients zr/zi [nfields][bw] (FZTSemiMemo per field). */
 int s2amd_zonal(s2amd_plan* plan, const double* re, const double* im, double* zr, double* zi, int nfields,
                 int data_format, void* stream);

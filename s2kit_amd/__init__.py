@@ -72,6 +72,7 @@ SIGNATURES = {
     "s2amd_plan_bytes": (ctypes.c_size_t, [_vp]),
     "s2amd_plan_tables": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_long)]),
     "s2amd_forward": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "s2amd_forward_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "s2amd_inverse": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "s2amd_zonal": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "s2amd_conv": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),

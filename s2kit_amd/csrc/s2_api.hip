@@ -355,14 +355,61 @@ static bool use_fast(const Plan* p, int F) {
     return F >= 2 || p->bw < 512;
 }
 
+// The forward pipeline of a plan on scratch X (order-major intermediate) and
+// Cb (cosine rows; unused by the mixed and split-BF16 forms).
+static void plan_forward_on(Plan* p, const double* re, const double* im, double* cre, double* cim, int F, Fmt fmt,
+                            void* X, void* Cb, hipStream_t s, StageTimer* tm) {
+    if (p->precision == S2AMD_MIXED || p->precision == S2AMD_SPLIT_BF16)
+        forward_mixed(p->c, p->g, re, im, cre, cim, F, fmt, X, s, tm);
+    else if (use_fast(p, F))
+        forward_fast(p->c, p->g, re, im, cre, cim, F, fmt, X, Cb, s, tm);
+    else
+        forward_fp64(p->c, p->t, re, im, cre, cim, F, fmt, (double2*)X, (double2*)Cb, s, tm);
+}
+
 static void plan_forward(Plan* p, const double* re, const double* im, double* cre, double* cim, int F, Fmt fmt,
                          hipStream_t s) {
-    if (p->precision == S2AMD_MIXED || p->precision == S2AMD_SPLIT_BF16)
-        forward_mixed(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, s, &p->timer);
-    else if (use_fast(p, F))
-        forward_fast(p->c, p->g, re, im, cre, cim, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
-    else
-        forward_fp64(p->c, p->t, re, im, cre, cim, F, fmt, p->X.as<double2>(), p->Cb.as<double2>(), s, &p->timer);
+    plan_forward_on(p, re, im, cre, cim, F, fmt, p->X.p, p->Cb.p, s, &p->timer);
+}
+
+// ---- scratch of s2amd_forward_noalloc --------------------------------------
+// One static device array, held by every device the library runs on until the
+// process ends; no call allocates it.  In host code `scratch` names a
+// host-side stand-in, not device memory: kernels get the array's device
+// address, looked up once per device by hipGetSymbolAddress and kept in
+// g_scratch_ptr.  Calls that overlap on one device share (and race on) it.
+static __device__ __align__(256) unsigned char scratch[S2AMD_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of `scratch` on the current device `dev`.
+static void* scratch_on(int dev) {
+    if (dev < 0 || dev >= kScratchDevices)
+        throw ArgError{"device index " + std::to_string(dev) + " beyond the " + std::to_string(kScratchDevices) +
+                       " devices the no-alloc scratch serves"};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        S2_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw HipError{hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch) returned NULL"};
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Scratch bytes of a forward over F fields: the plan's own X (and Cb) sizes
+// per field, Cb placed at a 256-byte boundary after X.
+static size_t noalloc_need(const Plan* p, int F, size_t* cb_off) {
+    const size_t N = 2 * (size_t)p->bw;
+    if (p->precision == S2AMD_MIXED || p->precision == S2AMD_SPLIT_BF16) {
+        *cb_off = 0;
+        return sizeof(float2) * N * N * F;
+    }
+    const size_t x = (sizeof(double2) * N * N * F + 255) / 256 * 256;
+    *cb_off = x;
+    return x + sizeof(double2) * N * p->bw * F;
 }
 
 static void plan_inverse(Plan* p, const double* cre, const double* cim, double* re, double* im, int F, Fmt fmt,
@@ -1169,6 +1216,31 @@ extern "C" int s2amd_forward(s2amd_plan* p, const double* re, const double* im, 
         need(cre, "cre");
         need(cim, "cim");
         plan_forward(p, re, im, cre, cim, nfields, (Fmt)data_format, (hipStream_t)stream);
+    });
+}
+
+extern "C" int s2amd_forward_noalloc(s2amd_plan* p, const double* re, const double* im, double* cre, double* cim,
+                                     int nfields, int data_format, void* stream) {
+    return run_plan("s2amd_forward_noalloc", [&] {
+        plan_check_call(p, nfields, data_format);
+        if (nfields == 0) return;
+        need(re, "re");
+        need(im, "im");
+        need(cre, "cre");
+        need(cim, "cim");
+        size_t cb_off = 0;
+        const size_t bytes = noalloc_need(p, nfields, &cb_off);
+        if (bytes > S2AMD_NOALLOC_SCRATCH_BYTES) {
+            size_t per = 0, off = 0;
+            per = noalloc_need(p, 1, &off);
+            throw ArgError{std::to_string(nfields) + " fields at bw=" + std::to_string(p->bw) + " need " +
+                           std::to_string(bytes) + " bytes of scratch, beyond the " +
+                           std::to_string((size_t)S2AMD_NOALLOC_SCRATCH_BYTES) + " of the static array (" +
+                           std::to_string(per) + " per field)"};
+        }
+        unsigned char* base = static_cast<unsigned char*>(scratch_on(p->device));
+        plan_forward_on(p, re, im, cre, cim, nfields, (Fmt)data_format, base, base + cb_off, (hipStream_t)stream,
+                        nullptr);
     });
 }
 
