@@ -365,6 +365,21 @@ def cpu_worker(args):
     return time.perf_counter() - t0
 
 
+# A one-GPU box allots this many host cores to its GPU; the CPU baseline runs
+# one process per core of that share (os.cpu_count() there reports the whole
+# host, which other GPUs' jobs share).
+CPU_SHARE_PER_GPU = 16
+
+
+def host_cores():
+    """(cores this process may run on, cores the host reports)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
+
+
 def cpu_baseline(bw, pairs_per_proc, procs):
     import multiprocessing as mp
     ctx = mp.get_context("fork")  # before any GPU initialisation in this process
@@ -373,14 +388,19 @@ def cpu_baseline(bw, pairs_per_proc, procs):
         times = pool.map(cpu_worker, [(bw, pairs_per_proc, 7 + i) for i in range(procs)])
         wall = time.perf_counter() - t0
     total = pairs_per_proc * procs
+    usable, reported = host_cores()
     return {
         "value": total / max(times),
         "unit": "fwd+inv pairs/s",
         "cores": procs,
         "kind": "port",
+        "host_cores_usable": usable,
+        "host_cores_reported": reported,
         "sample": f"{total} fwd+inv pairs at bw={bw} (power-law coefficients), {procs} single-threaded "
-                  f"processes x {pairs_per_proc} pairs, tables precomputed; slowest process {max(times):.2f} s, "
-                  f"pool wall {wall:.2f} s; own FFT/DCT (no FFTW on this host)",
+                  f"processes x {pairs_per_proc} pairs (one per core of the {CPU_SHARE_PER_GPU}-core share of "
+                  f"one GPU; {usable} cores usable by this process, {reported} reported by the host), tables "
+                  f"precomputed; slowest process {max(times):.2f} s, pool wall {wall:.2f} s; own FFT/DCT "
+                  f"(no FFTW on this host)",
     }
 
 
@@ -393,7 +413,8 @@ def main():
     ap.add_argument("--fields", type=int, default=1024)
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed", "split"], default="mixed")
     ap.add_argument("--cpu-pairs", type=int, default=150, help="oracle pairs per CPU process (0: skip)")
-    ap.add_argument("--cpu-procs", type=int, default=0, help="CPU processes (0: min(16, cores))")
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="CPU processes (0: one per usable core, at most the 16-core share of one GPU)")
     ap.add_argument("--plain-fields", type=int, default=64, help="fields timed in the batch-1 FP64 form")
     ap.add_argument("--other-forms", type=int, default=1,
                     help="also time the other batched forms (FP64, mixed / split BF16) on the same fields (0: skip)")
@@ -422,7 +443,7 @@ def main():
     bw, F = args.bw, args.fields
     cpu = None
     if args.cpu_pairs > 0:
-        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        procs = args.cpu_procs or min(CPU_SHARE_PER_GPU, host_cores()[0])
         cpu = cpu_baseline(bw, args.cpu_pairs, procs)
         if args.dropin_reps > 0:
             cpu["dropin_cases"] = dropin_cpu(args.dropin_reps)

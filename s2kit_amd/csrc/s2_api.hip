@@ -306,6 +306,16 @@ static const DevConsts& device_consts(Ctx& c, int bw, const double* weights) {
     return c.consts.front().c;
 }
 
+// The caller's DataFormat as an int, read through its object representation:
+// C callers may pass any int, and reading a C++ enum parameter outside the
+// enum's values is undefined, so it is checked before it is used as one.
+static int fmt_of(const DataFormat& d) {
+    static_assert(sizeof(DataFormat) == sizeof(int), "DataFormat is an int-sized enum");
+    int v;
+    std::memcpy(&v, &d, sizeof v);
+    return v;
+}
+
 static void check_common(int bw, int fmt) {
     if (bw < 2 || bw > 1024) throw ArgError{"bw must be in [2, 1024], got " + std::to_string(bw)};
     if (fmt != 0 && fmt != 1) throw ArgError{"data_format must be COMPLEX (0) or REAL (1)"};
@@ -701,7 +711,7 @@ extern "C" void FSTSemiMemo(double* rdata, double* idata, double* rcoeffs, doubl
     (void)DCT_plan;
     (void)FFT_plan;
     run_dropin("FSTSemiMemo", [&](Ctx& c) {
-        check_common(bw, (int)data_format);
+        check_common(bw, fmt_of(data_format));
         if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
         need(rdata, "rdata");
         need(idata, "idata");
@@ -724,7 +734,7 @@ extern "C" void InvFSTSemiMemo(double* rcoeffs, double* icoeffs, double* rdata, 
     (void)inv_DCT_plan;
     (void)inv_FFT_plan;
     run_dropin("InvFSTSemiMemo", [&](Ctx& c) {
-        check_common(bw, (int)data_format);
+        check_common(bw, fmt_of(data_format));
         if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
         need(rdata, "rdata");
         need(idata, "idata");
@@ -744,7 +754,7 @@ extern "C" void FZTSemiMemo(double* rdata, double* idata, double* rres, double* 
     (void)workspace;
     (void)DCT_plan;
     run_dropin("FZTSemiMemo", [&](Ctx& c) {
-        check_common(bw, (int)data_format);
+        check_common(bw, fmt_of(data_format));
         need(rdata, "rdata");
         need(idata, "idata");
         need(rres, "rres");
@@ -969,7 +979,7 @@ extern "C" void FSTSemiFly(double* rdata, double* idata, double* rcoeffs, double
     (void)DCT_plan;
     (void)FFT_plan;
     run_dropin("FSTSemiFly", [&](Ctx& c) {
-        check_common(bw, (int)data_format);
+        check_common(bw, fmt_of(data_format));
         if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
         need(rdata, "rdata");
         need(idata, "idata");
@@ -988,7 +998,7 @@ extern "C" void InvFSTSemiFly(double* rcoeffs, double* icoeffs, double* rdata, d
     (void)inv_DCT_plan;
     (void)inv_FFT_plan;
     run_dropin("InvFSTSemiFly", [&](Ctx& c) {
-        check_common(bw, (int)data_format);
+        check_common(bw, fmt_of(data_format));
         if (cutoff < 0 || cutoff > bw) throw ArgError{"cutoff must be in [0, bw]"};
         need(rdata, "rdata");
         need(idata, "idata");
@@ -1004,7 +1014,7 @@ extern "C" void FZTSemiFly(double* rdata, double* idata, double* rres, double* i
     (void)workspace;
     (void)DCT_plan;
     run_dropin("FZTSemiFly", [&](Ctx& c) {
-        check_common(bw, (int)data_format);
+        check_common(bw, fmt_of(data_format));
         need(rdata, "rdata");
         need(idata, "idata");
         need(rres, "rres");
