@@ -170,6 +170,28 @@ def non_pow2(dev, steps):
     return out
 
 
+def graph_replay_time(fn, reps):
+    """Seconds per call of fn (a sequence of plan calls on torch's current
+    stream), recorded once in a HIP graph and replayed: the device time of
+    the calls with their launch gaps, without the Python and ctypes cost of
+    issuing each launch from a loop."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    del g
+    return dt
+
+
 def single_field_bw512(dev, pairs):
     """One power-law field at bw=512 (1024x1024 grid): the FP64 form called
     per field streams both FP64 tables from HBM on every pair; the FP32, mixed
@@ -194,10 +216,15 @@ def single_field_bw512(dev, pairs):
         p64.forward(gre, gim, cr, ci)
         p64.inverse(cr, ci, gre, gim)
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / pairs
+    dt_loop = (time.perf_counter() - t0) / pairs
+
+    def pair():
+        p64.forward(gre, gim, cr, ci)
+        p64.inverse(cr, ci, gre, gim)
+    dt = graph_replay_time(pair, max(pairs, 10))
     tbytes = 2 * 8 * table_doubles(bw)  # forward + transposed table, FP64
-    out = {"bw": bw, "fp64_pair_ms": dt * 1e3, "table_bytes_per_pair": tbytes,
-           "table_stream_GBps": tbytes / dt / 1e9, "errors_vs_fp64": {}}
+    out = {"bw": bw, "fp64_pair_ms": dt * 1e3, "fp64_pair_ms_python_loop": dt_loop * 1e3,
+           "table_bytes_per_pair": tbytes, "table_stream_GBps": tbytes / dt / 1e9, "errors_vs_fp64": {}}
     p64.inverse(c0r, c0i, gre, gim)
     ref_r, ref_i = torch.empty_like(c0r), torch.empty_like(c0i)
     p64.forward(gre, gim, ref_r, ref_i)
@@ -555,7 +582,7 @@ def main():
         pl.close()
 
     # ---- batch-1 FP64 form: one field per call -----------------------------
-    plain = None
+    plain = plain_loop = None
     if args.plain_fields > 0:
         pf = min(args.plain_fields, F)
         p1 = S.Plan(bw, 1, precision=S.FP64)
@@ -568,7 +595,15 @@ def main():
             p1.forward(gre[f:f + 1], gim[f:f + 1], cre[f:f + 1], cim[f:f + 1])
             p1.inverse(cre[f:f + 1], cim[f:f + 1], ore[f:f + 1], oim[f:f + 1])
         torch.cuda.synchronize()
-        plain = pf / (time.perf_counter() - t1)
+        plain_loop = pf / (time.perf_counter() - t1)
+        views = [(gre[f:f + 1], gim[f:f + 1], cre[f:f + 1], cim[f:f + 1], ore[f:f + 1], oim[f:f + 1])
+                 for f in range(pf)]
+
+        def per_field():
+            for g0, g1, c0, c1, o0, o1 in views:
+                p1.forward(g0, g1, c0, c1)
+                p1.inverse(c0, c1, o0, o1)
+        plain = pf / graph_replay_time(per_field, 5)
         p1.close()
 
     # ---- the drop-ins on host buffers (PCIe included) ------------------------
@@ -631,6 +666,7 @@ def main():
         "other_forms": forms,
         "speedup_vs_fp64_batched": (value / forms["fp64"]["pairs_per_s"]) if "fp64" in forms else None,
         "fp64_batch1_pairs_per_s": plain,
+        "fp64_batch1_python_loop_pairs_per_s": plain_loop,
         "speedup_vs_fp64_batch1": (value / plain) if plain else None,
         "single_field_bw512": single,
         "non_pow2_bw255": npow2,

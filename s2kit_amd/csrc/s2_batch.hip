@@ -94,7 +94,9 @@ static void check_plan(int rc) {
     throw HipError{rc == S2AMD_ENOMEM ? hipErrorOutOfMemory : hipErrorUnknown, last_error()};
 }
 
-static bool mixed_ok(int bw) { return bw >= 64 && bw <= 512 && !(bw & (bw - 1)); }
+static bool form_ok(int precision, int bw) {
+    return precision == S2AMD_MIXED ? mixed_supported(bw) : split_supported(bw);
+}
 
 // Launch grid of the copy kernels: enough blocks to stream nn doubles per request.
 static dim3 copy_grid(long nn, int n, int arrays) {
@@ -133,7 +135,7 @@ extern "C" int s2amd_conv_batch_create(s2amd_conv_batch** out, const s2amd_conv_
             const std::vector<int>& idx = kv.second;
             g.bw = kv.first;
             g.n = (int)idx.size();
-            g.precision = (precision != S2AMD_FP64 && mixed_ok(g.bw)) ? precision : S2AMD_FP64;
+            g.precision = (precision != S2AMD_FP64 && form_ok(precision, g.bw)) ? precision : S2AMD_FP64;
             check_plan(s2amd_plan_create(&g.plan, g.bw, g.bw, g.n, g.precision));
             // tile shapes timed at this group's size before the graph fixes them
             check_plan(s2amd_plan_autotune(g.plan, g.n, b->stream));

@@ -63,6 +63,19 @@ __host__ __device__ inline int mix_pos(int t, int x, int nt) {
     return (t < 4 * g4) ? 64 * (t >> 2) + 4 * x + (t & 3) : 64 * g4 + 2 * x + (t & 1);
 }
 __host__ __device__ inline int ilv64(int j) { return (j & ~63) + 4 * (j & 15) + ((j >> 4) & 3); }
+// bw = 32 has one group of 32 folded rings: position 2x + v holds ring 16v + x
+// (one float2 per lane feeds two 16x16 MFMA tiles).
+__host__ __device__ inline int ilv_ring(int j, int B) { return B >= 64 ? ilv64(j) : 2 * (j & 15) + ((j >> 4) & 1); }
+// Folded rings (JW) and 16-ring tiles (JT) of an inverse block: 64 GJ rings,
+// or the 32 rings of bw = 32 for GJ = 0.
+template <int GJ>
+__host__ __device__ constexpr int mix_jw() {
+    return GJ ? 64 * GJ : 32;
+}
+template <int GJ>
+__host__ __device__ constexpr int mix_jt() {
+    return GJ ? 4 * GJ : 2;
+}
 
 // Rings per forward chunk: at least 16 MFMAs per wave between barriers.
 #ifndef S2_MIX_KC_SMALL
@@ -89,7 +102,7 @@ __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restr
         for (int l = m; l < B; ++l) {
             const int p = (l - m) & 1, i = (l - m) >> 1;
             NF[g0.offF + (long)j * W + mix_pos(p * g0.n0 + (i >> 4), i & 15, nt)] = (float)(cur * wj);
-            NI[(p ? g1.offI : g0.offI) + (long)i * B + ilv64(j)] = (float)(cur * 0.39894228040143267794);
+            NI[(p ? g1.offI : g0.offI) + (long)i * B + ilv_ring(j, B)] = (float)(cur * 0.39894228040143267794);
             if (l + 1 < B) {
                 const double nx = l2_an(m, l) * (cur * x) + l2_cn(m, l) * prev;
                 prev = cur;
@@ -244,17 +257,19 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // ---------------------------------------------------------------------------
 // inverse
 // ---------------------------------------------------------------------------
-// A block owns kMixCols columns of one order and JW = 64*GJ of the bw folded rings
-// (jb selects which); chunks of KC degrees per parity.
+// A block owns kMixCols columns of one order and JW = 64*GJ of the bw folded
+// rings (32 for GJ = 0, bw = 32; jb selects which); chunks of KC degrees per
+// parity.
 template <int GJ>
 __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, const double* __restrict__ cim,
                                              const float* __restrict__ NI, const GemmMeta* __restrict__ meta,
                                              float2* __restrict__ X, int B, int F, int fmt, int m, int cz, float* sm) {
-    constexpr int JW = 64 * GJ;
+    constexpr int JW = mix_jw<GJ>(), JT = mix_jt<GJ>();
     constexpr int KC = kMixInvKC, S = KC / 4;
     constexpr int NV = 2 * KC * JW / 4;  // float4 per table chunk (both parities)
-    constexpr int PT = NV / kMixThreads;
-    static_assert(NV % kMixThreads == 0, "table chunk must split evenly over the block");
+    constexpr int PT = (NV + kMixThreads - 1) / kMixThreads;
+    static_assert(NV % kMixThreads == 0 || NV < kMixThreads, "table chunk must split evenly over the block");
+    constexpr bool FULL = NV % kMixThreads == 0;  // else threads tid >= NV carry no table float4
     const int nz = B / JW;
     const int cb = cz / nz, jbase = (cz - cb * nz) * JW;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
@@ -279,7 +294,7 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     const float* tsrc[PT];
 #pragma unroll
     for (int u = 0; u < PT; ++u) {
-        const int e = tid + u * kMixThreads;
+        const int e = FULL ? tid + u * kMixThreads : min(tid, NV - 1);
         const int p = e / (KC * JW / 4), rest = e - p * (KC * JW / 4);
         const int rw = rest / (JW / 4), c4 = rest - rw * (JW / 4);
         tsrc[u] = NI + (p ? g1.offI : g0.offI) + (size_t)rw * B + jbase + 4 * c4;
@@ -303,7 +318,8 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     };
     auto stash = [&](float* tb) __attribute__((always_inline)) {
 #pragma unroll
-        for (int u = 0; u < PT; ++u) reinterpret_cast<fvec4*>(tb)[tid + u * kMixThreads] = tv[u];
+        for (int u = 0; u < PT; ++u)
+            if (FULL || tid < NV) reinterpret_cast<fvec4*>(tb)[tid + u * kMixThreads] = tv[u];
     };
     float a[2][S];
     auto convert = [&]() __attribute__((always_inline)) {
@@ -314,11 +330,11 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
         }
     };
 
-    fvec4 acc[2][4 * GJ];
+    fvec4 acc[2][JT];
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int t = 0; t < 4 * GJ; ++t) acc[p][t] = fvec4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < JT; ++t) acc[p][t] = fvec4{0.f, 0.f, 0.f, 0.f};
     constexpr int TB = 2 * KC * JW;  // floats per table buffer
     fetch(0);
     stash(sm);
@@ -329,13 +345,22 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
         convert();
         const bool more = ch + 1 < nch;
         if (more) fetch(ch + 1);
-        fvec4 tq[2][2][GJ];
+        // table operands of one K step: a float4 feeds 4 ring tiles (a float2
+        // the 2 tiles of a 32-ring block)
+        constexpr int GQ = GJ ? GJ : 1, TV = GJ ? 4 : 2;
+        fvec4 tq[2][2][GQ];
         auto lds = [&](int s, int b) __attribute__((always_inline)) {
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const float* r = tb + (p * KC + kq * S + s) * JW;
+                if constexpr (GJ == 0) {
+                    const fvec2 h = *reinterpret_cast<const fvec2*>(r + 2 * x16);
+                    tq[b][p][0] = fvec4{h.x, h.y, 0.f, 0.f};
+                } else {
 #pragma unroll
-                for (int g = 0; g < GJ; ++g) tq[b][p][g] = *reinterpret_cast<const fvec4*>(r + 64 * g + 4 * x16);
+                    for (int g = 0; g < GJ; ++g)
+                        tq[b][p][g] = *reinterpret_cast<const fvec4*>(r + 64 * g + 4 * x16);
+                }
             }
         };
         lds(0, 0);
@@ -346,18 +371,18 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
 #pragma unroll
             for (int p = 0; p < 2; ++p)
 #pragma unroll
-                for (int g = 0; g < GJ; ++g) {
+                for (int g = 0; g < GQ; ++g) {
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        acc[p][4 * g + v] =
-                            __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][s], tq[b][p][g][v], acc[p][4 * g + v], 0, 0, 0);
+                    for (int v = 0; v < TV; ++v)
+                        acc[p][TV * g + v] =
+                            __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][s], tq[b][p][g][v], acc[p][TV * g + v], 0, 0, 0);
                 }
         }
         if (more) stash(sm + ((ch + 1) & 1) * TB);
         __syncthreads();
     }
 
-    mix_inv_store<4 * GJ>(acc, X, m, cb, jbase, B, F, fmt);
+    mix_inv_store<JT>(acc, X, m, cb, jbase, B, F, fmt);
 }
 
 template <int GJ>
@@ -368,7 +393,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
     float* sm = reinterpret_cast<float*>(smem);
     const int B = 1 << lgB;
     int mp, cz;
-    mix_pair(nb * (B / (64 * GJ)), mp, cz);
+    mix_pair(nb * (B / mix_jw<GJ>()), mp, cz);
     mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, mp, cz, sm);
     __syncthreads();  // the partner order's prologue reuses the table buffers
     mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, B - 1 - mp, cz, sm);
@@ -378,6 +403,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // host side
 // ---------------------------------------------------------------------------
 std::vector<int> mix_gj_choices(int bw) {
+    if (bw < 64) return {0};  // one 32-ring block
     std::vector<int> v{std::min(S2_MIX_GJMAX, bw / 64)};
     for (int gj = 1; gj <= 4 && gj <= bw / 64; gj <<= 1)
         if (gj != v[0]) v.push_back(gj);
@@ -386,7 +412,7 @@ std::vector<int> mix_gj_choices(int bw) {
 
 bool mixed_supported(int bw) {
     const int N = 2 * bw;
-    return bw >= 64 && bw <= 512 && !(N & (N - 1));
+    return bw >= 32 && bw <= 512 && !(N & (N - 1));
 }
 
 void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
@@ -451,13 +477,15 @@ static void mix_fwd_gemm(const DevConsts& c, const GemmTables& g, const float2* 
 static void mix_inv_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, float2* X,
                          int F, Fmt fmt, hipStream_t s) {
     const int B = c.bw;
-    const int gj = g.t_gj ? g.t_gj : std::min(S2_MIX_GJMAX, B / 64);
+    const int gj = B < 64 ? 0 : g.t_gj ? g.t_gj : std::min(S2_MIX_GJMAX, B / 64);
+    const int jw = gj ? 64 * gj : 32;
     const int nb = (4 * F + kMixCols - 1) / kMixCols;
-    const dim3 grid(B / 2 * nb * (B / (64 * gj)));  // order pairs x column blocks x ring ranges
-    const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * 64 * gj);
+    const dim3 grid(B / 2 * nb * (B / jw));  // order pairs x column blocks x ring ranges
+    const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * jw);
     const float* ni = (const float*)g.ai;
     const GemmMeta* meta = g.meta;
     switch (gj) {
+        case 0: launch_lds(k_mix_inv<0>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
         case 1: launch_lds(k_mix_inv<1>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
         case 2: launch_lds(k_mix_inv<2>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
         default: launch_lds(k_mix_inv<4>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
@@ -473,7 +501,12 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
     {
         Span sp(tm, ST_PHI_FWD, s);
         const size_t lds = sizeof(float2) * ((size_t)rpb * (N + 1) + N);
-        if (k_fphi_fwd_nt(rpb, sizeof(float2)))
+        if (rpb > N) {  // bw = 32: the 64 rings of a field in one 256-thread block
+            constexpr int T2 = 256;
+            const size_t l2 = sizeof(float2) * ((size_t)N * (N + 1) + N);
+            launch_lds(k_fphi_fwd<float, V, T2, true>, dim3(1, F), T2, l2, s, re, im, X, lgN,
+                       (const float2*)c.tw32);
+        } else if (k_fphi_fwd_nt(rpb, sizeof(float2)))
             launch_lds(k_fphi_fwd<float, V, kMixFftThreads, true>, dim3(N / rpb, F), kMixFftThreads, lds, s, re, im,
                        X, lgN, (const float2*)c.tw32);
         else
@@ -505,7 +538,12 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
     {
         Span sp(tm, ST_PHI_INV, s);
         const size_t lds = sizeof(float2) * ((size_t)rpb * (N + 1) + N);
-        if ((((uintptr_t)re | (uintptr_t)im) & 15) == 0)
+        if (rpb > N) {  // bw = 32: the 64 rings of a field in one 256-thread block
+            constexpr int T2 = 256;
+            const size_t l2 = sizeof(float2) * ((size_t)N * (N + 1) + N);
+            launch_lds(k_fphi_inv<float, V, T2>, dim3(1, F), T2, l2, s, (const float2*)X, re, im, lgN,
+                       (const float2*)c.tw32);
+        } else if ((((uintptr_t)re | (uintptr_t)im) & 15) == 0)
             launch_lds(k_fphi_inv<float, V, kMixFftThreads, true>, dim3(N / rpb, F), kMixFftThreads, lds, s,
                        (const float2*)X, re, im, lgN, (const float2*)c.tw32);
         else

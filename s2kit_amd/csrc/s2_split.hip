@@ -476,6 +476,11 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+bool split_supported(int bw) {
+    const int N = 2 * bw;
+    return bw >= 64 && bw <= 512 && !(N & (N - 1));
+}
+
 void split_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
     const int B = c.bw;
     g.bw = B;

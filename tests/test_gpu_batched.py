@@ -42,7 +42,8 @@ def to_dev(torch, dev, arrs):
                                                 (90, 30, 0, 0), (45, 45, 1, 0), (28, 28, 0, 0), (21, 7, 1, 0),
                                                 (32, 32, 0, 1), (64, 64, 0, 1), (128, 128, 1, 1), (256, 256, 0, 1),
                                                 (64, 64, 0, 2), (128, 128, 1, 2), (256, 256, 0, 2),
-                                                (256, 256, 1, 2), (64, 64, 0, 3), (128, 128, 1, 3),
+                                                (256, 256, 1, 2), (32, 32, 0, 2), (32, 32, 1, 2), (32, 16, 0, 2),
+                                                (64, 64, 0, 3), (128, 128, 1, 3),
                                                 (256, 256, 0, 3), (256, 256, 1, 3)])
 def test_plan_matches_oracle(env, O, bw, cutoff, fmt, prec):
     S, torch, dev = env
