@@ -46,7 +46,7 @@ typedef enum {
     /* split BF16: the mixed form's pipeline with the Legendre GEMMs on BF16
      * MFMA; each FP32 operand is split into three BF16 pieces and six piece
      * products are accumulated in FP32, which reproduces the FP32 GEMM.
-     * Error ~4e-7 relative.  bw a power of two in [64, 512]. */
+     * Error ~4e-7 relative.  bw a power of two in [32, 512]. */
     S2AMD_SPLIT_BF16 = 3
 } s2amd_precision;
 
@@ -125,8 +125,7 @@ int s2amd_conv(s2amd_plan* plan, const double* sre, const double* sim, const dou
  * copied at creation: their buffers must stay allocated while the batch is
  * used; their contents are read at every _run.  precision: S2AMD_FP64,
  * S2AMD_MIXED or S2AMD_SPLIT_BF16; a group whose bandwidth the last two do not
- * support (mixed: not a power of two in [32, 512]; split BF16: in [64, 512])
- * runs in FP64.  No counterpart in the reference,
+ * support (not a power of two in [32, 512]) runs in FP64.  No counterpart in the reference,
  * which convolves one pair of host grids per call (FST_semi_memo.c:439-518). */
 typedef struct {
     int bw;

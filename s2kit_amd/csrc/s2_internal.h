@@ -230,7 +230,7 @@ void inverse_fast(const DevConsts& c, const GemmTables& g, const double* cre, co
 
 // ---- mixed form (s2_mixed.hip): FP32 ring FFTs + naive DLT on FP32 MFMA ----
 bool mixed_supported(int bw);  // power-of-two bw in [32, 512]
-bool split_supported(int bw);  // power-of-two bw in [64, 512]
+bool split_supported(int bw);  // power-of-two bw in [32, 512]
 void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s);
 void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
                    double* cim, int F, Fmt fmt, void* X, hipStream_t s, StageTimer* tm);

@@ -40,6 +40,17 @@ __device__ __forceinline__ void mix_pair(int nb, int& m, int& cb) {
     m = 8 * q + k;  // < bw/2; its partner is bw-1-m
 }
 
+// Folded rings (JW) and 16-ring tiles (JT) of an inverse block: 64 GJ rings,
+// or the 32 rings of bw = 32 for GJ = 0.
+template <int GJ>
+__host__ __device__ constexpr int mix_jw() {
+    return GJ ? 64 * GJ : 32;
+}
+template <int GJ>
+__host__ __device__ constexpr int mix_jt() {
+    return GJ ? 4 * GJ : 2;
+}
+
 // Data columns of order m: (field, +m or -m row, re or im), cpf per field.
 __host__ __device__ inline int mix_cpf(int m, int fmt) { return (m > 0 && fmt == 0) ? 4 : 2; }
 

@@ -66,16 +66,7 @@ __host__ __device__ inline int ilv64(int j) { return (j & ~63) + 4 * (j & 15) + 
 // bw = 32 has one group of 32 folded rings: position 2x + v holds ring 16v + x
 // (one float2 per lane feeds two 16x16 MFMA tiles).
 __host__ __device__ inline int ilv_ring(int j, int B) { return B >= 64 ? ilv64(j) : 2 * (j & 15) + ((j >> 4) & 1); }
-// Folded rings (JW) and 16-ring tiles (JT) of an inverse block: 64 GJ rings,
-// or the 32 rings of bw = 32 for GJ = 0.
-template <int GJ>
-__host__ __device__ constexpr int mix_jw() {
-    return GJ ? 64 * GJ : 32;
-}
-template <int GJ>
-__host__ __device__ constexpr int mix_jt() {
-    return GJ ? 4 * GJ : 2;
-}
+
 
 // Rings per forward chunk: at least 16 MFMAs per wave between barriers.
 #ifndef S2_MIX_KC_SMALL

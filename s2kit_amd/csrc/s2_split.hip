@@ -340,14 +340,15 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // inverse
 // ---------------------------------------------------------------------------
 // Order m, ring range jr (JT = 4*GJ ring tiles, 64*GJ of the bw folded
-// rings), column blocks cb0, cb0+1, ... (nblk): one stream of chunks of 32
+// rings; 2 tiles, the 32 rings of bw = 32, for GJ = 0), column blocks cb0,
+// cb0+1, ... (nblk): one stream of chunks of 32
 // degree indices of both parities; a stage is one parity's tiles (8 at most).
 template <int GJ>
 __device__ __forceinline__ void split_inv_body(const double* __restrict__ cre, const double* __restrict__ cim,
                                                const __bf16* __restrict__ SI, const GemmMeta* __restrict__ meta,
                                                float2* __restrict__ X, int B, int F, int fmt, int m, int cb0,
                                                int nblk, int jr, unsigned char* sm) {
-    constexpr int JT = 4 * GJ;
+    constexpr int JT = mix_jt<GJ>();
     constexpr int TPS = JT < kSplitStageTiles ? JT : kSplitStageTiles;
     constexpr int SPP = JT / TPS;  // stages per parity
     constexpr int SPC = 2 * SPP;   // stages per chunk
@@ -464,7 +465,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
                                                            int lgB, int F, int fmt, int nbw) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int B = 1 << lgB;
-    const int nz = B / (64 * GJ);
+    const int nz = B / mix_jw<GJ>();
     int mp, cz;
     mix_pair(nbw * nz, mp, cz);
     const int cw = cz / nz, jr = cz - cw * nz;
@@ -478,7 +479,7 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // ---------------------------------------------------------------------------
 bool split_supported(int bw) {
     const int N = 2 * bw;
-    return bw >= 64 && bw <= 512 && !(N & (N - 1));
+    return bw >= 32 && bw <= 512 && !(N & (N - 1));
 }
 
 void split_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
@@ -537,14 +538,16 @@ void split_fwd_gemm(const DevConsts& c, const GemmTables& g, const float2* X, do
 void split_inv_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, float2* X, int F,
                     Fmt fmt, hipStream_t s) {
     const int B = c.bw;
-    const int gj = std::min(S2_SPLIT_GJMAX, B / 64);
+    const int gj = B < 64 ? 0 : std::min(S2_SPLIT_GJMAX, B / 64);
+    const int jw = gj ? 64 * gj : 32, jt = jw / 16;
     const int nb = (4 * F + kMixCols - 1) / kMixCols;
     const int nbw = (nb + kSplitBlocks - 1) / kSplitBlocks;
-    const dim3 grid(B / 2 * nbw * (B / (64 * gj)));  // order pairs x groups of column blocks x ring ranges
-    const size_t lds = 2 * (size_t)std::min(4 * gj, kSplitStageTiles) * kSplitTile;
+    const dim3 grid(B / 2 * nbw * (B / jw));  // order pairs x groups of column blocks x ring ranges
+    const size_t lds = 2 * (size_t)std::min(jt, kSplitStageTiles) * kSplitTile;
     const __bf16* si = (const __bf16*)g.ai;
     const GemmMeta* meta = g.meta;
     switch (gj) {
+        case 0: launch_lds(k_split_inv<0>, grid, kMixThreads, lds, s, cre, cim, si, meta, X, lg2(B), F, (int)fmt, nbw); break;
         case 1: launch_lds(k_split_inv<1>, grid, kMixThreads, lds, s, cre, cim, si, meta, X, lg2(B), F, (int)fmt, nbw); break;
         case 2: launch_lds(k_split_inv<2>, grid, kMixThreads, lds, s, cre, cim, si, meta, X, lg2(B), F, (int)fmt, nbw); break;
         default: launch_lds(k_split_inv<4>, grid, kMixThreads, lds, s, cre, cim, si, meta, X, lg2(B), F, (int)fmt, nbw); break;

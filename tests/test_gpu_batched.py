@@ -43,7 +43,7 @@ def to_dev(torch, dev, arrs):
                                                 (32, 32, 0, 1), (64, 64, 0, 1), (128, 128, 1, 1), (256, 256, 0, 1),
                                                 (64, 64, 0, 2), (128, 128, 1, 2), (256, 256, 0, 2),
                                                 (256, 256, 1, 2), (32, 32, 0, 2), (32, 32, 1, 2), (32, 16, 0, 2),
-                                                (64, 64, 0, 3), (128, 128, 1, 3),
+                                                (32, 32, 0, 3), (32, 32, 1, 3), (64, 64, 0, 3), (128, 128, 1, 3),
                                                 (256, 256, 0, 3), (256, 256, 1, 3)])
 def test_plan_matches_oracle(env, O, bw, cutoff, fmt, prec):
     S, torch, dev = env

@@ -5,8 +5,8 @@ go through the gather/scatter kernels, and back-to-back buffers, used in
 place), and across graph replays with eager work in between.
 
 FP64 batches: 1e-11 relative (normwise) per request.  Mixed and split-BF16
-batches: 1e-6 for bandwidths those forms support (mixed from bw = 32, split
-BF16 from bw = 64); other bandwidths run in FP64 and keep the FP64 bound.
+batches: 1e-6 for bandwidths those forms support (powers of two from bw =
+32); other bandwidths run in FP64 and keep the FP64 bound.
 """
 import numpy as np
 import pytest
@@ -86,8 +86,8 @@ def test_conv_batch_matches_oracle(env, O, prec):
     groups = batch.groups()
     assert [g[0] for g in groups] == [16, 32, 64, 128]
     assert [g[1] for g in groups] == [1, 2, 3, 2]
-    # the mixed form covers bw >= 32, split BF16 bw >= 64; other groups run in FP64
-    low = {0: 1 << 30, 2: 32, 3: 64}[prec]
+    # the mixed and split-BF16 forms cover bw >= 32; other groups run in FP64
+    low = {0: 1 << 30, 2: 32, 3: 32}[prec]
     assert [g[2] for g in groups] == [prec if bw >= low else 0 for bw in (16, 32, 64, 128)]
 
     def tol(bw):
