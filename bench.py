@@ -134,7 +134,11 @@ MFMA_PEAK = {"fp64": (78.6, 1), "fp32": (157.3, 1), "mixed": (157.3, 1), "split"
 def non_pow2(dev, steps):
     """FP64 plans at bw=255 (510 = 2*3*5*17 rings: mixed-radix FFTs and DCTs,
     MFMA Legendre GEMMs) and bw=256, 256 fields each; the bw=255 batch is
-    checked against the drop-ins (per-field kernels) on its first field."""
+    checked against the drop-ins (per-field kernels) on its first field.  The
+    mixed form (FP32 ring FFTs, mixed-radix at bw=255, naive GEMMs over rings
+    padded to whole 64-ring groups) is timed on the same fields, with its
+    errors against the FP64 plan at cutoff 0 (the all-naive transform, which
+    the mixed form computes: at odd bw it synthesises degree bw-1 whole)."""
     import torch
     import s2kit_amd as S
     from s2kit_amd import synth
@@ -165,7 +169,27 @@ def non_pow2(dev, steps):
             g = (gre[0] + 1j * gim[0]).cpu().numpy().ravel()
             out["bw255_inverse_vs_per_field"] = float(np.abs(g - (g1r + 1j * g1i)).max() / np.abs(g1r + 1j * g1i).max())
         p.close()
-        del gre, gim
+        # the mixed form on the same fields
+        pr = S.Plan(bw, F, precision=S.FP64, cutoff=0)
+        pr.inverse(c0r, c0i, gre, gim)
+        pr.forward(gre, gim, cr, ci)
+        pr.close()
+        pm = S.Plan(bw, F, precision=S.MIXED)
+        gmr, gmi = torch.empty_like(gre), torch.empty_like(gim)
+        cmr, cmi = torch.empty_like(cr), torch.empty_like(ci)
+        pm.inverse(c0r, c0i, gmr, gmi)
+        pm.forward(gre, gim, cmr, cmi)
+        torch.cuda.synchronize()
+        out[f"bw{bw}_mixed_inverse_vs_fp64"] = float(torch.hypot(gmr - gre, gmi - gim).max() / torch.hypot(gre, gim).max())
+        out[f"bw{bw}_mixed_forward_vs_fp64"] = float(torch.hypot(cmr - cr, cmi - ci).max() / torch.hypot(cr, ci).max())
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pm.inverse(c0r, c0i, gmr, gmi)
+            pm.forward(gmr, gmi, cmr, cmi)
+        torch.cuda.synchronize()
+        out[f"bw{bw}_mixed_pairs_per_s"] = F * steps / (time.perf_counter() - t0)
+        pm.close()
+        del gre, gim, gmr, gmi
         torch.cuda.empty_cache()
     return out
 

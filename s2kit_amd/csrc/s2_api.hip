@@ -1050,7 +1050,7 @@ extern "C" int s2amd_plan_create(s2amd_plan** out, int bw, int cutoff, int max_f
             precision != S2AMD_SPLIT_BF16)
             throw ArgError{"unknown precision"};
         if (precision == S2AMD_MIXED && !mixed_supported(bw))
-            throw ArgError{"the mixed form needs a power-of-two grid with bw in [32, 512]"};
+            throw ArgError{"the mixed form needs bw in [32, 512]"};
         if (precision == S2AMD_SPLIT_BF16 && !split_supported(bw))
             throw ArgError{"the split-BF16 form needs a power-of-two grid with bw in [32, 512]"};
         if (precision == S2AMD_FP32 && !fast_supported(bw, cutoff, 1))

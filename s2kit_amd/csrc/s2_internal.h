@@ -229,7 +229,10 @@ void inverse_fast(const DevConsts& c, const GemmTables& g, const double* cre, co
                   double* im, int F, Fmt fmt, void* X, void* Cb, hipStream_t s, StageTimer* tm);
 
 // ---- mixed form (s2_mixed.hip): FP32 ring FFTs + naive DLT on FP32 MFMA ----
-bool mixed_supported(int bw);  // power-of-two bw in [32, 512]
+// FP32 ring FFTs on the mixed-radix kernels (ring lengths that are not powers of two)
+void ring_fft_fwd_f32(const DevConsts& c, const double* re, const double* im, float2* X, int F, hipStream_t s);
+void ring_fft_inv_f32(const DevConsts& c, const float2* X, double* re, double* im, int F, hipStream_t s);
+bool mixed_supported(int bw);  // bw in [32, 512]
 bool split_supported(int bw);  // power-of-two bw in [32, 512]
 void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s);
 void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,

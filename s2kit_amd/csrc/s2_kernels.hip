@@ -122,10 +122,13 @@ __device__ __forceinline__ void mr_pass(const T* a, T* b, int N, int Ns, int nff
 // twiddles are applied first, in place, once per element (each pair item
 // would otherwise twiddle all R inputs: at R = 17 the sweep saves about 8% of
 // a B=255 FFT/DCT launch; at R = 5 its extra LDS pass and barrier cost 8-12%);
-// smaller R twiddle inside the items.  Powers of wk and w by repeated
-// multiplication (relative error ~q ulp).
+// smaller R twiddle inside the items.  FP64: powers of wk and w by repeated
+// multiplication (relative error ~q ulp).  FP32: every power read from the
+// table (rounded once), since q ulp of FP32 would reach the 1e-6 the FP32
+// forms are held to.
 template <typename T, int SIGN>
 __device__ __forceinline__ void mr_pass_odd(T* a, T* b, int N, int Ns, int R, int nfft, const T* __restrict__ tw) {
+    using F = typename Real<T>::type;
     const int NR = N / R, step = N / (Ns * R), H = (R - 1) >> 1, H1 = H + 1;
     const unsigned long long mH1 = magic16(H1), mNR = magic16(NR), mNs = magic16(Ns), mN = magic16(N);
     const bool sweep = R >= 11 && Ns > 1;
@@ -156,9 +159,26 @@ __device__ __forceinline__ void mr_pass_odd(T* a, T* b, int N, int Ns, int R, in
             om.y = -om.y;
         }
         T wq = wk, oq = om;  // oq = w^{sq}
+        int tq = k * step, ts = sp_;  // FP32: table indices of wk^q (mod N) and w^{sq} (mod R)
         for (int q = 1; q <= H; ++q) {
             T vq = in[q * NR], vr = in[(R - q) * NR];
-            if (!sweep) {  // wk^q and wk^{R-q} = wk^R conj(wk^q)
+            if constexpr (sizeof(F) == 4) {
+                if (!sweep) {
+                    T a = tw[tq], b = tw[tq == 0 ? 0 : N - tq];  // wk^q and wk^{-q}; wk^{R-q} = wk^R wk^{-q}
+                    if (SIGN > 0) {
+                        a.y = -a.y;
+                        b.y = -b.y;
+                    }
+                    vq = cmul(vq, a);
+                    vr = cmul(vr, cmul(WR, b));
+                    tq += k * step;
+                    if (tq >= N) tq -= N;
+                }
+                oq = tw[ts * NR];
+                if (SIGN > 0) oq.y = -oq.y;
+                ts += sp_;
+                if (ts >= R) ts -= R;
+            } else if (!sweep) {  // wk^q and wk^{R-q} = wk^R conj(wk^q)
                 vq = cmul(vq, wq);
                 vr = cmul(vr, cmul(WR, cconj(wq)));
                 wq = cmul(wq, wk);
@@ -168,7 +188,7 @@ __device__ __forceinline__ void mr_pass_odd(T* a, T* b, int N, int Ns, int R, in
             acc1.y += su.y * oq.x + df.x * oq.y;
             acc2.x += su.x * oq.x + df.y * oq.y;
             acc2.y += su.y * oq.x - df.x * oq.y;
-            oq = cmul(oq, om);
+            if constexpr (sizeof(F) != 4) oq = cmul(oq, om);
         }
         T* out = b + f * N + jq * Ns * R + k;
         if (sp_ == 0) {
@@ -1532,6 +1552,27 @@ static void rows_launch(int B, int nf, bool wide, L&& launch, P&& pick) {
 static bool reg_rings(const DevConsts& c) { return c.pow2 && c.N >= 64 && c.N <= 2048; }
 static int reg_rpb(int N) { return 8 * kThreads / N; }
 static size_t reg_lds(int N) { return sizeof(double2) * ((size_t)reg_rpb(N) * (N + 1) + N); }
+
+// FP32 ring FFTs of grids whose ring length is not a power of two (the mixed
+// form's first and last stages at such bandwidths), on the mixed-radix
+// instances; same order-major FP32 intermediate X as the register kernels.
+void ring_fft_fwd_f32(const DevConsts& c, const double* re, const double* im, float2* X, int F, hipStream_t s) {
+    const int N = c.N;
+    const int rpb = rings_per_block(N, false);
+    const size_t lds = 2 * sizeof(float2) * rpb * N;
+    fft_launch(false, k_phi_fwd<float2, false>, k_phi_fwd<float2, true>, dim3((N + rpb - 1) / rpb, F), lds, N, s,
+               re, im, X, N, c.logN, 0, rpb, c.tw32);
+    S2_HIP(hipGetLastError());
+}
+
+void ring_fft_inv_f32(const DevConsts& c, const float2* X, double* re, double* im, int F, hipStream_t s) {
+    const int N = c.N;
+    const int rpb = rings_per_block(N, false);
+    const size_t lds = 2 * sizeof(float2) * rpb * N;
+    fft_launch(false, k_phi_inv<float2, false>, k_phi_inv<float2, true>, dim3((N + rpb - 1) / rpb, F), lds, N, s,
+               X, re, im, N, c.logN, 0, c.bw, rpb, c.tw32);
+    S2_HIP(hipGetLastError());
+}
 
 void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, const double* im, double* cre,
                   double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {

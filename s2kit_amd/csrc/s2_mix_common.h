@@ -33,12 +33,21 @@ __host__ __device__ inline int mix_n0(int m, int B) { return (gm_lp(m, 0, B) + 1
 // round-robin over the 8 XCDs (id mod 8); XCD k takes the pairs m = 8q+k, all
 // column blocks of one pair back to back, so the pair's table tiles stay in
 // that XCD's L2.  bw % 16 == 0.
-__device__ __forceinline__ void mix_pair(int nb, int& m, int& cb) {
+__device__ __forceinline__ void mix_pair(int nb, int& m, int& cb) {  // grid: mix_npairs8(bw) * nb
     const int id = blockIdx.x, k = id & 7, local = id >> 3;
     const int q = local / nb;
     cb = local - q * nb;
     m = 8 * q + k;  // < bw/2; its partner is bw-1-m
 }
+
+// Folded rings the mixed form's GEMMs run over: bw rounded up to whole
+// 64-ring groups (32 at bw = 32); the tables are zero past bw, and the
+// inverse stores no ring past bw.  Equal to bw for powers of two.
+__host__ __device__ inline int mix_bp(int B) { return B <= 32 ? 32 : (B + 63) & ~63; }
+// Order pairs (m, bw-1-m) of a GEMM launch, ceil(bw/2) (the middle order of
+// an odd bw pairs with itself), rounded up to the 8 XCDs of mix_pair.
+__host__ __device__ inline int mix_npairs(int B) { return (B + 1) / 2; }
+__host__ __device__ inline int mix_npairs8(int B) { return (mix_npairs(B) + 7) & ~7; }
 
 // Folded rings (JW) and 16-ring tiles (JT) of an inverse block: 64 GJ rings,
 // or the 32 rings of bw = 32 for GJ = 0.
@@ -104,8 +113,9 @@ __device__ __forceinline__ void mix_fwd_store(const fvec4 (&acc)[2 * N0], double
 // Inverse epilogue: acc[p][t] holds the parity-p sums E (p = 0) and O
 // (p = 1) at rings jbase + 16t + x16 of columns 4(lane/16) + r (the table
 // carries 1/sqrt(2pi), the operand the row sign); rows r, r+1 are the re, im
-// parts of one (field, sign) row.  Ring j <- E+O, ring N-1-j <- E-O, and for
-// REAL data the conjugate mirror row N-m.
+// parts of one (field, sign) row.  Ring j <- E+O, ring N-1-j <- E-O (j < bw:
+// padded rings are not stored), and for REAL data the conjugate mirror row
+// N-m.
 template <int JT>
 __device__ __forceinline__ void mix_inv_store(const fvec4 (&acc)[2][JT], float2* __restrict__ X, int m, int cb,
                                               int jbase, int B, int F, int fmt) {
@@ -126,6 +136,7 @@ __device__ __forceinline__ void mix_inv_store(const fvec4 (&acc)[2][JT], float2*
         const int r = 2 * rr;
 #pragma unroll
         for (int t = 0; t < JT; ++t) {
+            if (jbase + 16 * t + x16 >= B) continue;
             const fvec4 e = acc[0][t], o = acc[1][t];
             plo[16 * t] = make_float2(e[r] + o[r], e[r + 1] + o[r + 1]);
             phi[-16 * t] = make_float2(e[r] - o[r], e[r + 1] - o[r + 1]);
@@ -135,6 +146,7 @@ __device__ __forceinline__ void mix_inv_store(const fvec4 (&acc)[2][JT], float2*
             float2* qhi = xm + (N - 1 - jbase - x16);
 #pragma unroll
             for (int t = 0; t < JT; ++t) {
+                if (jbase + 16 * t + x16 >= B) continue;
                 const fvec4 e = acc[0][t], o = acc[1][t];
                 qlo[16 * t] = make_float2(e[r] + o[r], -(e[r + 1] + o[r + 1]));
                 qhi[-16 * t] = make_float2(e[r] - o[r], -(e[r + 1] - o[r + 1]));

@@ -79,6 +79,7 @@ __host__ __device__ constexpr int mix_kc(int n0) { return n0 <= 2 ? S2_MIX_KC_SM
 
 __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restrict__ meta, float* __restrict__ NF,
                            float* __restrict__ NI, int B) {
+    const int Bp = mix_bp(B);
     const int m = blockIdx.x;
     const int N = 2 * B;
     const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
@@ -93,7 +94,7 @@ __global__ void k_mix_pack(const double* __restrict__ w, const GemmMeta* __restr
         for (int l = m; l < B; ++l) {
             const int p = (l - m) & 1, i = (l - m) >> 1;
             NF[g0.offF + (long)j * W + mix_pos(p * g0.n0 + (i >> 4), i & 15, nt)] = (float)(cur * wj);
-            NI[(p ? g1.offI : g0.offI) + (long)i * B + ilv_ring(j, B)] = (float)(cur * 0.39894228040143267794);
+            NI[(p ? g1.offI : g0.offI) + (long)i * Bp + ilv_ring(j, Bp)] = (float)(cur * 0.39894228040143267794);
             if (l + 1 < B) {
                 const double nx = l2_an(m, l) * (cur * x) + l2_cn(m, l) * prev;
                 prev = cur;
@@ -178,7 +179,7 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
     fetch(0);
     stash(sm);
     __syncthreads();
-    const int nch = B / KC;
+    const int nch = mix_bp(B) / KC;  // rings past bw meet zero table rows
     for (int ch = 0; ch < nch; ++ch) {
         const float* tb = sm + (ch & 1) * TB + kq * S * W;  // this lane's table rows: ring kq*S + s
         fold();
@@ -209,13 +210,14 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
 template <int MAXN0>
 __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_MIX_WPE_FWD))) void k_mix_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
                                                         const GemmMeta* __restrict__ meta, double* __restrict__ cre,
-                                                        double* __restrict__ cim, int lgB, int F, int fmt, int nb) {
+                                                        double* __restrict__ cim, int B, int F, int fmt, int nb) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* sm = reinterpret_cast<float*>(smem);
-    const int B = 1 << lgB;
     int mp, cb;
     mix_pair(nb, mp, cb);
+    if (mp >= mix_npairs(B)) return;
     for (int h = 0; h < 2; ++h) {
+        if (h && B - 1 - mp == mp) break;  // odd bw: the middle order pairs with itself
         const int m = h ? B - 1 - mp : mp;
         const GemmMeta g0 = meta[m * 2], g1 = meta[m * 2 + 1];
 #define S2_MIX_FWD(n) \
@@ -261,7 +263,8 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     constexpr int PT = (NV + kMixThreads - 1) / kMixThreads;
     static_assert(NV % kMixThreads == 0 || NV < kMixThreads, "table chunk must split evenly over the block");
     constexpr bool FULL = NV % kMixThreads == 0;  // else threads tid >= NV carry no table float4
-    const int nz = B / JW;
+    const int Bp = mix_bp(B);  // rings of the table rows (zero past bw)
+    const int nz = Bp / JW;
     const int cb = cz / nz, jbase = (cz - cb * nz) * JW;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
@@ -288,7 +291,7 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
         const int e = FULL ? tid + u * kMixThreads : min(tid, NV - 1);
         const int p = e / (KC * JW / 4), rest = e - p * (KC * JW / 4);
         const int rw = rest / (JW / 4), c4 = rest - rw * (JW / 4);
-        tsrc[u] = NI + (p ? g1.offI : g0.offI) + (size_t)rw * B + jbase + 4 * c4;
+        tsrc[u] = NI + (p ? g1.offI : g0.offI) + (size_t)rw * Bp + jbase + 4 * c4;
     }
     // rows -m of odd m carry the sign (-1)^m: applied to the operand
     const unsigned nsign = ((((wave * 16 + x16) % cpf) >> 1) && (m & 1)) ? 0x80000000u : 0u;
@@ -296,7 +299,7 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     fvec4 tv[PT];
     auto fetch = [&](int ch) __attribute__((always_inline)) {
 #pragma unroll
-        for (int u = 0; u < PT; ++u) tv[u] = *reinterpret_cast<const fvec4*>(tsrc[u] + (size_t)ch * KC * B);
+        for (int u = 0; u < PT; ++u) tv[u] = *reinterpret_cast<const fvec4*>(tsrc[u] + (size_t)ch * KC * Bp);
         const int d0 = 2 * (ch * KC + kq * S);
         if (d0 + 2 * S <= L) {
 #pragma unroll
@@ -379,13 +382,14 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
 template <int GJ>
 __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_MIX_WPE_INV))) void k_mix_inv(
     const double* __restrict__ cre, const double* __restrict__ cim, const float* __restrict__ NI,
-    const GemmMeta* __restrict__ meta, float2* __restrict__ X, int lgB, int F, int fmt, int nb) {
+    const GemmMeta* __restrict__ meta, float2* __restrict__ X, int B, int F, int fmt, int nb) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* sm = reinterpret_cast<float*>(smem);
-    const int B = 1 << lgB;
     int mp, cz;
-    mix_pair(nb * (B / mix_jw<GJ>()), mp, cz);
+    mix_pair(nb * (mix_bp(B) / mix_jw<GJ>()), mp, cz);
+    if (mp >= mix_npairs(B)) return;
     mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, mp, cz, sm);
+    if (B - 1 - mp == mp) return;  // odd bw: the middle order pairs with itself
     __syncthreads();  // the partner order's prologue reuses the table buffers
     mix_inv_body<GJ>(cre, cim, NI, meta, X, B, F, fmt, B - 1 - mp, cz, sm);
 }
@@ -393,18 +397,24 @@ __global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// 64-ring groups per inverse block that divide the padded ring count,
+// the default (the largest) first.
+static int mix_gj_default(int bw) {
+    const int g = mix_bp(bw) / 64;
+    if (g == 0) return 0;  // one 32-ring block
+    int gj = 1;
+    while (2 * gj <= S2_MIX_GJMAX && g % (2 * gj) == 0) gj *= 2;
+    return gj;
+}
+
 std::vector<int> mix_gj_choices(int bw) {
-    if (bw < 64) return {0};  // one 32-ring block
-    std::vector<int> v{std::min(S2_MIX_GJMAX, bw / 64)};
-    for (int gj = 1; gj <= 4 && gj <= bw / 64; gj <<= 1)
-        if (gj != v[0]) v.push_back(gj);
+    std::vector<int> v{mix_gj_default(bw)};
+    if (v[0] == 0) return v;
+    for (int gj = 1; gj < v[0]; gj <<= 1) v.push_back(gj);
     return v;
 }
 
-bool mixed_supported(int bw) {
-    const int N = 2 * bw;
-    return bw >= 32 && bw <= 512 && !(N & (N - 1));
-}
+bool mixed_supported(int bw) { return bw >= 32 && bw <= 512; }
 
 void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
     const int B = c.bw;
@@ -422,9 +432,9 @@ void naive_tables_create(GemmTables& g, const DevConsts& c, hipStream_t s) {
             e.ki = ki;
             e.offF = of;
             e.offI = oi;
-            oi += (long)ki * B;
+            oi += (long)ki * mix_bp(B);
         }
-        of += (long)B * 32 * n0;
+        of += (long)mix_bp(B) * 32 * n0;
     }
     g.nf = of;
     g.ni = oi;
@@ -450,36 +460,36 @@ static void mix_fwd_gemm(const DevConsts& c, const GemmTables& g, const float2* 
                          Fmt fmt, hipStream_t s) {
     const int B = c.bw;
     const int nb = (4 * F + kMixCols - 1) / kMixCols;
-    const dim3 grid(B / 2 * nb);  // order pairs x column blocks
+    const dim3 grid(mix_npairs8(B) * nb);  // order pairs x column blocks
     const size_t lds = mix_fwd_lds(B);
     const float* nf = (const float*)g.af;
     const GemmMeta* meta = g.meta;
     const int n0max = mix_n0(0, B);
     if (n0max <= 2)
-        launch_lds(k_mix_fwd<2>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+        launch_lds(k_mix_fwd<2>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, B, F, (int)fmt, nb);
     else if (n0max <= 4)
-        launch_lds(k_mix_fwd<4>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+        launch_lds(k_mix_fwd<4>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, B, F, (int)fmt, nb);
     else if (n0max <= 8)
-        launch_lds(k_mix_fwd<8>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+        launch_lds(k_mix_fwd<8>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, B, F, (int)fmt, nb);
     else
-        launch_lds(k_mix_fwd<16>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, lg2(B), F, (int)fmt, nb);
+        launch_lds(k_mix_fwd<16>, grid, kMixThreads, lds, s, X, nf, meta, cre, cim, B, F, (int)fmt, nb);
 }
 
 static void mix_inv_gemm(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, float2* X,
                          int F, Fmt fmt, hipStream_t s) {
     const int B = c.bw;
-    const int gj = B < 64 ? 0 : g.t_gj ? g.t_gj : std::min(S2_MIX_GJMAX, B / 64);
+    const int gj = B <= 32 ? 0 : g.t_gj ? g.t_gj : mix_gj_default(B);
     const int jw = gj ? 64 * gj : 32;
     const int nb = (4 * F + kMixCols - 1) / kMixCols;
-    const dim3 grid(B / 2 * nb * (B / jw));  // order pairs x column blocks x ring ranges
+    const dim3 grid(mix_npairs8(B) * nb * (mix_bp(B) / jw));  // order pairs x column blocks x ring ranges
     const size_t lds = sizeof(float) * 2 * (2 * kMixInvKC * jw);
     const float* ni = (const float*)g.ai;
     const GemmMeta* meta = g.meta;
     switch (gj) {
-        case 0: launch_lds(k_mix_inv<0>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
-        case 1: launch_lds(k_mix_inv<1>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
-        case 2: launch_lds(k_mix_inv<2>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
-        default: launch_lds(k_mix_inv<4>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, lg2(B), F, (int)fmt, nb); break;
+        case 0: launch_lds(k_mix_inv<0>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, B, F, (int)fmt, nb); break;
+        case 1: launch_lds(k_mix_inv<1>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, B, F, (int)fmt, nb); break;
+        case 2: launch_lds(k_mix_inv<2>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, B, F, (int)fmt, nb); break;
+        default: launch_lds(k_mix_inv<4>, grid, kMixThreads, lds, s, cre, cim, ni, meta, X, B, F, (int)fmt, nb); break;
     }
 }
 
@@ -492,7 +502,9 @@ void forward_mixed(const DevConsts& c, const GemmTables& g, const double* re, co
     {
         Span sp(tm, ST_PHI_FWD, s);
         const size_t lds = sizeof(float2) * ((size_t)rpb * (N + 1) + N);
-        if (rpb > N) {  // bw = 32: the 64 rings of a field in one 256-thread block
+        if (!c.pow2) {  // mixed-radix FP32 ring FFTs
+            ring_fft_fwd_f32(c, re, im, X, F, s);
+        } else if (rpb > N) {  // bw = 32: the 64 rings of a field in one 256-thread block
             constexpr int T2 = 256;
             const size_t l2 = sizeof(float2) * ((size_t)N * (N + 1) + N);
             launch_lds(k_fphi_fwd<float, V, T2, true>, dim3(1, F), T2, l2, s, re, im, X, lgN,
@@ -529,7 +541,9 @@ void inverse_mixed(const DevConsts& c, const GemmTables& g, const double* cre, c
     {
         Span sp(tm, ST_PHI_INV, s);
         const size_t lds = sizeof(float2) * ((size_t)rpb * (N + 1) + N);
-        if (rpb > N) {  // bw = 32: the 64 rings of a field in one 256-thread block
+        if (!c.pow2) {  // mixed-radix FP32 ring FFTs
+            ring_fft_inv_f32(c, X, re, im, F, s);
+        } else if (rpb > N) {  // bw = 32: the 64 rings of a field in one 256-thread block
             constexpr int T2 = 256;
             const size_t l2 = sizeof(float2) * ((size_t)N * (N + 1) + N);
             launch_lds(k_fphi_inv<float, V, T2>, dim3(1, F), T2, l2, s, (const float2*)X, re, im, lgN,

@@ -104,8 +104,10 @@ def test_plan_rejects_bad_arguments(S):
     assert L.s2amd_plan_create(ctypes.byref(h), 16, 16, 0, 0) == -1
     assert L.s2amd_plan_create(ctypes.byref(h), 16, 16, 4, 9) == -1
     assert "precision" in S.last_error()
-    for prec, name in ((S.MIXED, "mixed"), (S.SPLIT_BF16, "split-BF16")):
-        assert L.s2amd_plan_create(ctypes.byref(h), 96, 96, 4, prec) == -1
+    # mixed: any bw in [32, 512]; split BF16: powers of two in [32, 512]
+    for bw, prec, name in ((16, S.MIXED, "mixed"), (600, S.MIXED, "mixed"), (96, S.SPLIT_BF16, "split-BF16"),
+                           (16, S.SPLIT_BF16, "split-BF16")):
+        assert L.s2amd_plan_create(ctypes.byref(h), bw, bw, 4, prec) == -1
         assert name in S.last_error()
     assert L.s2amd_forward(None, None, None, None, None, 1, 0, None) == -1
     assert "plan is NULL" in S.last_error()
