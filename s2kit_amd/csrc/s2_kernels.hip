@@ -1556,9 +1556,22 @@ static size_t reg_lds(int N) { return sizeof(double2) * ((size_t)reg_rpb(N) * (N
 // FP32 ring FFTs of grids whose ring length is not a power of two (the mixed
 // form's first and last stages at such bandwidths), on the mixed-radix
 // instances; same order-major FP32 intermediate X as the register kernels.
+#ifndef S2_MR_ELEMS32
+#define S2_MR_ELEMS32 4096
+#endif
+// Rings per block of the FP32 mixed-radix ring FFTs: 4096 values (8 rings at
+// bw = 255; FP32 values leave room for twice the FP64 blocks' rings).
+// Measured at bw = 255 x 1024 fields: 2048 values 2.92 / 2.95 ms per forward /
+// inverse FFT launch, 4096 values 2.67 / 2.85 ms, 8192 values 3.89 / 4.21 ms
+// (one block per CU); 1024-thread blocks slower at both sizes.
+static int rings_per_block_f32(int N) {
+    const int cap = (163840 / (int)sizeof(float2) - N) / (2 * N);
+    return std::max(1, std::min(S2_MR_ELEMS32 / N, cap));
+}
+
 void ring_fft_fwd_f32(const DevConsts& c, const double* re, const double* im, float2* X, int F, hipStream_t s) {
     const int N = c.N;
-    const int rpb = rings_per_block(N, false);
+    const int rpb = rings_per_block_f32(N);
     const size_t lds = 2 * sizeof(float2) * rpb * N;
     fft_launch(false, k_phi_fwd<float2, false>, k_phi_fwd<float2, true>, dim3((N + rpb - 1) / rpb, F), lds, N, s,
                re, im, X, N, c.logN, 0, rpb, c.tw32);
@@ -1567,7 +1580,7 @@ void ring_fft_fwd_f32(const DevConsts& c, const double* re, const double* im, fl
 
 void ring_fft_inv_f32(const DevConsts& c, const float2* X, double* re, double* im, int F, hipStream_t s) {
     const int N = c.N;
-    const int rpb = rings_per_block(N, false);
+    const int rpb = rings_per_block_f32(N);
     const size_t lds = 2 * sizeof(float2) * rpb * N;
     fft_launch(false, k_phi_inv<float2, false>, k_phi_inv<float2, true>, dim3((N + rpb - 1) / rpb, F), lds, N, s,
                X, re, im, N, c.logN, 0, c.bw, rpb, c.tw32);
