@@ -41,7 +41,8 @@ typedef enum {
     S2AMD_FP32 = 1,
     /* mixed: FP32 ring FFTs and the naive Legendre transform (naive.c) as an
      * FP32-MFMA GEMM over m on tables from the FP64 recurrence; FP64 in/out.
-     * Error ~4e-7 relative.  bw a power of two in [32, 512]. */
+     * Error ~4e-7 relative.  Any bw in [32, 512] (grids that are not powers
+     * of two on FP32 mixed-radix FFTs). */
     S2AMD_MIXED = 2,
     /* split BF16: the mixed form's pipeline with the Legendre GEMMs on BF16
      * MFMA; each FP32 operand is split into three BF16 pieces and six piece
@@ -124,8 +125,9 @@ int s2amd_conv(s2amd_plan* plan, const double* sre, const double* sim, const dou
  * outputs -- as one HIP graph; _run replays it on a stream.  Requests are
  * copied at creation: their buffers must stay allocated while the batch is
  * used; their contents are read at every _run.  precision: S2AMD_FP64,
- * S2AMD_MIXED or S2AMD_SPLIT_BF16; a group whose bandwidth the last two do not
- * support (not a power of two in [32, 512]) runs in FP64.  No counterpart in the reference,
+ * S2AMD_MIXED or S2AMD_SPLIT_BF16; a group whose bandwidth the chosen form does
+ * not support (mixed: outside [32, 512]; split-BF16: not a power of two in
+ * [32, 512]) runs in FP64.  No counterpart in the reference,
  * which convolves one pair of host grids per call (FST_semi_memo.c:439-518). */
 typedef struct {
     int bw;

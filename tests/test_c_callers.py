@@ -1,7 +1,7 @@
 """C callers of the library, built with gcc against include/, and the
 AddressSanitizer / UndefinedBehaviorSanitizer CPU builds (tests/c/Makefile).
 
-* The integration guide's C example (`round_trip`) is
+* The integration guide's C example (its `round_trip` block) is
   tests/c/round_trip.c verbatim; it compiles as C11 with -Wall -Wextra
   -Werror and links against libs2kit_amd.so.  On the GPU its output (an
   InvFSTSemiMemo grid and the FSTSemiMemo coefficients of that grid) matches
