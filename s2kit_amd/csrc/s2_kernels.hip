@@ -207,6 +207,176 @@ __device__ __forceinline__ void mr_pass_odd(T* a, T* b, int N, int Ns, int R, in
 #define S2_MR_THREADS 512
 #endif
 constexpr int kMrThreads = S2_MR_THREADS;
+
+// ---------------------------------------------------------------------------
+// In-place mixed-radix FFTs with register butterflies (lengths whose prime
+// factors are at most kRegMaxPrime).  Each pass of radix R is one Stockham
+// step: every thread loads the R inputs of each of its butterflies (up to
+// PER of them) into registers, the block synchronises, and each thread
+// twiddles, transforms and stores its butterflies over the same LDS buffer.
+// Against the two-buffer passes above this halves the LDS of a block (twice
+// the rings per block) and reads each input once per pass (the pair items of
+// mr_pass_odd read all R inputs once per output pair).
+// ---------------------------------------------------------------------------
+constexpr int kRegMaxPrime = 19;
+#ifndef S2_RM_ELEMS64
+#define S2_RM_ELEMS64 4096
+#endif
+#ifndef S2_RM_ELEMS32
+#define S2_RM_ELEMS32 8192
+#endif
+// values per block of the register instances (FP64: 64 KiB, FP32: 64 KiB)
+template <typename T>
+constexpr int rm_elems() {
+    return sizeof(T) == 16 ? S2_RM_ELEMS64 : S2_RM_ELEMS32;
+}
+
+// a block-uniform value in scalar registers
+__device__ __forceinline__ double uniform(double x) {
+    const long long b = __builtin_bit_cast(long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float uniform(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
+// R-point DFT of v stored to out[p*Ns], y_p = sum_q v_q w^{pq},
+// w = exp(SIGN 2 pi i/R).  Odd R by conjugate pairs: with s_q = v_q + v_{R-q},
+// d_q = v_q - v_{R-q} (q <= H = (R-1)/2),
+//   y_p, y_{R-p} = v_0 + sum_q s_q cos(2 pi pq/R) +- i SIGN sum_q d_q sin(2 pi pq/R),
+// cs[t], sn[t] = cos, sin(2 pi t/R) for t <= H.  v is overwritten.
+template <typename T, int SIGN, int R>
+__device__ __forceinline__ void dft_store(T* v, const typename Real<T>::type* cs, const typename Real<T>::type* sn,
+                                          T* out, int Ns) {
+    using F = typename Real<T>::type;
+    if constexpr (R == 2) {
+        out[0] = cadd(v[0], v[1]);
+        out[Ns] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        const T s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const T s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        const T r13 = SIGN < 0 ? mk<T>(d13.y, -d13.x) : mk<T>(-d13.y, d13.x);  // SIGN i d13
+        out[0] = cadd(s02, s13);
+        out[2 * Ns] = csub(s02, s13);
+        out[Ns] = cadd(d02, r13);
+        out[3 * Ns] = csub(d02, r13);
+    } else {
+        constexpr int H = (R - 1) / 2;
+        T y0 = v[0];
+#pragma unroll
+        for (int q = 1; q <= H; ++q) {
+            const T s = cadd(v[q], v[R - q]), d = csub(v[q], v[R - q]);
+            v[q] = s;
+            v[R - q] = d;
+            y0 = cadd(y0, s);
+        }
+        out[0] = y0;
+#pragma unroll
+        for (int p = 1; p <= H; ++p) {
+            F ar = v[0].x, ai = v[0].y, br = 0, bi = 0;
+#pragma unroll
+            for (int q = 1; q <= H; ++q) {
+                const int r = (p * q) % R;
+                const int t = r <= H ? r : R - r;
+                const F sv = r <= H ? sn[t] : -sn[t];
+                ar += v[q].x * cs[t];
+                ai += v[q].y * cs[t];
+                br += v[R - q].x * sv;
+                bi += v[R - q].y * sv;
+            }
+            // +- i SIGN (br + i bi) = +- SIGN (-bi + i br)
+            out[p * Ns] = mk<T>(ar - (F)SIGN * bi, ai + (F)SIGN * br);
+            out[(R - p) * Ns] = mk<T>(ar + (F)SIGN * bi, ai - (F)SIGN * br);
+        }
+    }
+}
+
+// One in-place pass of radix R over nfft length-N transforms (row stride N):
+// butterfly j = jq*Ns + k (k < Ns, j < N/R) reads a[j + q*N/R], applies the
+// twiddles exp(SIGN 2 pi i q k / (Ns*R)), runs an R-point DFT and writes
+// output p to a[jq*Ns*R + p*Ns + k].  tw[t] = exp(-2 pi i t / N) (in LDS).
+template <typename T, int SIGN, int R, int PER>
+__device__ __forceinline__ void mr_reg_pass(T* a, int N, int Ns, int nfft, const T* __restrict__ tw) {
+    using F = typename Real<T>::type;
+    constexpr int H = (R - 1) / 2;
+    const int NR = N / R, step = N / (Ns * R);
+    const unsigned long long mNR = magic16(NR), mNs = magic16(Ns);
+    const int total = nfft * NR;
+    T v[PER][R];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int bf = threadIdx.x + u * kMrThreads;
+        if (bf < total) {
+            const int f = div16(bf, mNR), j = bf - f * NR;
+            const T* in = a + f * N + j;
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[u][q] = in[q * NR];
+        }
+    }
+    // cos and sin of 2 pi t/R from the table: tw[t N/R] = exp(-2 pi i t/R)
+    F cs[H + 1], sn[H + 1];
+#pragma unroll
+    for (int t = 1; t <= H; ++t) {
+        const T w = tw[t * NR];
+        cs[t] = uniform(w.x);
+        sn[t] = uniform(-w.y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int bf = threadIdx.x + u * kMrThreads;
+        if (bf < total) {
+            const int f = div16(bf, mNR), j = bf - f * NR;
+            const int jq = div16(j, mNs), k = j - jq * Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    T w = tw[q * k * step];  // q*k*step < R*Ns*step = N
+                    if (SIGN > 0) w.y = -w.y;
+                    v[u][q] = cmul(v[u][q], w);
+                }
+            }
+            dft_store<T, SIGN, R>(v[u], cs, sn, a + f * N + jq * Ns * R + k, Ns);
+        }
+    }
+    __syncthreads();
+}
+
+// Butterflies per thread of a radix-R pass over at most E values.
+template <int E, int R>
+constexpr int rm_per() {
+    return (E + R * kMrThreads - 1) / (R * kMrThreads);
+}
+
+// In-place FFTs (natural order in and out) of nfft rows of length N, row
+// stride N, nfft*N <= rm_elems<T>(); every prime factor of N at most
+// kRegMaxPrime (rm_supported).  Passes: radix 4 while 4 divides, then 2, then
+// the odd primes in ascending order.
+template <typename T, int SIGN>
+__device__ __forceinline__ void mr_reg_fft(T* a, int N, int nfft, const T* __restrict__ tw) {
+    constexpr int E = rm_elems<T>();
+    int rem = N, Ns = 1;
+    while (rem > 1) {
+        int R = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : 3;
+        while (rem % R) R += 2;
+        switch (R) {
+            case 2: mr_reg_pass<T, SIGN, 2, rm_per<E, 2>()>(a, N, Ns, nfft, tw); break;
+            case 3: mr_reg_pass<T, SIGN, 3, rm_per<E, 3>()>(a, N, Ns, nfft, tw); break;
+            case 4: mr_reg_pass<T, SIGN, 4, rm_per<E, 4>()>(a, N, Ns, nfft, tw); break;
+            case 5: mr_reg_pass<T, SIGN, 5, rm_per<E, 5>()>(a, N, Ns, nfft, tw); break;
+            case 7: mr_reg_pass<T, SIGN, 7, rm_per<E, 7>()>(a, N, Ns, nfft, tw); break;
+            case 11: mr_reg_pass<T, SIGN, 11, rm_per<E, 11>()>(a, N, Ns, nfft, tw); break;
+            case 13: mr_reg_pass<T, SIGN, 13, rm_per<E, 13>()>(a, N, Ns, nfft, tw); break;
+            case 17: mr_reg_pass<T, SIGN, 17, rm_per<E, 17>()>(a, N, Ns, nfft, tw); break;
+            case 19: mr_reg_pass<T, SIGN, 19, rm_per<E, 19>()>(a, N, Ns, nfft, tw); break;
+            default: break;  // excluded on the host (rm_supported)
+        }
+        rem /= R;
+        Ns *= R;
+    }
+}
 // Loads per thread that the load phases of the per-field FFT/DCT kernels keep
 // in flight before storing to LDS (4 covers a 2048-value block at 512
 // threads in one batch).
@@ -267,6 +437,19 @@ __device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T
     return a;
 }
 
+// The FFT stage of the per-field kernels: MODE 0 radix-2 passes
+// (power-of-two N), 1 the two-buffer mixed-radix passes, 2 the in-place
+// register passes (mr_reg_fft).  Returns the buffer holding the result.
+template <typename T, int SIGN, int MODE>
+__device__ __forceinline__ T* fft_rows(T* a, T* b, int N, int logN, int pow2, int nfft, const T* __restrict__ tw) {
+    if constexpr (MODE == 2) {
+        mr_reg_fft<T, SIGN>(a, N, nfft, tw);
+        return a;
+    } else {
+        return block_fft<T, SIGN, MODE == 1>(a, b, N, logN, pow2, nfft, tw);
+    }
+}
+
 // Row of the order-major buffers that holds semi/naive row number q:
 // q < cut -> order q; then the negative orders -(cut-1) .. -1 in rows
 // N-cut+1 .. N-1 (rows bw+1 .. N-1 when cut = bw).
@@ -277,19 +460,19 @@ __device__ __forceinline__ int row_of(int q, int N, int cut) { return q < cut ? 
 // ---------------------------------------------------------------------------
 
 // grid (ceil(N/rpb), F): rpb rings of one field per block.
-template <typename T, bool MR>
-__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
+template <typename T, int MODE>
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
                                                  T* __restrict__ X, int N, int logN, int pow2, int rpb,
                                                  const T* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
     int bx = blockIdx.x, f = blockIdx.y;
-    if constexpr (MR) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
+    if constexpr (MODE != 0) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
     const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
     const size_t base = ((size_t)f * N + j0) * N;
-    const int tot = nr * N;  // <= 2048
+    const int tot = nr * N;  // < 2^16 (div16)
     using F = typename Real<T>::type;
     for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
         double vr[kFillBatch], vi[kFillBatch];
@@ -308,13 +491,13 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double*
         }
     }
     const T* twp = tw;
-    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = b + rpb * N;
+    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
         for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
         twp = twl;
     }
     __syncthreads();
-    T* r = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, twp);
+    T* r = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp);
     const unsigned long long mnr = magic16(nr);
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const int row = div16(i, mnr), jl = i - row * nr;
@@ -325,8 +508,8 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_fwd(const double*
 // Weighted DCT-II of every seminaive order row, first bw outputs kept.
 // grid (ceil(nrows/rpb), F).  w_semi holds the even then the odd weights,
 // premultiplied by sqrt(2 pi)/N (FFT scaling) and 1/sqrt(2N) (DCT scaling).
-template <typename T, bool MR>
-__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
+template <typename T, int MODE>
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
                                                  const double* __restrict__ w_semi, const T* __restrict__ tw,
                                                  const T* __restrict__ ph, int N, int logN, int pow2, int B,
                                                  int cut, int nrows, int rpb) {
@@ -334,10 +517,10 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __re
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
     int bx = blockIdx.x, f = blockIdx.y;
-    if constexpr (MR) xcd_block(bx, f);
+    if constexpr (MODE != 0) xcd_block(bx, f);
     const int q0 = bx * rpb;
     const int nr = min(rpb, nrows - q0);
-    const int tot = nr * N;  // <= 2048
+    const int tot = nr * N;  // < 2^16 (div16)
     const unsigned long long mN = magic16(N);
     for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
         T xv[kFillBatch];
@@ -364,13 +547,13 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_fwd(const T* __re
         }
     }
     const T* twp = tw;
-    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = b + rpb * N;
+    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
         for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
         twp = twl;
     }
     __syncthreads();
-    T* V = block_fft<T, -1, MR>(a, b, N, logN, pow2, nr, twp);
+    T* V = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp);
     const unsigned long long mB = magic16(B);
     for (int i = threadIdx.x; i < nr * B; i += blockDim.x) {
         const int rl = div16(i, mB), k = i - rl * B;
@@ -881,8 +1064,8 @@ __global__ __launch_bounds__(256) void k_leg_inv(const double* __restrict__ cre,
 // DCT-III of every seminaive order row (inputs 0..B-1, zero above), then
 // x sin(theta) for odd orders, x (-1) on odd negative rows, x 1/sqrt(2pi).
 // REAL mirrors row N-m = conj(row m).  grid (ceil(nrows/rpb), F).
-template <typename T, bool MR>
-__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
+template <typename T, int MODE>
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
                                                  const T* __restrict__ tw, const T* __restrict__ ph,
                                                  const double* __restrict__ sinv, int N, int logN, int pow2, int B,
                                                  int cut, int nrows, int rpb, int fmt) {
@@ -890,13 +1073,13 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __re
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
     int bx = blockIdx.x, f = blockIdx.y;
-    if constexpr (MR) xcd_block(bx, f);
+    if constexpr (MODE != 0) xcd_block(bx, f);
     const int q0 = bx * rpb;
     const int nr = min(rpb, nrows - q0);
     using R = typename Real<T>::type;
     // V[k] = e^{+i pi k/2N} (X[k] - i X[N-k]), X[N] = 0, X[k >= B] = 0: one
     // of the two terms is nonzero, X[k] for k < B, X[N-k] for k > B.
-    const int tot = nr * N;  // <= 2048
+    const int tot = nr * N;  // < 2^16 (div16)
     const unsigned long long mN = magic16(N);
     for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
         T xv[kFillBatch], e[kFillBatch];
@@ -922,13 +1105,13 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_dct_inv(const T* __re
         }
     }
     const T* twp = tw;
-    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = b + rpb * N;
+    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
         for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
         twp = twl;
     }
     __syncthreads();
-    T* u = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, twp);
+    T* u = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp);
     const R inv_s2pi = (R)0.39894228040143267794;  // 1/sqrt(2 pi)
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const int rl = div16(i, mN), j = i - rl * N;
@@ -986,18 +1169,18 @@ __global__ __launch_bounds__(256) void k_naive_inv(const double* __restrict__ cr
 
 // Inverse ring FFT (+i, unnormalised) per ring; row B (Nyquist) is zero.
 // grid (ceil(N/rpb), F).
-template <typename T, bool MR>
-__global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
+template <typename T, int MODE>
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
                                                  double* __restrict__ im, int N, int logN, int pow2, int B, int rpb,
                                                  const T* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     T* b = a + rpb * N;
     int bx = blockIdx.x, f = blockIdx.y;
-    if constexpr (MR) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
+    if constexpr (MODE != 0) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
     const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
-    const int tot = nr * N;  // <= 2048
+    const int tot = nr * N;  // < 2^16 (div16)
     const unsigned long long mnr = magic16(nr);
     for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
         T v[kFillBatch];
@@ -1019,13 +1202,13 @@ __global__ __launch_bounds__(MR ? kMrThreads : 256) void k_phi_inv(const T* __re
         }
     }
     const T* twp = tw;
-    if constexpr (MR) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = b + rpb * N;
+    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
+        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
         for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
         twp = twl;
     }
     __syncthreads();
-    T* y = block_fft<T, +1, MR>(a, b, N, logN, pow2, nr, twp);
+    T* y = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp);
     const size_t base = ((size_t)f * N + j0) * N;
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         re[base + i] = (double)y[i].x;
@@ -1305,32 +1488,56 @@ static int ilog2_exact(int n) {
     return ((1 << l) == n) ? l : -1;
 }
 
-// Launch the power-of-two (kp) or the mixed-radix (km) instance of a
-// generic FFT kernel: separate instances keep the mixed-radix passes'
-// registers out of the power-of-two kernels.
-// The mixed-radix instance also holds the N twiddles in LDS.
-template <typename KP, typename KM, typename... A>
-static void fft_launch(bool pow2, KP kp, KM km, dim3 grid, size_t lds, int N, hipStream_t s, A... args) {
-    if (pow2) {
-        kp<<<grid, 256, lds, s>>>(args...);
-    } else {
-        const size_t lm = lds + sizeof(double2) * N;
-        if (lm > 65536) S2_HIP(hipFuncSetAttribute((const void*)km, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lm));
-        km<<<grid, kMrThreads, lm, s>>>(args...);
-    }
-}
-
-// Rings per block of the per-field FFT/DCT kernels: 2048 values in LDS for
-// power-of-two N; S2_MR_ELEMS for the mixed-radix instances, capped so that
-// both buffers and the twiddles fit gfx950's 160 KiB.
+// The per-field FFT/DCT kernels come in three instances (MODE of
+// fft_rows): 0 for power-of-two N (radix-2 passes, 256 threads, 2048 values
+// per block in two buffers), 2 for N whose prime factors are at most
+// kRegMaxPrime (in-place register passes, 512 threads, rm_elems<T>() values
+// per block), 1 for the other N (two-buffer passes, 512 threads, S2_MR_ELEMS
+// FP64 / S2_MR_ELEMS32 FP32 values per block).  The mixed-radix instances
+// also hold the N twiddles in LDS.  Separate instances keep the mixed-radix
+// passes' registers out of the power-of-two kernels.
 #ifndef S2_MR_ELEMS
 #define S2_MR_ELEMS 2048
 #endif
-static int rings_per_block(int N, bool pow2) {
+#ifndef S2_MR_ELEMS32
+#define S2_MR_ELEMS32 4096
+#endif
+struct FftCfg {
+    int mode;    // instance
+    int rpb;     // rows (rings or order rows) per block
+    size_t lds;  // dynamic LDS bytes
+};
+
+static bool rm_supported(int N) {
+    int rem = N;
+    while (rem % 2 == 0) rem /= 2;
+    for (int p = 3; p <= kRegMaxPrime; p += 2)
+        while (rem % p == 0) rem /= p;
+    return rem == 1;
+}
+
+template <typename T>
+static FftCfg fft_cfg(int N, bool pow2) {
     if (N > 2048) throw ArgError{"ring length above 2048"};  // block loops index below 2^16
-    if (pow2) return N >= 2048 ? 1 : 2048 / N;
-    const int cap = (163840 / (int)sizeof(double2) - N) / (2 * N);
-    return std::max(1, std::min(S2_MR_ELEMS / N, cap));
+    constexpr int kLdsValues = 163840 / (int)sizeof(T);     // gfx950: 160 KiB per workgroup
+    if (pow2) {
+        const int rpb = N >= 2048 ? 1 : 2048 / N;
+        return {0, rpb, 2 * sizeof(T) * rpb * N};
+    }
+    if (rm_supported(N)) {
+        const int rpb = std::max(1, std::min(rm_elems<T>() / N, (kLdsValues - N) / N));
+        return {2, rpb, sizeof(T) * ((size_t)rpb * N + N)};
+    }
+    const int elems = sizeof(T) == 16 ? S2_MR_ELEMS : S2_MR_ELEMS32;
+    const int rpb = std::max(1, std::min(elems / N, (kLdsValues - N) / (2 * N)));
+    return {1, rpb, sizeof(T) * (2 * (size_t)rpb * N + N)};
+}
+
+template <typename K0, typename K1, typename K2, typename... A>
+static void fft_launch(const FftCfg& g, K0 k0, K1 k1, K2 k2, dim3 grid, hipStream_t s, A... args) {
+    if (g.mode == 0) launch_lds(k0, grid, 256, g.lds, s, args...);
+    else if (g.mode == 1) launch_lds(k1, grid, kMrThreads, g.lds, s, args...);
+    else launch_lds(k2, grid, kMrThreads, g.lds, s, args...);
 }
 
 void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s) {
@@ -1556,33 +1763,24 @@ static size_t reg_lds(int N) { return sizeof(double2) * ((size_t)reg_rpb(N) * (N
 // FP32 ring FFTs of grids whose ring length is not a power of two (the mixed
 // form's first and last stages at such bandwidths), on the mixed-radix
 // instances; same order-major FP32 intermediate X as the register kernels.
-#ifndef S2_MR_ELEMS32
-#define S2_MR_ELEMS32 4096
-#endif
-// Rings per block of the FP32 mixed-radix ring FFTs: 4096 values (8 rings at
-// bw = 255; FP32 values leave room for twice the FP64 blocks' rings).
-// Measured at bw = 255 x 1024 fields: 2048 values 2.92 / 2.95 ms per forward /
-// inverse FFT launch, 4096 values 2.67 / 2.85 ms, 8192 values 3.89 / 4.21 ms
-// (one block per CU); 1024-thread blocks slower at both sizes.
-static int rings_per_block_f32(int N) {
-    const int cap = (163840 / (int)sizeof(float2) - N) / (2 * N);
-    return std::max(1, std::min(S2_MR_ELEMS32 / N, cap));
-}
-
+// Measured at bw = 255 x 1024 fields on the two-buffer instance: 2048 values
+// per block 2.92 / 2.95 ms per forward / inverse FFT launch, 4096 values
+// 2.67 / 2.85 ms, 8192 values 3.89 / 4.21 ms (one block per CU);
+// 1024-thread blocks slower at both sizes.
 void ring_fft_fwd_f32(const DevConsts& c, const double* re, const double* im, float2* X, int F, hipStream_t s) {
     const int N = c.N;
-    const int rpb = rings_per_block_f32(N);
-    const size_t lds = 2 * sizeof(float2) * rpb * N;
-    fft_launch(false, k_phi_fwd<float2, false>, k_phi_fwd<float2, true>, dim3((N + rpb - 1) / rpb, F), lds, N, s,
+    const FftCfg fg = fft_cfg<float2>(N, false);
+    const int rpb = fg.rpb;
+    fft_launch(fg, k_phi_fwd<float2, 0>, k_phi_fwd<float2, 1>, k_phi_fwd<float2, 2>, dim3((N + rpb - 1) / rpb, F), s,
                re, im, X, N, c.logN, 0, rpb, c.tw32);
     S2_HIP(hipGetLastError());
 }
 
 void ring_fft_inv_f32(const DevConsts& c, const float2* X, double* re, double* im, int F, hipStream_t s) {
     const int N = c.N;
-    const int rpb = rings_per_block_f32(N);
-    const size_t lds = 2 * sizeof(float2) * rpb * N;
-    fft_launch(false, k_phi_inv<float2, false>, k_phi_inv<float2, true>, dim3((N + rpb - 1) / rpb, F), lds, N, s,
+    const FftCfg fg = fft_cfg<float2>(N, false);
+    const int rpb = fg.rpb;
+    fft_launch(fg, k_phi_inv<float2, 0>, k_phi_inv<float2, 1>, k_phi_inv<float2, 2>, dim3((N + rpb - 1) / rpb, F), s,
                X, re, im, N, c.logN, 0, c.bw, rpb, c.tw32);
     S2_HIP(hipGetLastError());
 }
@@ -1591,15 +1789,15 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
                   double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
-    const int rpb = rings_per_block(N, c.pow2);
-    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const int rpb = fg.rpb;
     {
     Span sp(tm, ST_PHI_FWD, s);
     if (reg_rings(c))
         launch_lds(k_fphi_fwd<double, 8>, dim3(N / reg_rpb(N), F), kThreads, reg_lds(N), s, re, im, X, c.logN, c.tw);
     else
-        fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft,
-                   N, s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
+        fft_launch(fg, k_phi_fwd<double2, 0>, k_phi_fwd<double2, 1>, k_phi_fwd<double2, 2>, dim3((N + rpb - 1) / rpb, F),
+                   s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
     S2_HIP(hipGetLastError());
     }
     if (cut > 0) {
@@ -1612,8 +1810,8 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
             launch_lds(k_fdct_fwd<double, 8>, dim3((nrows + reg_rpb(N) - 1) / reg_rpb(N), F), kThreads, reg_lds(N), s,
                        (const double2*)X, (double*)Cb, c.w_semi, c.logN, cut, nrows, c.tw, c.ph);
         else
-            fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                       lds_fft, N, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+            fft_launch(fg, k_dct_fwd<double2, 0>, k_dct_fwd<double2, 1>, k_dct_fwd<double2, 2>,
+                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
         }
         Span sp(tm, ST_LEG_FWD, s);
@@ -1646,8 +1844,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
                   double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
-    const int rpb = rings_per_block(N, c.pow2);
-    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const int rpb = fg.rpb;
     // the register DCT reads the row GEMVs' planar rows
     const int planar = (reg_rings(c) && F <= kRowsMaxFields) ? 1 : 0;
     if (cut > 0) {
@@ -1676,8 +1874,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
             launch_lds(k_fdct_inv<double, 8>, dim3((nrows + reg_rpb(N) - 1) / reg_rpb(N), F), kThreads, reg_lds(N), s,
                        (const double*)Cb, X, c.sinv, c.logN, cut, nrows, f, c.tw, c.ph);
         else
-            fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                       lds_fft, N, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+            fft_launch(fg, k_dct_inv<double2, 0>, k_dct_inv<double2, 1>, k_dct_inv<double2, 2>,
+                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
@@ -1692,8 +1890,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         launch_lds(k_fphi_inv<double, 8>, dim3(N / reg_rpb(N), F), kThreads, reg_lds(N), s, (const double2*)X, re, im,
                    c.logN, c.tw);
     else
-        fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft,
-                   N, s, X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
+        fft_launch(fg, k_phi_inv<double2, 0>, k_phi_inv<double2, 1>, k_phi_inv<double2, 2>, dim3((N + rpb - 1) / rpb, F),
+                   s, X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
     S2_HIP(hipGetLastError());
 }
 
@@ -1701,19 +1899,19 @@ void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re
                        double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = g.cut;
-    const int rpb = rings_per_block(N, c.pow2);
-    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const int rpb = fg.rpb;
     {
         Span sp(tm, ST_PHI_FWD, s);
-        fft_launch(c.pow2, k_phi_fwd<double2, false>, k_phi_fwd<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N,
+        fft_launch(fg, k_phi_fwd<double2, 0>, k_phi_fwd<double2, 1>, k_phi_fwd<double2, 2>, dim3((N + rpb - 1) / rpb, F),
                    s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
         S2_HIP(hipGetLastError());
     }
     if (cut > 0) {
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_FWD, s);
-        fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, N, s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+        fft_launch(fg, k_dct_fwd<double2, 0>, k_dct_fwd<double2, 1>, k_dct_fwd<double2, 2>,
+                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_LEG_FWD, s);
@@ -1724,8 +1922,8 @@ void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cr
                        double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = g.cut, f = (int)fmt;
-    const int rpb = rings_per_block(N, c.pow2);
-    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
+    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const int rpb = fg.rpb;
     {
         Span sp(tm, ST_LEG_INV, s);
         gemm_inv_launch(c, g, cre, cim, Cb, X, F, fmt, s);
@@ -1733,12 +1931,12 @@ void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cr
     if (cut > 0) {
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_INV, s);
-        fft_launch(c.pow2, k_dct_inv<double2, false>, k_dct_inv<double2, true>, dim3((nrows + rpb - 1) / rpb, F),
-                   lds_fft, N, s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+        fft_launch(fg, k_dct_inv<double2, 0>, k_dct_inv<double2, 1>, k_dct_inv<double2, 2>,
+                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
-    fft_launch(c.pow2, k_phi_inv<double2, false>, k_phi_inv<double2, true>, dim3((N + rpb - 1) / rpb, F), lds_fft, N, s,
+    fft_launch(fg, k_phi_inv<double2, 0>, k_phi_inv<double2, 1>, k_phi_inv<double2, 2>, dim3((N + rpb - 1) / rpb, F), s,
                X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
     S2_HIP(hipGetLastError());
 }
@@ -1751,9 +1949,9 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
     const int N = c.N, B = c.bw;
     k_ring_sum<<<dim3((N + 3) / 4, F), 256, 0, s>>>(re, im, X, N);
     S2_HIP(hipGetLastError());
-    const int rpb = rings_per_block(N, c.pow2);
-    const size_t lds_fft = 2 * sizeof(double2) * rpb * N;
-    fft_launch(c.pow2, k_dct_fwd<double2, false>, k_dct_fwd<double2, true>, dim3(1, F), lds_fft, N, s, X, Cb, c.w_semi,
+    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const int rpb = fg.rpb;
+    fft_launch(fg, k_dct_fwd<double2, 0>, k_dct_fwd<double2, 1>, k_dct_fwd<double2, 2>, dim3(1, F), s, X, Cb, c.w_semi,
                c.tw, c.ph, N, c.logN, c.pow2, B, t.cutoff, 1, 1);
     S2_HIP(hipGetLastError());
     const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
