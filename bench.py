@@ -21,8 +21,9 @@ Also reported on the same JSON line:
               MFMA GEMMs; split: BF16 MFMA with split operands) timed on the
               same fields, with their errors against FP64; and the FP64 form
               at cutoff = bw/4 (naive branch for orders >= 64);
-  non_pow2_bw255  FP64 pairs/s at bw=255 (mixed-radix FFTs) beside bw=256,
-              256 fields, and the bw=255 batch against its per-field path;
+  non_pow2    FP64 and mixed pairs/s at bw=255 (register mixed-radix FFTs)
+              beside bw=256, bw=127 and bw=263 (Bluestein FFTs) beside bw=128,
+              256 fields, and the bw=255 / 127 batches against their per-field path;
   cpu_baseline  the CPU FP64 oracle (a port of the reference path) timed on
               the host, one process per core, on a bounded sample.
 
@@ -80,10 +81,44 @@ KERNELS = {
              "leg_fwd": "k_gemm_fwd<double, false, CT>", "leg_inv": "k_gemm_inv<double, false, CT>",
              "dct_inv": "k_fdct_inv<double, 8>", "phi_inv": "k_fphi_inv<double, 8, 512, false>"},
 }
-# FP64 at a bandwidth that is not a power of two: mixed-radix FFT/DCT kernels
-KERNELS_GEN = {"phi_fwd": "k_phi_fwd<double2, true>", "dct_fwd": "k_dct_fwd<double2, true>",
-               "leg_fwd": "k_gemm_fwd<double, true, CT>", "leg_inv": "k_gemm_inv<double, true, CT>",
-               "dct_inv": "k_dct_inv<double2, true>", "phi_inv": "k_phi_inv<double2, true>"}
+
+def smooth(n, pmax):
+    """n has no prime factor above pmax."""
+    for p in range(2, pmax + 1):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+def ring_fft_instance(n):
+    """Instance of the per-field FFT/DCT kernels for ring length n that is
+    not a power of two (the host's choice in s2_kernels.hip, fft_cfg): 2 the
+    register mixed-radix passes (prime factors up to 19), else Bluestein with
+    a convolution of length M >= 2n-1, 3 for a power-of-two M (taken when
+    within 5/4 of the smallest 7-smooth length), 4 for that 7-smooth M."""
+    if smooth(n, 19):
+        return 2
+    p2 = 1
+    while p2 < 2 * n - 1:
+        p2 <<= 1
+    ms = 2 * n - 1
+    while not smooth(ms, 7):
+        ms += 1
+    return 3 if 4 * p2 <= 5 * ms else 4
+
+
+def kernels_gen(bw, form):
+    """Kernel behind each timed stage at a bandwidth that is not a power of
+    two: the per-field FFT/DCT instances and the GEMMs."""
+    mode = ring_fft_instance(2 * bw)
+    if form in ("mixed", "split"):
+        k = KERNELS[form]
+        return {"phi_fwd": f"k_phi_fwd<float2, {mode}>", "leg_fwd": k["leg_fwd"], "leg_inv": k["leg_inv"],
+                "phi_inv": f"k_phi_inv<float2, {mode}>"}
+    return {"phi_fwd": f"k_phi_fwd<double2, {mode}>", "dct_fwd": f"k_dct_fwd<double2, {mode}>",
+            "leg_fwd": "k_gemm_fwd<double, true, CT>", "leg_inv": "k_gemm_inv<double, true, CT>",
+            "dct_inv": f"k_dct_inv<double2, {mode}>", "phi_inv": f"k_phi_inv<double2, {mode}>"}
+
 
 
 def measured_traffic(form, bw, nfields, stage):
@@ -132,10 +167,13 @@ MFMA_PEAK = {"fp64": (78.6, 1), "fp32": (157.3, 1), "mixed": (157.3, 1), "split"
 
 
 def non_pow2(dev, steps):
-    """FP64 plans at bw=255 (510 = 2*3*5*17 rings: mixed-radix FFTs and DCTs,
-    MFMA Legendre GEMMs) and bw=256, 256 fields each; the bw=255 batch is
-    checked against the drop-ins (per-field kernels) on its first field.  The
-    mixed form (FP32 ring FFTs, mixed-radix at bw=255, naive GEMMs over rings
+    """FP64 plans at bandwidths whose grids are not powers of two, beside the
+    power of two next to them, 256 fields each: bw=255 (510 = 2*3*5*17 rings:
+    register mixed-radix FFTs and DCTs), bw=127 (254 = 2*127: Bluestein, a
+    convolution of length 512) and bw=263 (526 = 2*263: Bluestein, length
+    1080 = 2^3 3^3 5); MFMA Legendre GEMMs throughout.  The bw=255 and bw=127
+    batches are checked against the drop-ins (per-field kernels) on their
+    first field.  The mixed form (FP32 ring FFTs, naive GEMMs over rings
     padded to whole 64-ring groups) is timed on the same fields, with its
     errors against the FP64 plan at cutoff 0 (the all-naive transform, which
     the mixed form computes: at odd bw it synthesises degree bw-1 whole)."""
@@ -143,7 +181,7 @@ def non_pow2(dev, steps):
     import s2kit_amd as S
     from s2kit_amd import synth
     out = {"fields": 256}
-    for bw in (255, 256):
+    for bw in (255, 256, 127, 128, 263):
         F, n = 256, 2 * bw
         c0r, c0i = synth.power_law_coeffs(bw, F, 7, dev)
         gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
@@ -160,14 +198,14 @@ def non_pow2(dev, steps):
             p.forward(gre, gim, cr, ci)
         torch.cuda.synchronize()
         out[f"bw{bw}_pairs_per_s"] = F * steps / (time.perf_counter() - t0)
-        if bw == 255:  # field 0 through the drop-ins (per-field kernels, host buffers)
+        if bw in (255, 127):  # field 0 through the drop-ins (per-field kernels, host buffers)
             tabs = S.HostTables(bw)
             r1, i1 = S.fst(gre[0].cpu().numpy(), gim[0].cpu().numpy(), bw, tabs)
             g1r, g1i = S.inv_fst(c0r[0].cpu().numpy(), c0i[0].cpu().numpy(), bw, tabs)
             a = cr[0].cpu().numpy() + 1j * ci[0].cpu().numpy()
-            out["bw255_forward_vs_per_field"] = float(np.abs(a - (r1 + 1j * i1)).max() / np.abs(r1 + 1j * i1).max())
+            out[f"bw{bw}_forward_vs_per_field"] = float(np.abs(a - (r1 + 1j * i1)).max() / np.abs(r1 + 1j * i1).max())
             g = (gre[0] + 1j * gim[0]).cpu().numpy().ravel()
-            out["bw255_inverse_vs_per_field"] = float(np.abs(g - (g1r + 1j * g1i)).max() / np.abs(g1r + 1j * g1i).max())
+            out[f"bw{bw}_inverse_vs_per_field"] = float(np.abs(g - (g1r + 1j * g1i)).max() / np.abs(g1r + 1j * g1i).max())
         p.close()
         # the mixed form on the same fields
         pr = S.Plan(bw, F, precision=S.FP64, cutoff=0)
@@ -647,7 +685,7 @@ def main():
     achieved = stage_bytes(dom, bw, F, elem, args.precision) / avg_s
     roofline = {
         "bound": "hbm",
-        "kernel": (KERNELS[args.precision] if bw & (bw - 1) == 0 else KERNELS_GEN).get(dom, dom),
+        "kernel": (KERNELS[args.precision] if bw & (bw - 1) == 0 else kernels_gen(bw, args.precision)).get(dom, dom),
         "stage": dom,
         "achieved": achieved / 1e9,
         "peak": HBM_PEAK / 1e9,
@@ -661,7 +699,7 @@ def main():
 
     # ---- MFMA utilisation of the Legendre GEMMs ---------------------------
     peak, issue = MFMA_PEAK[args.precision]
-    kmap = KERNELS[args.precision] if bw & (bw - 1) == 0 else KERNELS_GEN
+    kmap = KERNELS[args.precision] if bw & (bw - 1) == 0 else kernels_gen(bw, args.precision)
     mfma = {}
     for st_ in ("leg_fwd", "leg_inv"):
         if st_ in stages:
@@ -693,7 +731,7 @@ def main():
         "fp64_batch1_python_loop_pairs_per_s": plain_loop,
         "speedup_vs_fp64_batch1": (value / plain) if plain else None,
         "single_field_bw512": single,
-        "non_pow2_bw255": npow2,
+        "non_pow2": npow2,
         "dropin_host_buffers": dropin,
         "roofline": roofline,
         "mfma": mfma,

@@ -89,6 +89,13 @@ struct DevConsts {
     float* w_semi32 = nullptr;   // w_semi rounded to FP32
     float* sinv32 = nullptr;
     int* trow = nullptr;         // [bw][bw+1]: start of row i of order m's transposed table
+    // Bluestein tables for ring lengths with a prime factor above the register
+    // passes' largest radix (else blu_M = 0): M >= 2N-1 values exp(-2 pi i t/M),
+    // N chirp values w_t = exp(-i pi t^2/N), and M values FFT_M(b)/M,
+    // b_t = conj(w_|t|) at t mod M for |t| < N
+    int blu_M = 0;
+    double2* blu = nullptr;
+    float2* blu32 = nullptr;
 };
 
 // Zero-padded, parity-split Legendre matrices for the MFMA GEMMs of the fast

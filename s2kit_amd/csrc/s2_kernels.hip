@@ -207,6 +207,9 @@ __device__ __forceinline__ void mr_pass_odd(T* a, T* b, int N, int Ns, int R, in
 #define S2_MR_THREADS 512
 #endif
 constexpr int kMrThreads = S2_MR_THREADS;
+// The in-place instances (MODE >= 2) hold two 512-thread blocks per CU when
+// they stay within 128 VGPRs (4 waves per SIMD).
+#define S2_FFT_OCCUPANCY(MODE) __attribute__((amdgpu_waves_per_eu((MODE) >= 2 ? 4 : 1)))
 
 // ---------------------------------------------------------------------------
 // In-place mixed-radix FFTs with register butterflies (lengths whose prime
@@ -219,6 +222,9 @@ constexpr int kMrThreads = S2_MR_THREADS;
 // mr_pass_odd read all R inputs once per output pair).
 // ---------------------------------------------------------------------------
 constexpr int kRegMaxPrime = 19;
+// Bluestein's convolution lengths use radices up to 7 only: the radix-11..19
+// butterflies' registers would not leave room for two blocks per CU there.
+constexpr int kBluMaxPrime = 7;
 #ifndef S2_RM_ELEMS64
 #define S2_RM_ELEMS64 4096
 #endif
@@ -243,7 +249,7 @@ __device__ __forceinline__ float uniform(float x) {
 }
 
 // R-point DFT of v stored to out[p*Ns], y_p = sum_q v_q w^{pq},
-// w = exp(SIGN 2 pi i/R).  Odd R by conjugate pairs: with s_q = v_q + v_{R-q},
+// w = exp(SIGN 2 pi i/R).  Odd R (prime or not) by conjugate pairs: with s_q = v_q + v_{R-q},
 // d_q = v_q - v_{R-q} (q <= H = (R-1)/2),
 //   y_p, y_{R-p} = v_0 + sum_q s_q cos(2 pi pq/R) +- i SIGN sum_q d_q sin(2 pi pq/R),
 // cs[t], sn[t] = cos, sin(2 pi t/R) for t <= H.  v is overwritten.
@@ -254,6 +260,10 @@ __device__ __forceinline__ void dft_store(T* v, const typename Real<T>::type* cs
     if constexpr (R == 2) {
         out[0] = cadd(v[0], v[1]);
         out[Ns] = csub(v[0], v[1]);
+    } else if constexpr (R == 8) {
+        dft8<SIGN>(v);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) out[p * Ns] = v[p];
     } else if constexpr (R == 4) {
         const T s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
         const T s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
@@ -316,7 +326,10 @@ __device__ __forceinline__ void mr_reg_pass(T* a, int N, int Ns, int nfft, const
         }
     }
     // cos and sin of 2 pi t/R from the table: tw[t N/R] = exp(-2 pi i t/R)
+    // (t = 0 is met by composite R: pq = 0 mod R)
     F cs[H + 1], sn[H + 1];
+    cs[0] = (F)1;
+    sn[0] = (F)0;
 #pragma unroll
     for (int t = 1; t <= H; ++t) {
         const T w = tw[t * NR];
@@ -350,28 +363,52 @@ constexpr int rm_per() {
     return (E + R * kMrThreads - 1) / (R * kMrThreads);
 }
 
+// Radix of the next pass over the remaining length rem: 8, 4 or 2 while
+// rem is even, then the smallest odd prime factor.  COMPOSITE also takes 15
+// and 9 whole in registers (fewer passes), measured on MI355X and left off:
+// FP64 bw=263 (Bluestein length 1080) -7%, bw=37 (length 147) +9% per step
+// (the smooth-length instance went from 86 to 124 VGPRs), bw=255 and 100
+// unchanged; in FP32 their O(R) conjugate-pair sums cost accuracy (the mixed
+// form at bw = 255 went from below 1e-6 to 1.02e-6).
+#ifndef S2_MR_COMPOSITE
+#define S2_MR_COMPOSITE 0
+#endif
+template <bool COMPOSITE>
+__device__ __forceinline__ int mr_reg_radix(int rem) {
+    if (rem % 8 == 0) return 8;
+    if (rem % 4 == 0) return 4;
+    if (rem % 2 == 0) return 2;
+    if (COMPOSITE && rem % 15 == 0) return 15;
+    if (COMPOSITE && rem % 9 == 0) return 9;
+    int R = 3;
+    while (rem % R) R += 2;
+    return R;
+}
+
 // In-place FFTs (natural order in and out) of nfft rows of length N, row
 // stride N, nfft*N <= rm_elems<T>(); every prime factor of N at most
-// kRegMaxPrime (rm_supported).  Passes: radix 4 while 4 divides, then 2, then
-// the odd primes in ascending order.
-template <typename T, int SIGN>
+// PMAX (rm_supported).  One pass per radix of mr_reg_radix.
+template <typename T, int SIGN, int PMAX = kRegMaxPrime>
 __device__ __forceinline__ void mr_reg_fft(T* a, int N, int nfft, const T* __restrict__ tw) {
     constexpr int E = rm_elems<T>();
     int rem = N, Ns = 1;
     while (rem > 1) {
-        int R = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : 3;
-        while (rem % R) R += 2;
+        constexpr bool comp = S2_MR_COMPOSITE && sizeof(T) == 16;
+        const int R = mr_reg_radix<comp>(rem);
         switch (R) {
             case 2: mr_reg_pass<T, SIGN, 2, rm_per<E, 2>()>(a, N, Ns, nfft, tw); break;
             case 3: mr_reg_pass<T, SIGN, 3, rm_per<E, 3>()>(a, N, Ns, nfft, tw); break;
             case 4: mr_reg_pass<T, SIGN, 4, rm_per<E, 4>()>(a, N, Ns, nfft, tw); break;
+            case 8: mr_reg_pass<T, SIGN, 8, rm_per<E, 8>()>(a, N, Ns, nfft, tw); break;
+            case 9: if constexpr (comp) mr_reg_pass<T, SIGN, 9, rm_per<E, 9>()>(a, N, Ns, nfft, tw); break;
+            case 15: if constexpr (comp) mr_reg_pass<T, SIGN, 15, rm_per<E, 15>()>(a, N, Ns, nfft, tw); break;
             case 5: mr_reg_pass<T, SIGN, 5, rm_per<E, 5>()>(a, N, Ns, nfft, tw); break;
             case 7: mr_reg_pass<T, SIGN, 7, rm_per<E, 7>()>(a, N, Ns, nfft, tw); break;
-            case 11: mr_reg_pass<T, SIGN, 11, rm_per<E, 11>()>(a, N, Ns, nfft, tw); break;
-            case 13: mr_reg_pass<T, SIGN, 13, rm_per<E, 13>()>(a, N, Ns, nfft, tw); break;
-            case 17: mr_reg_pass<T, SIGN, 17, rm_per<E, 17>()>(a, N, Ns, nfft, tw); break;
-            case 19: mr_reg_pass<T, SIGN, 19, rm_per<E, 19>()>(a, N, Ns, nfft, tw); break;
-            default: break;  // excluded on the host (rm_supported)
+            case 11: if constexpr (PMAX >= 11) mr_reg_pass<T, SIGN, 11, rm_per<E, 11>()>(a, N, Ns, nfft, tw); break;
+            case 13: if constexpr (PMAX >= 13) mr_reg_pass<T, SIGN, 13, rm_per<E, 13>()>(a, N, Ns, nfft, tw); break;
+            case 17: if constexpr (PMAX >= 17) mr_reg_pass<T, SIGN, 17, rm_per<E, 17>()>(a, N, Ns, nfft, tw); break;
+            case 19: if constexpr (PMAX >= 19) mr_reg_pass<T, SIGN, 19, rm_per<E, 19>()>(a, N, Ns, nfft, tw); break;
+            default: break;  // excluded on the host (rm_supported, bluestein_length)
         }
         rem /= R;
         Ns *= R;
@@ -381,6 +418,7 @@ __device__ __forceinline__ void mr_reg_fft(T* a, int N, int nfft, const T* __res
 // in flight before storing to LDS (4 covers a 2048-value block at 512
 // threads in one batch).
 constexpr int kFillBatch = 4;
+
 
 // nfft independent length-N transforms stored back to back in a (b is
 // scratch of the same size).  SIGN = -1: forward, +1: inverse (unnormalised).
@@ -437,12 +475,120 @@ __device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T
     return a;
 }
 
+// ---------------------------------------------------------------------------
+// Bluestein's algorithm for ring lengths with a prime factor above
+// kRegMaxPrime: with w_t = exp(SIGN i pi t^2 / N),
+//   y_k = sum_n x_n exp(SIGN 2 pi i nk/N) = w_k sum_n (x_n w_n) conj(w_{k-n}),
+// a circular convolution of length M >= 2N-1 run as two FFTs of length M:
+// radix-8 register passes (fft_block) for a power of two, the register
+// mixed-radix passes (mr_reg_fft) for M whose primes are at most
+// kRegMaxPrime.  Tables (consts_create), for the forward sign:
+// g[0..M) = exp(-2 pi i t/M), g[M..M+N) = w_t, g[M+N..2M+N) = FFT_M(b)/M with
+// b_t = conj(w_|t|) at t mod M (|t| < N); the inverse sign uses conj(w) and
+// conj(Bh[-k mod M]).  blu_rows rows of blu_stride values per block, all of
+// them transformed (rows >= nfft are zeroed first).
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr bool is_pow2(int M) { return (M & (M - 1)) == 0; }
+__host__ __device__ constexpr int imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ constexpr int imax(int a, int b) { return a > b ? a : b; }
+// row stride: power-of-two rows padded against bank conflicts
+__host__ __device__ constexpr int blu_stride(int M) { return is_pow2(M) ? M + 1 : M; }
+// rows per block: fft_block transforms exactly rm_elems/M rows; mixed-radix
+// rows: at most rm_elems values, and with the M twiddles within 80 KiB of
+// LDS (two blocks per CU)
+template <typename T>
+__host__ __device__ constexpr int blu_rows(int M) {
+    return is_pow2(M) ? rm_elems<T>() / M
+                      : imax(1, imin(rm_elems<T>() / M, (81920 / (int)sizeof(T) - M) / M));
+}
+
+template <typename T, int SIGN, bool P2>
+__device__ __forceinline__ void bluestein_rows(T* a, int N, int M, int nfft, const T* twl, const T* __restrict__ g) {
+    constexpr int V = rm_elems<T>() / kMrThreads;
+    constexpr bool p2 = P2;  // is_pow2(M): separate instances keep each path's registers
+    const int S = blu_stride(M), rows = blu_rows<T>(M), tot = rows * M;
+    const unsigned long long mM = magic16(M);
+    const T* __restrict__ w = g + M;
+    const T* __restrict__ bh = g + M + N;
+    for (int i = threadIdx.x; i < tot; i += kMrThreads) {
+        const int r = div16(i, mM), n = i - r * M;
+        T v = mk<T>(0, 0);
+        if (n < N && r < nfft) {
+            T c = w[n];
+            if (SIGN > 0) c.y = -c.y;
+            v = cmul(a[r * S + n], c);
+        }
+        a[r * S + n] = v;
+    }
+    __syncthreads();
+    // Both FFTs forward (one instance of the passes per kernel): the inverse
+    // FFT of z is conj(FFT(conj z)), the conjugations folded into the products.
+    if constexpr (p2) fft_block<T, -1, V, kMrThreads>(a, __builtin_ctz(M), S, twl);
+    else mr_reg_fft<T, -1, kBluMaxPrime>(a, M, rows, twl);
+    for (int i = threadIdx.x; i < tot; i += kMrThreads) {
+        const int r = div16(i, mM), k = i - r * M;
+        T c = bh[SIGN < 0 ? k : (k == 0 ? 0 : M - k)];
+        if (SIGN > 0) c.y = -c.y;
+        a[r * S + k] = cconj(cmul(a[r * S + k], c));
+    }
+    __syncthreads();
+    if constexpr (p2) fft_block<T, -1, V, kMrThreads>(a, __builtin_ctz(M), S, twl);
+    else mr_reg_fft<T, -1, kBluMaxPrime>(a, M, rows, twl);
+    const unsigned long long mN = magic16(N);
+    for (int i = threadIdx.x; i < nfft * N; i += kMrThreads) {
+        const int r = div16(i, mN), k = i - r * N;
+        T c = w[k];
+        if (SIGN > 0) c.y = -c.y;
+        a[r * S + k] = cmul(cconj(a[r * S + k]), c);
+    }
+    __syncthreads();
+}
+
+// Row stride of the per-field kernels' LDS rows: N, or blu_stride(M) for the
+// Bluestein instance (whose logN argument holds M).
+template <int MODE>
+__device__ __forceinline__ int row_stride(int N, int logN) {
+    return MODE >= 3 ? blu_stride(logN) : N;
+}
+// Rows of the block's LDS buffer: rpb, or blu_rows(M) for Bluestein (which
+// transforms all of them).
+template <typename T, int MODE>
+__device__ __forceinline__ int lds_rows(int rpb, int logN) {
+    return MODE >= 3 ? blu_rows<T>(logN) : rpb;
+}
+// LDS position of the flat index i = row * N + column.
+template <int MODE>
+__device__ __forceinline__ int lds_row(int i, int N, int RS, unsigned long long mN) {
+    if constexpr (MODE >= 3) {
+        const int r = div16(i, mN);
+        return r * RS + (i - r * N);
+    } else {
+        return i;
+    }
+}
+// Stage the FFT twiddles in LDS at b (MODE 1: past the second buffer; the
+// others transform in place): N values, M for Bluestein.
+template <typename T, int MODE>
+__device__ __forceinline__ const T* stage_fft_twiddles(T* b, int rpb, int N, int logN, const T* __restrict__ tw) {
+    T* twl = MODE == 1 ? b + rpb * N : b;
+    const int n = MODE >= 3 ? logN : N;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = tw[i];
+    return twl;
+}
+
 // The FFT stage of the per-field kernels: MODE 0 radix-2 passes
 // (power-of-two N), 1 the two-buffer mixed-radix passes, 2 the in-place
-// register passes (mr_reg_fft).  Returns the buffer holding the result.
+// register passes (mr_reg_fft), 3 and 4 Bluestein with M a power of two and
+// M of small primes (bluestein_rows; logN holds M, tw the LDS copy of the M
+// twiddles, gtab the global tables).
+// Returns the buffer holding the result.
 template <typename T, int SIGN, int MODE>
-__device__ __forceinline__ T* fft_rows(T* a, T* b, int N, int logN, int pow2, int nfft, const T* __restrict__ tw) {
-    if constexpr (MODE == 2) {
+__device__ __forceinline__ T* fft_rows(T* a, T* b, int N, int logN, int pow2, int nfft, const T* __restrict__ tw,
+                                       const T* __restrict__ gtab) {
+    if constexpr (MODE >= 3) {
+        bluestein_rows<T, SIGN, MODE == 3>(a, N, logN, nfft, tw, gtab);
+        return a;
+    } else if constexpr (MODE == 2) {
         mr_reg_fft<T, SIGN>(a, N, nfft, tw);
         return a;
     } else {
@@ -461,23 +607,26 @@ __device__ __forceinline__ int row_of(int q, int N, int cut) { return q < cut ? 
 
 // grid (ceil(N/rpb), F): rpb rings of one field per block.
 template <typename T, int MODE>
-__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) void k_phi_fwd(const double* __restrict__ re, const double* __restrict__ im,
                                                  T* __restrict__ X, int N, int logN, int pow2, int rpb,
                                                  const T* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
-    T* b = a + rpb * N;
+    const int RS = row_stride<MODE>(N, logN);
+    T* b = a + lds_rows<T, MODE>(rpb, logN) * RS;
+    constexpr int FB = kFillBatch;
     int bx = blockIdx.x, f = blockIdx.y;
     if constexpr (MODE != 0) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
     const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
     const size_t base = ((size_t)f * N + j0) * N;
     const int tot = nr * N;  // < 2^16 (div16)
+    const unsigned long long mN = magic16(N);
     using F = typename Real<T>::type;
-    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
-        double vr[kFillBatch], vi[kFillBatch];
+    for (int i0 = threadIdx.x; i0 < tot; i0 += FB * blockDim.x) {
+        double vr[FB], vi[FB];
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
             if (i < tot) {
                 vr[u] = re[base + i];
@@ -485,23 +634,19 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_phi_fwd(const doubl
             }
         }
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
-            if (i < tot) a[i] = mk<T>((F)vr[u], (F)vi[u]);
+            if (i < tot) a[lds_row<MODE>(i, N, RS, mN)] = mk<T>((F)vr[u], (F)vi[u]);
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
-        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
-        twp = twl;
-    }
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
     __syncthreads();
-    T* r = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp);
+    T* r = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const unsigned long long mnr = magic16(nr);
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const int row = div16(i, mnr), jl = i - row * nr;
-        X[((size_t)f * N + row) * N + j0 + jl] = r[jl * N + row];
+        X[((size_t)f * N + row) * N + j0 + jl] = r[jl * RS + row];
     }
 }
 
@@ -509,24 +654,26 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_phi_fwd(const doubl
 // grid (ceil(nrows/rpb), F).  w_semi holds the even then the odd weights,
 // premultiplied by sqrt(2 pi)/N (FFT scaling) and 1/sqrt(2N) (DCT scaling).
 template <typename T, int MODE>
-__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
                                                  const double* __restrict__ w_semi, const T* __restrict__ tw,
                                                  const T* __restrict__ ph, int N, int logN, int pow2, int B,
                                                  int cut, int nrows, int rpb) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
-    T* b = a + rpb * N;
+    const int RS = row_stride<MODE>(N, logN);
+    T* b = a + lds_rows<T, MODE>(rpb, logN) * RS;
+    constexpr int FB = kFillBatch;
     int bx = blockIdx.x, f = blockIdx.y;
     if constexpr (MODE != 0) xcd_block(bx, f);
     const int q0 = bx * rpb;
     const int nr = min(rpb, nrows - q0);
     const int tot = nr * N;  // < 2^16 (div16)
     const unsigned long long mN = magic16(N);
-    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
-        T xv[kFillBatch];
-        double wv[kFillBatch];
+    for (int i0 = threadIdx.x; i0 < tot; i0 += FB * blockDim.x) {
+        T xv[FB];
+        double wv[FB];
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
             if (i < tot) {
                 const int rl = div16(i, mN), j = i - rl * N;
@@ -537,28 +684,24 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_fwd(const T* __
             }
         }
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
             if (i < tot) {
                 const int rl = div16(i, mN), j = i - rl * N;
                 const int p = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
-                a[rl * N + p] = cscale(xv[u], (typename Real<T>::type)wv[u]);
+                a[rl * RS + p] = cscale(xv[u], (typename Real<T>::type)wv[u]);
             }
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
-        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
-        twp = twl;
-    }
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
     __syncthreads();
-    T* V = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp);
+    T* V = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const unsigned long long mB = magic16(B);
     for (int i = threadIdx.x; i < nr * B; i += blockDim.x) {
         const int rl = div16(i, mB), k = i - rl * B;
         const int r = row_of(q0 + rl, N, cut);
-        const T* v = V + rl * N;
+        const T* v = V + rl * RS;
         const T e = ph[k];
         T c = cadd(cmul(e, v[k]), cmul(cconj(e), v[k ? N - k : 0]));
         if (k == 0) c = cscale(c, (typename Real<T>::type)0.70710678118654752440);
@@ -1065,13 +1208,15 @@ __global__ __launch_bounds__(256) void k_leg_inv(const double* __restrict__ cre,
 // x sin(theta) for odd orders, x (-1) on odd negative rows, x 1/sqrt(2pi).
 // REAL mirrors row N-m = conj(row m).  grid (ceil(nrows/rpb), F).
 template <typename T, int MODE>
-__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
                                                  const T* __restrict__ tw, const T* __restrict__ ph,
                                                  const double* __restrict__ sinv, int N, int logN, int pow2, int B,
                                                  int cut, int nrows, int rpb, int fmt) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
-    T* b = a + rpb * N;
+    const int RS = row_stride<MODE>(N, logN);
+    T* b = a + lds_rows<T, MODE>(rpb, logN) * RS;
+    constexpr int FB = kFillBatch;
     int bx = blockIdx.x, f = blockIdx.y;
     if constexpr (MODE != 0) xcd_block(bx, f);
     const int q0 = bx * rpb;
@@ -1081,10 +1226,10 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_inv(const T* __
     // of the two terms is nonzero, X[k] for k < B, X[N-k] for k > B.
     const int tot = nr * N;  // < 2^16 (div16)
     const unsigned long long mN = magic16(N);
-    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
-        T xv[kFillBatch], e[kFillBatch];
+    for (int i0 = threadIdx.x; i0 < tot; i0 += FB * blockDim.x) {
+        T xv[FB], e[FB];
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
             if (i < tot) {
                 const int rl = div16(i, mN), k = i - rl * N;
@@ -1095,23 +1240,19 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_inv(const T* __
             }
         }
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
             if (i < tot) {
-                const int k = i - div16(i, mN) * N;
+                const int rl = div16(i, mN), k = i - rl * N;
                 const T x = xv[u];
-                a[i] = cmul(cconj(e[u]), (k <= B) ? x : mk<T>(x.y, -x.x));  // k = B: x is zero
+                a[rl * RS + k] = cmul(cconj(e[u]), (k <= B) ? x : mk<T>(x.y, -x.x));  // k = B: x is zero
             }
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
-        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
-        twp = twl;
-    }
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
     __syncthreads();
-    T* u = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp);
+    T* u = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const R inv_s2pi = (R)0.39894228040143267794;  // 1/sqrt(2 pi)
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const int rl = div16(i, mN), j = i - rl * N;
@@ -1121,7 +1262,7 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_dct_inv(const T* __
         R sc = inv_s2pi;
         if (am & 1) sc *= (R)sinv[j];
         if (r > B && (r & 1)) sc = -sc;
-        const T y = cscale(u[rl * N + n], sc);
+        const T y = cscale(u[rl * RS + n], sc);
         X[((size_t)f * N + r) * N + j] = y;
         if (fmt == 1 && r > 0) X[((size_t)f * N + (N - r)) * N + j] = cconj(y);
     }
@@ -1170,22 +1311,24 @@ __global__ __launch_bounds__(256) void k_naive_inv(const double* __restrict__ cr
 // Inverse ring FFT (+i, unnormalised) per ring; row B (Nyquist) is zero.
 // grid (ceil(N/rpb), F).
 template <typename T, int MODE>
-__global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
+__global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) void k_phi_inv(const T* __restrict__ X, double* __restrict__ re,
                                                  double* __restrict__ im, int N, int logN, int pow2, int B, int rpb,
                                                  const T* __restrict__ tw) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
-    T* b = a + rpb * N;
+    const int RS = row_stride<MODE>(N, logN);
+    T* b = a + lds_rows<T, MODE>(rpb, logN) * RS;
+    constexpr int FB = kFillBatch;
     int bx = blockIdx.x, f = blockIdx.y;
     if constexpr (MODE != 0) xcd_block(bx, f);  // neighbouring ring blocks share 128-byte lines: same L2
     const int j0 = bx * rpb;
     const int nr = min(rpb, N - j0);
     const int tot = nr * N;  // < 2^16 (div16)
     const unsigned long long mnr = magic16(nr);
-    for (int i0 = threadIdx.x; i0 < tot; i0 += kFillBatch * blockDim.x) {
-        T v[kFillBatch];
+    for (int i0 = threadIdx.x; i0 < tot; i0 += FB * blockDim.x) {
+        T v[FB];
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
             if (i < tot) {
                 const int r = div16(i, mnr), jl = i - r * nr;
@@ -1193,26 +1336,24 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) void k_phi_inv(const T* __
             }
         }
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * blockDim.x;
             if (i < tot) {
                 const int r = div16(i, mnr), jl = i - r * nr;
-                a[jl * N + r] = v[u];
+                a[jl * RS + r] = v[u];
             }
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) {  // twiddles in LDS: the mixed-radix passes read them at scattered indices
-        T* twl = MODE == 2 ? b : b + rpb * N;  // MODE 2 transforms in place: no second buffer
-        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
-        twp = twl;
-    }
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
     __syncthreads();
-    T* y = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp);
+    T* y = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const size_t base = ((size_t)f * N + j0) * N;
+    const unsigned long long mN = magic16(N);
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-        re[base + i] = (double)y[i].x;
-        im[base + i] = (double)y[i].y;
+        const T v = y[lds_row<MODE>(i, N, RS, mN)];
+        re[base + i] = (double)v.x;
+        im[base + i] = (double)v.y;
     }
 }
 
@@ -1502,42 +1643,127 @@ static int ilog2_exact(int n) {
 #ifndef S2_MR_ELEMS32
 #define S2_MR_ELEMS32 4096
 #endif
+template <typename T>
 struct FftCfg {
     int mode;    // instance
     int rpb;     // rows (rings or order rows) per block
+    int lg;      // the kernels' logN argument: log2 N, or M for Bluestein
     size_t lds;  // dynamic LDS bytes
+    const T* tw; // the kernels' tw argument: twiddles, or the Bluestein tables
 };
 
-static bool rm_supported(int N) {
+// N has no prime factor above pmax
+static bool rm_supported(int N, int pmax = kRegMaxPrime) {
     int rem = N;
     while (rem % 2 == 0) rem /= 2;
-    for (int p = 3; p <= kRegMaxPrime; p += 2)
+    for (int p = 3; p <= pmax; p += 2)
         while (rem % p == 0) rem /= p;
     return rem == 1;
 }
 
 template <typename T>
-static FftCfg fft_cfg(int N, bool pow2) {
+static FftCfg<T> fft_cfg(const DevConsts& c) {
+    const int N = c.N;
     if (N > 2048) throw ArgError{"ring length above 2048"};  // block loops index below 2^16
-    constexpr int kLdsValues = 163840 / (int)sizeof(T);     // gfx950: 160 KiB per workgroup
-    if (pow2) {
+    constexpr bool f64 = sizeof(T) == 16;
+    const T* tw = reinterpret_cast<const T*>(f64 ? (const void*)c.tw : (const void*)c.tw32);
+    constexpr int kLdsValues = 163840 / (int)sizeof(T);  // gfx950: 160 KiB per workgroup
+    if (c.pow2) {
         const int rpb = N >= 2048 ? 1 : 2048 / N;
-        return {0, rpb, 2 * sizeof(T) * rpb * N};
+        return {0, rpb, c.logN, 2 * sizeof(T) * rpb * N, tw};
     }
     if (rm_supported(N)) {
         const int rpb = std::max(1, std::min(rm_elems<T>() / N, (kLdsValues - N) / N));
-        return {2, rpb, sizeof(T) * ((size_t)rpb * N + N)};
+        return {2, rpb, c.logN, sizeof(T) * ((size_t)rpb * N + N), tw};
     }
-    const int elems = sizeof(T) == 16 ? S2_MR_ELEMS : S2_MR_ELEMS32;
+    if (c.blu_M > 0) {
+        const int M = c.blu_M, rpb = blu_rows<T>(M);
+        const T* g = reinterpret_cast<const T*>(f64 ? (const void*)c.blu : (const void*)c.blu32);
+        return {is_pow2(M) ? 3 : 4, rpb, M, sizeof(T) * ((size_t)rpb * blu_stride(M) + M), g};
+    }
+    const int elems = f64 ? S2_MR_ELEMS : S2_MR_ELEMS32;
     const int rpb = std::max(1, std::min(elems / N, (kLdsValues - N) / (2 * N)));
-    return {1, rpb, sizeof(T) * (2 * (size_t)rpb * N + N)};
+    return {1, rpb, c.logN, sizeof(T) * (2 * (size_t)rpb * N + N), tw};
 }
 
-template <typename K0, typename K1, typename K2, typename... A>
-static void fft_launch(const FftCfg& g, K0 k0, K1 k1, K2 k2, dim3 grid, hipStream_t s, A... args) {
-    if (g.mode == 0) launch_lds(k0, grid, 256, g.lds, s, args...);
-    else if (g.mode == 1) launch_lds(k1, grid, kMrThreads, g.lds, s, args...);
-    else launch_lds(k2, grid, kMrThreads, g.lds, s, args...);
+// FFT_INSTANCES(k, T): the five instances of a per-field FFT/DCT kernel.
+#define FFT_INSTANCES(k, T) k<T, 0>, k<T, 1>, k<T, 2>, k<T, 3>, k<T, 4>
+template <typename T, typename K0, typename K1, typename K2, typename K3, typename K4, typename... A>
+static void fft_launch(const FftCfg<T>& g, K0 k0, K1 k1, K2 k2, K3 k3, K4 k4, dim3 grid, hipStream_t s,
+                       A... args) {
+    switch (g.mode) {
+        case 0: launch_lds(k0, grid, 256, g.lds, s, args...); break;
+        case 1: launch_lds(k1, grid, kMrThreads, g.lds, s, args...); break;
+        case 2: launch_lds(k2, grid, kMrThreads, g.lds, s, args...); break;
+        case 3: launch_lds(k3, grid, kMrThreads, g.lds, s, args...); break;
+        default: launch_lds(k4, grid, kMrThreads, g.lds, s, args...); break;
+    }
+}
+
+// Bluestein tables of ring length N (layout in DevConsts), in long double.
+// M: the power of two >= 2N-1 when it is at most 5/4 of the smallest
+// length >= 2N-1 whose primes are at most kBluMaxPrime (radix-8 passes beat
+// the mixed-radix ones on nearly equal lengths), else that smooth length.
+// The chirp's angle is reduced exactly (t^2 mod 2N); FFT_M(b) is a
+// recursive mixed-radix transform whose twiddles are evaluated directly.
+typedef long double LD;
+static const LD kPiL = 3.141592653589793238462643383279502884L;
+
+static void host_fft(std::vector<LD>& re, std::vector<LD>& im) {  // exp(-2 pi i / n), any n
+    const int n = (int)re.size();
+    if (n == 1) return;
+    int p = 2;
+    while (n % p) ++p;
+    const int m = n / p;
+    std::vector<std::vector<LD>> sr(p, std::vector<LD>(m)), si(p, std::vector<LD>(m));
+    for (int r = 0; r < p; ++r)
+        for (int j = 0; j < m; ++j) {
+            sr[r][j] = re[(size_t)j * p + r];
+            si[r][j] = im[(size_t)j * p + r];
+        }
+    for (int r = 0; r < p; ++r) host_fft(sr[r], si[r]);
+    for (int k = 0; k < n; ++k) {
+        LD ar = 0, ai = 0;
+        for (int r = 0; r < p; ++r) {
+            const LD a = -2 * kPiL * (LD)(((long long)r * k) % n) / n, cr = cosl(a), ci = sinl(a);
+            const LD xr = sr[r][k % m], xi = si[r][k % m];
+            ar += xr * cr - xi * ci;
+            ai += xr * ci + xi * cr;
+        }
+        re[k] = ar;
+        im[k] = ai;
+    }
+}
+
+static int bluestein_length(int N) {
+    int p2 = 1;
+    while (p2 < 2 * N - 1) p2 <<= 1;
+    int ms = 2 * N - 1;
+    while (!rm_supported(ms, kBluMaxPrime)) ++ms;
+    return 4 * p2 <= 5 * ms ? p2 : ms;
+}
+
+static void bluestein_tables(int N, int& M, std::vector<double2>& out) {
+    M = bluestein_length(N);
+    out.assign((size_t)2 * M + N, make_double2(0.0, 0.0));
+    for (int t = 0; t < M; ++t) {
+        const LD a = 2 * kPiL * t / M;
+        out[t] = make_double2((double)cosl(a), (double)-sinl(a));
+    }
+    std::vector<LD> br(M, 0), bi(M, 0);
+    for (int t = 0; t < N; ++t) {
+        const long long q = ((long long)t * t) % (2LL * N);
+        const LD a = kPiL * (LD)q / N, wr = cosl(a), wi = -sinl(a);
+        out[M + t] = make_double2((double)wr, (double)wi);
+        br[t] = wr;  // b_t = conj(w_t)
+        bi[t] = -wi;
+        if (t > 0) {
+            br[M - t] = wr;
+            bi[M - t] = -wi;
+        }
+    }
+    host_fft(br, bi);
+    for (int k = 0; k < M; ++k) out[M + N + k] = make_double2((double)(br[k] / M), (double)(bi[k] / M));
 }
 
 void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s) {
@@ -1599,13 +1825,25 @@ void consts_create(DevConsts& c, int bw, const double* h_weights, hipStream_t s)
     k_to_float<<<(2 * N + 255) / 256, 256, 0, s>>>(c.w_semi, c.w_semi32, 2 * N);
     S2_HIP(hipGetLastError());
     S2_HIP(hipMemcpyAsync(c.deg, deg.data(), sizeof(int) * deg.size(), hipMemcpyHostToDevice, s));
+    std::vector<double2> blu;
+    std::vector<float2> blu32;
+    if (!c.pow2 && !rm_supported(N)) {
+        bluestein_tables(N, c.blu_M, blu);
+        blu32.resize(blu.size());
+        for (size_t i = 0; i < blu.size(); ++i) blu32[i] = make_float2((float)blu[i].x, (float)blu[i].y);
+        S2_HIP(hipMalloc(&c.blu, sizeof(double2) * blu.size()));
+        S2_HIP(hipMalloc(&c.blu32, sizeof(float2) * blu.size()));
+        S2_HIP(hipMemcpyAsync(c.blu, blu.data(), sizeof(double2) * blu.size(), hipMemcpyHostToDevice, s));
+        S2_HIP(hipMemcpyAsync(c.blu32, blu32.data(), sizeof(float2) * blu.size(), hipMemcpyHostToDevice, s));
+    }
     S2_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
     S2_HIP(hipFree(d_w));
 }
 
 void consts_destroy(DevConsts& c) {
     for (void* p : {(void*)c.w_semi, (void*)c.w_naive, (void*)c.tw, (void*)c.ph, (void*)c.sinv, (void*)c.deg,
-                    (void*)c.tw32, (void*)c.ph32, (void*)c.w_semi32, (void*)c.sinv32, (void*)c.trow})
+                    (void*)c.tw32, (void*)c.ph32, (void*)c.w_semi32, (void*)c.sinv32, (void*)c.trow, (void*)c.blu,
+                    (void*)c.blu32})
         if (p) (void)hipFree(p);
     c = DevConsts{};
 }
@@ -1769,19 +2007,19 @@ static size_t reg_lds(int N) { return sizeof(double2) * ((size_t)reg_rpb(N) * (N
 // 1024-thread blocks slower at both sizes.
 void ring_fft_fwd_f32(const DevConsts& c, const double* re, const double* im, float2* X, int F, hipStream_t s) {
     const int N = c.N;
-    const FftCfg fg = fft_cfg<float2>(N, false);
+    const FftCfg<float2> fg = fft_cfg<float2>(c);
     const int rpb = fg.rpb;
-    fft_launch(fg, k_phi_fwd<float2, 0>, k_phi_fwd<float2, 1>, k_phi_fwd<float2, 2>, dim3((N + rpb - 1) / rpb, F), s,
-               re, im, X, N, c.logN, 0, rpb, c.tw32);
+    fft_launch(fg, FFT_INSTANCES(k_phi_fwd, float2), dim3((N + rpb - 1) / rpb, F), s,
+               re, im, X, N, fg.lg, 0, rpb, fg.tw);
     S2_HIP(hipGetLastError());
 }
 
 void ring_fft_inv_f32(const DevConsts& c, const float2* X, double* re, double* im, int F, hipStream_t s) {
     const int N = c.N;
-    const FftCfg fg = fft_cfg<float2>(N, false);
+    const FftCfg<float2> fg = fft_cfg<float2>(c);
     const int rpb = fg.rpb;
-    fft_launch(fg, k_phi_inv<float2, 0>, k_phi_inv<float2, 1>, k_phi_inv<float2, 2>, dim3((N + rpb - 1) / rpb, F), s,
-               X, re, im, N, c.logN, 0, c.bw, rpb, c.tw32);
+    fft_launch(fg, FFT_INSTANCES(k_phi_inv, float2), dim3((N + rpb - 1) / rpb, F), s,
+               X, re, im, N, fg.lg, 0, c.bw, rpb, fg.tw);
     S2_HIP(hipGetLastError());
 }
 
@@ -1789,15 +2027,15 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
                   double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
-    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const FftCfg<double2> fg = fft_cfg<double2>(c);
     const int rpb = fg.rpb;
     {
     Span sp(tm, ST_PHI_FWD, s);
     if (reg_rings(c))
         launch_lds(k_fphi_fwd<double, 8>, dim3(N / reg_rpb(N), F), kThreads, reg_lds(N), s, re, im, X, c.logN, c.tw);
     else
-        fft_launch(fg, k_phi_fwd<double2, 0>, k_phi_fwd<double2, 1>, k_phi_fwd<double2, 2>, dim3((N + rpb - 1) / rpb, F),
-                   s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
+        fft_launch(fg, FFT_INSTANCES(k_phi_fwd, double2), dim3((N + rpb - 1) / rpb, F),
+                   s, re, im, X, N, fg.lg, c.pow2, rpb, fg.tw);
     S2_HIP(hipGetLastError());
     }
     if (cut > 0) {
@@ -1810,8 +2048,8 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
             launch_lds(k_fdct_fwd<double, 8>, dim3((nrows + reg_rpb(N) - 1) / reg_rpb(N), F), kThreads, reg_lds(N), s,
                        (const double2*)X, (double*)Cb, c.w_semi, c.logN, cut, nrows, c.tw, c.ph);
         else
-            fft_launch(fg, k_dct_fwd<double2, 0>, k_dct_fwd<double2, 1>, k_dct_fwd<double2, 2>,
-                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+            fft_launch(fg, FFT_INSTANCES(k_dct_fwd, double2),
+                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, fg.tw, c.ph, N, fg.lg, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
         }
         Span sp(tm, ST_LEG_FWD, s);
@@ -1844,7 +2082,7 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
                   double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = t.cutoff, f = (int)fmt;
-    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const FftCfg<double2> fg = fft_cfg<double2>(c);
     const int rpb = fg.rpb;
     // the register DCT reads the row GEMVs' planar rows
     const int planar = (reg_rings(c) && F <= kRowsMaxFields) ? 1 : 0;
@@ -1874,8 +2112,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
             launch_lds(k_fdct_inv<double, 8>, dim3((nrows + reg_rpb(N) - 1) / reg_rpb(N), F), kThreads, reg_lds(N), s,
                        (const double*)Cb, X, c.sinv, c.logN, cut, nrows, f, c.tw, c.ph);
         else
-            fft_launch(fg, k_dct_inv<double2, 0>, k_dct_inv<double2, 1>, k_dct_inv<double2, 2>,
-                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+            fft_launch(fg, FFT_INSTANCES(k_dct_inv, double2),
+                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, fg.tw, c.ph, c.sinv, N, fg.lg, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
@@ -1890,8 +2128,8 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
         launch_lds(k_fphi_inv<double, 8>, dim3(N / reg_rpb(N), F), kThreads, reg_lds(N), s, (const double2*)X, re, im,
                    c.logN, c.tw);
     else
-        fft_launch(fg, k_phi_inv<double2, 0>, k_phi_inv<double2, 1>, k_phi_inv<double2, 2>, dim3((N + rpb - 1) / rpb, F),
-                   s, X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
+        fft_launch(fg, FFT_INSTANCES(k_phi_inv, double2), dim3((N + rpb - 1) / rpb, F),
+                   s, X, re, im, N, fg.lg, c.pow2, B, rpb, fg.tw);
     S2_HIP(hipGetLastError());
 }
 
@@ -1899,19 +2137,19 @@ void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re
                        double* cim, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = g.cut;
-    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const FftCfg<double2> fg = fft_cfg<double2>(c);
     const int rpb = fg.rpb;
     {
         Span sp(tm, ST_PHI_FWD, s);
-        fft_launch(fg, k_phi_fwd<double2, 0>, k_phi_fwd<double2, 1>, k_phi_fwd<double2, 2>, dim3((N + rpb - 1) / rpb, F),
-                   s, re, im, X, N, c.logN, c.pow2, rpb, c.tw);
+        fft_launch(fg, FFT_INSTANCES(k_phi_fwd, double2), dim3((N + rpb - 1) / rpb, F),
+                   s, re, im, X, N, fg.lg, c.pow2, rpb, fg.tw);
         S2_HIP(hipGetLastError());
     }
     if (cut > 0) {
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_FWD, s);
-        fft_launch(fg, k_dct_fwd<double2, 0>, k_dct_fwd<double2, 1>, k_dct_fwd<double2, 2>,
-                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, c.tw, c.ph, N, c.logN, c.pow2, B, cut, nrows, rpb);
+        fft_launch(fg, FFT_INSTANCES(k_dct_fwd, double2),
+                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, fg.tw, c.ph, N, fg.lg, c.pow2, B, cut, nrows, rpb);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_LEG_FWD, s);
@@ -1922,7 +2160,7 @@ void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cr
                        double* im, int F, Fmt fmt, double2* X, double2* Cb, hipStream_t s, StageTimer* tm) {
     check_launch_shape(c);
     const int N = c.N, B = c.bw, cut = g.cut, f = (int)fmt;
-    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const FftCfg<double2> fg = fft_cfg<double2>(c);
     const int rpb = fg.rpb;
     {
         Span sp(tm, ST_LEG_INV, s);
@@ -1931,13 +2169,13 @@ void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cr
     if (cut > 0) {
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_INV, s);
-        fft_launch(fg, k_dct_inv<double2, 0>, k_dct_inv<double2, 1>, k_dct_inv<double2, 2>,
-                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, c.tw, c.ph, c.sinv, N, c.logN, c.pow2, B, cut, nrows, rpb, f);
+        fft_launch(fg, FFT_INSTANCES(k_dct_inv, double2),
+                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, fg.tw, c.ph, c.sinv, N, fg.lg, c.pow2, B, cut, nrows, rpb, f);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
-    fft_launch(fg, k_phi_inv<double2, 0>, k_phi_inv<double2, 1>, k_phi_inv<double2, 2>, dim3((N + rpb - 1) / rpb, F), s,
-               X, re, im, N, c.logN, c.pow2, B, rpb, c.tw);
+    fft_launch(fg, FFT_INSTANCES(k_phi_inv, double2), dim3((N + rpb - 1) / rpb, F), s,
+               X, re, im, N, fg.lg, c.pow2, B, rpb, fg.tw);
     S2_HIP(hipGetLastError());
 }
 
@@ -1949,10 +2187,10 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
     const int N = c.N, B = c.bw;
     k_ring_sum<<<dim3((N + 3) / 4, F), 256, 0, s>>>(re, im, X, N);
     S2_HIP(hipGetLastError());
-    const FftCfg fg = fft_cfg<double2>(N, c.pow2);
+    const FftCfg<double2> fg = fft_cfg<double2>(c);
     const int rpb = fg.rpb;
-    fft_launch(fg, k_dct_fwd<double2, 0>, k_dct_fwd<double2, 1>, k_dct_fwd<double2, 2>, dim3(1, F), s, X, Cb, c.w_semi,
-               c.tw, c.ph, N, c.logN, c.pow2, B, t.cutoff, 1, 1);
+    fft_launch(fg, FFT_INSTANCES(k_dct_fwd, double2), dim3(1, F), s, X, Cb, c.w_semi,
+               fg.tw, c.ph, N, fg.lg, c.pow2, B, t.cutoff, 1, 1);
     S2_HIP(hipGetLastError());
     const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
     k_leg_fwd<<<dim3(1, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(

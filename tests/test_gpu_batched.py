@@ -48,7 +48,11 @@ def to_dev(torch, dev, arrs):
                                                 (100, 100, 0, 2), (100, 100, 1, 2), (255, 255, 0, 2), (255, 255, 1, 2),
                                                 (45, 45, 0, 2), (33, 33, 1, 2), (150, 150, 0, 2), (300, 75, 1, 2),
                                                 (32, 32, 0, 3), (32, 32, 1, 3), (64, 64, 0, 3), (128, 128, 1, 3),
-                                                (256, 256, 0, 3), (256, 256, 1, 3)])
+                                                (256, 256, 0, 3), (256, 256, 1, 3),
+                                                # rings with a prime factor above 19 (Bluestein FFTs): 2*127,
+                                                # 2*59, 2*37, 2*47, 2*263
+                                                (127, 127, 0, 0), (59, 20, 1, 0), (37, 0, 0, 0), (127, 127, 0, 2),
+                                                (47, 47, 1, 2), (37, 37, 0, 2), (263, 263, 0, 2)])
 def test_plan_matches_oracle(env, O, bw, cutoff, fmt, prec):
     S, torch, dev = env
     print(f"plan bw={bw} cutoff={cutoff} fmt={fmt} prec={prec}", flush=True)
@@ -272,13 +276,16 @@ def test_cutoff_plans_agree(env, bw, prec, tol, fmt):
         assert eg < tol and ec < tol
 
 
-def test_large_bw_not_power_of_two_round_trip(env):
-    # bw=1000: 2000 = 4*4*5*5*5 rings (mixed-radix FFTs and DCTs), 1008-double
-    # LDS rows (GEMM tiles in up to 160 KiB); two fields take the GEMM path.
-    # Inverse then forward is the identity on band-limited coefficients (bw even).
+@pytest.mark.parametrize("bw", [1000, 1018])
+def test_large_bw_not_power_of_two_round_trip(env, bw):
+    # bw=1000: 2000 = 4*4*5*5*5 rings (register mixed-radix FFTs and DCTs);
+    # bw=1018: 2036 = 4*509 rings (Bluestein, M = 4096, one ring per block);
+    # 1008- and 1024-double LDS rows (GEMM tiles in up to 160 KiB); two fields
+    # take the GEMM path.  Inverse then forward is the identity on band-limited
+    # coefficients (bw even).
     S, torch, dev = env
     from s2kit_amd import synth
-    bw, F = 1000, 2
+    F = 2
     plan = S.Plan(bw, F, precision=0)
     cre, cim = synth.power_law_coeffs(bw, F, 78, dev)
     n = 2 * bw

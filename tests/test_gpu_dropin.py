@@ -92,7 +92,7 @@ def inv_with(S, rc, ic, bw, tabs, fmt, cutoff):
 
 
 CASES = [(8, 8, 0), (17, 17, 0), (23, 23, 0), (64, 64, 0), (64, 16, 0), (64, 1, 0), (64, 0, 0), (64, 64, 1),
-         (64, 20, 1), (128, 128, 0), (256, 256, 0)]
+         (64, 20, 1), (128, 128, 0), (256, 256, 0), (31, 31, 1), (127, 40, 0)]
 
 
 @pytest.mark.parametrize("bw,cutoff,fmt", CASES)
