@@ -4,9 +4,11 @@
 //   k_fphi_fwd -> k_fdct_fwd -> k_gemm_fwd     forward  (ring FFT, DCT: s2_blocks.h)
 //   k_gemm_inv -> k_fdct_inv -> k_fphi_inv     inverse
 //
-// Other bw in [16, 1024] (FP64): the per-field kernels' mixed-radix FFTs and
-// DCTs (s2_kernels.hip) around the same GEMMs, which then read and write the
-// natural-order complex DCT rows (GEN).
+// Other bw in [16, 1024] (FP64, GEN): the per-field kernels' mixed-radix or
+// Bluestein FFTs and DCTs (s2_kernels.hip) around the same GEMMs; those DCTs
+// then read and write the same parity-split re/im planes (their planar mode),
+// whose rows the GEMMs stream in whole 128-byte lines (with natural-order
+// complex rows the inverse GEMM's stores moved 1.66x their bytes at bw=255).
 //
 //   k_gemm_fwd   seminaive projections as MFMA GEMMs            (seminaive.c:183-197)
 //                naive projections (orders >= cutoff) likewise  (naive.c:34-59)
@@ -118,10 +120,10 @@ __host__ __device__ constexpr int lds_pad() {
 }
 
 // Extents of the GEMMs at bandwidth B.  Cosine index k = 2t + p: hE = ceil(B/2)
-// even and hO = floor(B/2) odd ones.  Power-of-two B uses the fast pipeline's
-// parity-split Cb planes; any other B (gen) the per-field path's natural-order
-// complex DCT rows, with an LDS row long enough for the K loops' round-up
-// (B + 3) and the inverse's two padded parities (B + 6), zero past B.
+// even and hO = floor(B/2) odd ones, at positions t and hE + t of the Cb
+// planes.  Any B that is not a power of two (gen) takes an LDS row long
+// enough for the K loops' round-up (B + 3) and the inverse's two padded
+// parities (B + 6), zero past B.
 struct GGeo {
     int B, N, hE, hO, rs, B4, B16, Tp16;
 };
@@ -241,19 +243,17 @@ __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __res
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
     const int rs = G.rs;
-    if constexpr (GEN) {  // natural-order complex rows; zero past B
-        const double2* C2 = reinterpret_cast<const double2*>(Cb);
+    if constexpr (GEN) {  // the same planes at any B; zero past B (the K loops' round-up)
         for (int idx = threadIdx.x; idx < ncb * rs; idx += kThreads) {
-            const int c = idx / rs, k = idx - c * rs;
+            const int c = idx / rs, k2 = idx - c * rs;
             const int col = cbase + c;
             R v = 0;
-            if (col < ncols && k < B) {
+            if (col < ncols && k2 < B) {
                 const int f = col / cpf, rem = col - f * cpf;
                 const int r = (rem >> 1) ? N - m : m;
-                const double2 x = C2[((size_t)f * N + r) * B + k];
-                v = (R)((rem & 1) ? x.y : x.x);
+                v = Cb[(((size_t)f * N + r) * 2 + (rem & 1)) * B + k2];
             }
-            Bd[c * rs + (k < B ? ((k & 1) ? G.hE + (k >> 1) : (k >> 1)) : k)] = v;
+            Bd[c * rs + k2] = v;
         }
     } else {
         const int lgB = __builtin_ctz(B);
@@ -455,10 +455,7 @@ __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* 
                     if (col >= ncols) continue;
                     const int f = col / cpf, rem = col - f * cpf;
                     const int row = (rem >> 1) ? N - m : m;
-                    if constexpr (GEN)
-                        reinterpret_cast<R*>(Cb)[(((size_t)f * N + row) * B + 2 * t + p) * 2 + (rem & 1)] = acc[c][r];
-                    else
-                        Cb[(((size_t)f * N + row) * 2 + (rem & 1)) * B + p * G.hE + t] = acc[c][r];
+                    Cb[(((size_t)f * N + row) * 2 + (rem & 1)) * B + p * G.hE + t] = acc[c][r];
                 }
         }
     }

@@ -657,7 +657,7 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) void k_dct_fwd(const T* __restrict__ X, T* __restrict__ Cb,
                                                  const double* __restrict__ w_semi, const T* __restrict__ tw,
                                                  const T* __restrict__ ph, int N, int logN, int pow2, int B,
-                                                 int cut, int nrows, int rpb) {
+                                                 int cut, int nrows, int rpb, int planar) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     const int RS = row_stride<MODE>(N, logN);
@@ -705,7 +705,14 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) voi
         const T e = ph[k];
         T c = cadd(cmul(e, v[k]), cmul(cconj(e), v[k ? N - k : 0]));
         if (k == 0) c = cscale(c, (typename Real<T>::type)0.70710678118654752440);
-        Cb[((size_t)f * N + r) * B + k] = c;
+        if (planar) {  // re and im planes, even cosine indices first (the GEMMs' layout)
+            auto* row = reinterpret_cast<typename Real<T>::type*>(Cb) + ((size_t)f * N + r) * 2 * B;
+            const int pos = (k & 1) ? ((B + 1) >> 1) + (k >> 1) : (k >> 1);
+            row[pos] = c.x;
+            row[B + pos] = c.y;
+        } else {
+            Cb[((size_t)f * N + r) * B + k] = c;
+        }
     }
 }
 
@@ -1211,7 +1218,7 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) void k_dct_inv(const T* __restrict__ Cb, T* __restrict__ X,
                                                  const T* __restrict__ tw, const T* __restrict__ ph,
                                                  const double* __restrict__ sinv, int N, int logN, int pow2, int B,
-                                                 int cut, int nrows, int rpb, int fmt) {
+                                                 int cut, int nrows, int rpb, int fmt, int planar) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* a = reinterpret_cast<T*>(smem);
     const int RS = row_stride<MODE>(N, logN);
@@ -1234,8 +1241,16 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) voi
             if (i < tot) {
                 const int rl = div16(i, mN), k = i - rl * N;
                 const int r = row_of(q0 + rl, N, cut);
-                const T* crow = Cb + ((size_t)f * N + r) * B;
-                xv[u] = (k != B) ? crow[k < B ? k : N - k] : mk<T>(0, 0);
+                const int ks = k < B ? k : N - k;
+                if (k == B) {
+                    xv[u] = mk<T>(0, 0);
+                } else if (planar) {  // re and im planes, even cosine indices first
+                    const R* row = reinterpret_cast<const R*>(Cb) + ((size_t)f * N + r) * 2 * B;
+                    const int pos = (ks & 1) ? ((B + 1) >> 1) + (ks >> 1) : (ks >> 1);
+                    xv[u] = mk<T>(row[pos], row[B + pos]);
+                } else {
+                    xv[u] = Cb[((size_t)f * N + r) * B + ks];
+                }
                 e[u] = ph[k];
             }
         }
@@ -2049,7 +2064,7 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
                        (const double2*)X, (double*)Cb, c.w_semi, c.logN, cut, nrows, c.tw, c.ph);
         else
             fft_launch(fg, FFT_INSTANCES(k_dct_fwd, double2),
-                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, fg.tw, c.ph, N, fg.lg, c.pow2, B, cut, nrows, rpb);
+                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, fg.tw, c.ph, N, fg.lg, c.pow2, B, cut, nrows, rpb, 0);
         S2_HIP(hipGetLastError());
         }
         Span sp(tm, ST_LEG_FWD, s);
@@ -2113,7 +2128,7 @@ void inverse_fp64(const DevConsts& c, const DevTables& t, const double* cre, con
                        (const double*)Cb, X, c.sinv, c.logN, cut, nrows, f, c.tw, c.ph);
         else
             fft_launch(fg, FFT_INSTANCES(k_dct_inv, double2),
-                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, fg.tw, c.ph, c.sinv, N, fg.lg, c.pow2, B, cut, nrows, rpb, f);
+                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, fg.tw, c.ph, c.sinv, N, fg.lg, c.pow2, B, cut, nrows, rpb, f, 0);
         S2_HIP(hipGetLastError());
     }
     if (cut < B) {
@@ -2149,7 +2164,7 @@ void forward_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* re
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_FWD, s);
         fft_launch(fg, FFT_INSTANCES(k_dct_fwd, double2),
-                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, fg.tw, c.ph, N, fg.lg, c.pow2, B, cut, nrows, rpb);
+                       dim3((nrows + rpb - 1) / rpb, F), s, X, Cb, c.w_semi, fg.tw, c.ph, N, fg.lg, c.pow2, B, cut, nrows, rpb, 1);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_LEG_FWD, s);
@@ -2170,7 +2185,7 @@ void inverse_fp64_gemm(const DevConsts& c, const GemmTables& g, const double* cr
         const int nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
         Span sp(tm, ST_DCT_INV, s);
         fft_launch(fg, FFT_INSTANCES(k_dct_inv, double2),
-                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, fg.tw, c.ph, c.sinv, N, fg.lg, c.pow2, B, cut, nrows, rpb, f);
+                       dim3((nrows + rpb - 1) / rpb, F), s, Cb, X, fg.tw, c.ph, c.sinv, N, fg.lg, c.pow2, B, cut, nrows, rpb, f, 1);
         S2_HIP(hipGetLastError());
     }
     Span sp(tm, ST_PHI_INV, s);
@@ -2187,10 +2202,9 @@ void zonal_fp64(const DevConsts& c, const DevTables& t, const double* re, const 
     const int N = c.N, B = c.bw;
     k_ring_sum<<<dim3((N + 3) / 4, F), 256, 0, s>>>(re, im, X, N);
     S2_HIP(hipGetLastError());
-    const FftCfg<double2> fg = fft_cfg<double2>(c);
-    const int rpb = fg.rpb;
+    const FftCfg<double2> fg = fft_cfg<double2>(c);  // one order row (m = 0) per field
     fft_launch(fg, FFT_INSTANCES(k_dct_fwd, double2), dim3(1, F), s, X, Cb, c.w_semi,
-               fg.tw, c.ph, N, fg.lg, c.pow2, B, t.cutoff, 1, 1);
+               fg.tw, c.ph, N, fg.lg, c.pow2, B, t.cutoff, 1, 1, 0);
     S2_HIP(hipGetLastError());
     const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
     k_leg_fwd<<<dim3(1, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
