@@ -92,19 +92,19 @@ def smooth(n, pmax):
 
 def ring_fft_instance(n):
     """Instance of the per-field FFT/DCT kernels for ring length n that is
-    not a power of two (the host's choice in s2_kernels.hip, fft_cfg): 2 the
+    not a power of two (the host's choice in s2_kernels.hip, fft_cfg): 1 the
     register mixed-radix passes (prime factors up to 19), else Bluestein with
-    a convolution of length M >= 2n-1, 3 for a power-of-two M (taken when
-    within 5/4 of the smallest 7-smooth length), 4 for that 7-smooth M."""
+    a convolution of length M >= 2n-1, 2 for a power-of-two M (taken when
+    within 5/4 of the smallest 7-smooth length), 3 for that 7-smooth M."""
     if smooth(n, 19):
-        return 2
+        return 1
     p2 = 1
     while p2 < 2 * n - 1:
         p2 <<= 1
     ms = 2 * n - 1
     while not smooth(ms, 7):
         ms += 1
-    return 3 if 4 * p2 <= 5 * ms else 4
+    return 2 if 4 * p2 <= 5 * ms else 3
 
 
 def kernels_gen(bw, form):

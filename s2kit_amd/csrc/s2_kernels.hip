@@ -42,174 +42,15 @@ __device__ __forceinline__ unsigned long long magic16(unsigned d) {
 }
 __device__ __forceinline__ int div16(int x, unsigned long long mg) { return (int)(((unsigned long long)x * mg) >> 32); }
 
-// One Stockham pass of radix R over nfft length-N transforms, a -> b, Ns the
-// product of the radices already applied.  Butterfly j = jq*Ns + k (k < Ns,
-// j < N/R) reads a[j + q*N/R] (q < R), applies the twiddles
-// exp(SIGN 2 pi i q k / (Ns*R)), runs an R-point DFT and writes output p to
-// b[jq*Ns*R + p*Ns + k].  tw[t] = exp(-2 pi i t / N).
-template <typename T, int SIGN, int R>
-__device__ __forceinline__ void mr_pass(const T* a, T* b, int N, int Ns, int nfft, const T* __restrict__ tw) {
-    using F = typename Real<T>::type;
-    const int NR = N / R, step = N / (Ns * R);
-    const unsigned long long mNR = magic16(NR), mNs = magic16(Ns);
-    const int total = nfft * NR;
-    for (int bf = threadIdx.x; bf < total; bf += blockDim.x) {
-        const int f = div16(bf, mNR), j = bf - f * NR;
-        const int jq = div16(j, mNs), k = j - jq * Ns;
-        const T* in = a + f * N + j;
-        T v[R];
-        v[0] = in[0];
-#pragma unroll
-        for (int q = 1; q < R; ++q) {
-            T w = tw[q * k * step];  // q*k*step < N
-            if (SIGN > 0) w.y = -w.y;
-            v[q] = cmul(in[q * NR], w);
-        }
-        T y[R];
-        if constexpr (R == 2) {
-            y[0] = cadd(v[0], v[1]);
-            y[1] = csub(v[0], v[1]);
-        } else if constexpr (R == 4) {
-            const T s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-            const T s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-            // SIGN * i * d13
-            const T r13 = SIGN < 0 ? mk<T>(d13.y, -d13.x) : mk<T>(-d13.y, d13.x);
-            y[0] = cadd(s02, s13);
-            y[2] = csub(s02, s13);
-            y[1] = cadd(d02, r13);
-            y[3] = csub(d02, r13);
-        } else {  // odd R: conjugate pairs (q, R-q), w[t] = exp(SIGN 2 pi i t / R)
-            constexpr int H = (R - 1) / 2;
-            T w[R];
-#pragma unroll
-            for (int t = 0; t < R; ++t) {
-                w[t] = tw[t * NR];
-                if (SIGN > 0) w[t].y = -w[t].y;
-            }
-            T sp[H], dm[H];
-            y[0] = v[0];
-#pragma unroll
-            for (int q = 1; q <= H; ++q) {
-                sp[q - 1] = cadd(v[q], v[R - q]);
-                dm[q - 1] = csub(v[q], v[R - q]);
-                y[0] = cadd(y[0], sp[q - 1]);
-            }
-            // v_q w^t + v_{R-q} w^-t = sp Re(w^t) + i dm Im(w^t), t = pq mod R
-#pragma unroll
-            for (int p = 1; p < R; ++p) {
-                F sr = v[0].x, si = v[0].y;
-#pragma unroll
-                for (int q = 1; q <= H; ++q) {
-                    const T c = w[(p * q) % R];
-                    sr += sp[q - 1].x * c.x - dm[q - 1].y * c.y;
-                    si += sp[q - 1].y * c.x + dm[q - 1].x * c.y;
-                }
-                y[p] = mk<T>(sr, si);
-            }
-        }
-        T* out = b + f * N + jq * Ns * R + k;
-#pragma unroll
-        for (int p = 0; p < R; ++p) out[p * Ns] = y[p];
-    }
-}
-
-// The same pass for an odd prime radix R >= 5 known only at run time.
-// Work item (butterfly j, s), s <= H = (R-1)/2, writes output p = 0 (s = 0)
-// or the pair p = s, R-s, which share their inputs:
-//   y_s, y_{R-s} = v_0 + sum_{q=1..H} (v_q + v_{R-q}) Re(w^{sq}) +- i (v_q - v_{R-q}) Im(w^{sq}),
-// v_q = a[j + q*N/R] exp(SIGN 2 pi i q k / (Ns*R)), k = j mod Ns,
-// w = exp(SIGN 2 pi i / R).  For R >= 11 the
-// twiddles are applied first, in place, once per element (each pair item
-// would otherwise twiddle all R inputs: at R = 17 the sweep saves about 8% of
-// a B=255 FFT/DCT launch; at R = 5 its extra LDS pass and barrier cost 8-12%);
-// smaller R twiddle inside the items.  FP64: powers of wk and w by repeated
-// multiplication (relative error ~q ulp).  FP32: every power read from the
-// table (rounded once), since q ulp of FP32 would reach the 1e-6 the FP32
-// forms are held to.
-template <typename T, int SIGN>
-__device__ __forceinline__ void mr_pass_odd(T* a, T* b, int N, int Ns, int R, int nfft, const T* __restrict__ tw) {
-    using F = typename Real<T>::type;
-    const int NR = N / R, step = N / (Ns * R), H = (R - 1) >> 1, H1 = H + 1;
-    const unsigned long long mH1 = magic16(H1), mNR = magic16(NR), mNs = magic16(Ns), mN = magic16(N);
-    const bool sweep = R >= 11 && Ns > 1;
-    if (sweep) {
-        for (int idx = threadIdx.x; idx < nfft * N; idx += blockDim.x) {
-            const int f = div16(idx, mN), e = idx - f * N;
-            const int q = div16(e, mNR), j = e - q * NR;
-            const int k = j - div16(j, mNs) * Ns;
-            if (q > 0 && k > 0) {
-                T w = tw[q * k * step];  // q*k*step < R*Ns*step = N
-                if (SIGN > 0) w.y = -w.y;
-                a[idx] = cmul(a[idx], w);
-            }
-        }
-        __syncthreads();
-    }
-    const int total = nfft * NR * H1;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int bfi = div16(idx, mH1), sp_ = idx - bfi * H1;
-        const int f = div16(bfi, mNR), j = bfi - f * NR;
-        const int jq = div16(j, mNs), k = j - jq * Ns;
-        const T* in = a + f * N + j;
-        T acc1 = in[0], acc2 = in[0];
-        T wk = tw[k * step], WR = tw[k * step * R], om = tw[sp_ * NR];  // indices < N
-        if (SIGN > 0) {
-            wk.y = -wk.y;
-            WR.y = -WR.y;
-            om.y = -om.y;
-        }
-        T wq = wk, oq = om;  // oq = w^{sq}
-        int tq = k * step, ts = sp_;  // FP32: table indices of wk^q (mod N) and w^{sq} (mod R)
-        for (int q = 1; q <= H; ++q) {
-            T vq = in[q * NR], vr = in[(R - q) * NR];
-            if constexpr (sizeof(F) == 4) {
-                if (!sweep) {
-                    T a = tw[tq], b = tw[tq == 0 ? 0 : N - tq];  // wk^q and wk^{-q}; wk^{R-q} = wk^R wk^{-q}
-                    if (SIGN > 0) {
-                        a.y = -a.y;
-                        b.y = -b.y;
-                    }
-                    vq = cmul(vq, a);
-                    vr = cmul(vr, cmul(WR, b));
-                    tq += k * step;
-                    if (tq >= N) tq -= N;
-                }
-                oq = tw[ts * NR];
-                if (SIGN > 0) oq.y = -oq.y;
-                ts += sp_;
-                if (ts >= R) ts -= R;
-            } else if (!sweep) {  // wk^q and wk^{R-q} = wk^R conj(wk^q)
-                vq = cmul(vq, wq);
-                vr = cmul(vr, cmul(WR, cconj(wq)));
-                wq = cmul(wq, wk);
-            }
-            const T su = cadd(vq, vr), df = csub(vq, vr);
-            acc1.x += su.x * oq.x - df.y * oq.y;
-            acc1.y += su.y * oq.x + df.x * oq.y;
-            acc2.x += su.x * oq.x + df.y * oq.y;
-            acc2.y += su.y * oq.x - df.x * oq.y;
-            if constexpr (sizeof(F) != 4) oq = cmul(oq, om);
-        }
-        T* out = b + f * N + jq * Ns * R + k;
-        if (sp_ == 0) {
-            out[0] = acc1;
-        } else {
-            out[sp_ * Ns] = acc1;
-            out[(R - sp_) * Ns] = acc2;
-        }
-    }
-}
-
-// Threads of the mixed-radix instances: their passes are LDS-latency bound,
-// and two waves per SIMD (256 threads, 64 KiB of LDS per block) leave that
-// latency exposed.
+// Threads of the mixed-radix and Bluestein instances (two blocks of 8 waves
+// per CU keep their LDS passes' latency covered).
 #ifndef S2_MR_THREADS
 #define S2_MR_THREADS 512
 #endif
 constexpr int kMrThreads = S2_MR_THREADS;
-// The in-place instances (MODE >= 2) hold two 512-thread blocks per CU when
+// The in-place instances (MODE >= 1) hold two 512-thread blocks per CU when
 // they stay within 128 VGPRs (4 waves per SIMD).
-#define S2_FFT_OCCUPANCY(MODE) __attribute__((amdgpu_waves_per_eu((MODE) >= 2 ? 4 : 1)))
+#define S2_FFT_OCCUPANCY(MODE) __attribute__((amdgpu_waves_per_eu((MODE) >= 1 ? 4 : 1)))
 
 // ---------------------------------------------------------------------------
 // In-place mixed-radix FFTs with register butterflies (lengths whose prime
@@ -217,9 +58,8 @@ constexpr int kMrThreads = S2_MR_THREADS;
 // step: every thread loads the R inputs of each of its butterflies (up to
 // PER of them) into registers, the block synchronises, and each thread
 // twiddles, transforms and stores its butterflies over the same LDS buffer.
-// Against the two-buffer passes above this halves the LDS of a block (twice
-// the rings per block) and reads each input once per pass (the pair items of
-// mr_pass_odd read all R inputs once per output pair).
+// Against two-buffer Stockham passes this halves the LDS of a block (twice
+// the rings per block); each input is read once per pass.
 // ---------------------------------------------------------------------------
 constexpr int kRegMaxPrime = 19;
 // Bluestein's convolution lengths use radices up to 7 only: the radix-11..19
@@ -420,57 +260,34 @@ __device__ __forceinline__ void mr_reg_fft(T* a, int N, int nfft, const T* __res
 constexpr int kFillBatch = 4;
 
 
-// nfft independent length-N transforms stored back to back in a (b is
-// scratch of the same size).  SIGN = -1: forward, +1: inverse (unnormalised).
-// tw[t] = exp(-2 pi i t / N).  Returns the buffer holding the result.
-template <typename T, int SIGN, bool MR>
-__device__ T* block_fft(T* a, T* b, int N, int logN, int pow2, int nfft, const T* __restrict__ tw) {
-    if (!MR) {  // power-of-two N
-        const int half = N >> 1;
-        const int total = nfft * half;
-        for (int s = 0, Ns = 1; s < logN; ++s, Ns <<= 1) {
-            const int tstride = half >> s;  // N / (2 Ns)
-            for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-                const int f = idx >> (logN - 1);
-                const int j = idx & (half - 1);
-                const T* in = a + f * N;
-                T* out = b + f * N;
-                const int k = j & (Ns - 1);
-                T w = tw[k * tstride];
-                if (SIGN > 0) w.y = -w.y;
-                const T x0 = in[j];
-                const T x1 = cmul(in[j + half], w);
-                const int o = ((j - k) << 1) + k;
-                out[o] = cadd(x0, x1);
-                out[o + Ns] = csub(x0, x1);
-            }
-            __syncthreads();
-            T* t = a;
-            a = b;
-            b = t;
-        }
-        return a;
-    }
-    // Other lengths: mixed-radix Stockham (autosort), one pass per factor R
-    // of N (4 while 4 divides, then 2, then odd factors in ascending order).
-    // Radices 2, 3 and 4 run as butterflies; larger primes as conjugate output
-    // pairs (more butterfly instances in one kernel made hipcc spill).
-    int rem = N, Ns = 1;
-    while (rem > 1) {
-        int R = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : 3;
-        while (rem % R) R += 2;
-        switch (R) {
-            case 2: mr_pass<T, SIGN, 2>(a, b, N, Ns, nfft, tw); break;
-            case 3: mr_pass<T, SIGN, 3>(a, b, N, Ns, nfft, tw); break;
-            case 4: mr_pass<T, SIGN, 4>(a, b, N, Ns, nfft, tw); break;
-            default: mr_pass_odd<T, SIGN>(a, b, N, Ns, R, nfft, tw); break;
+// nfft independent power-of-two transforms of length N stored back to back
+// in a, radix-2 Stockham passes between a and b (scratch of the same size).
+// SIGN = -1: forward, +1: inverse (unnormalised).  tw[t] = exp(-2 pi i t / N).
+// Returns the buffer holding the result.
+template <typename T, int SIGN>
+__device__ T* radix2_fft(T* a, T* b, int N, int logN, int nfft, const T* __restrict__ tw) {
+    const int half = N >> 1;
+    const int total = nfft * half;
+    for (int s = 0, Ns = 1; s < logN; ++s, Ns <<= 1) {
+        const int tstride = half >> s;  // N / (2 Ns)
+        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+            const int f = idx >> (logN - 1);
+            const int j = idx & (half - 1);
+            const T* in = a + f * N;
+            T* out = b + f * N;
+            const int k = j & (Ns - 1);
+            T w = tw[k * tstride];
+            if (SIGN > 0) w.y = -w.y;
+            const T x0 = in[j];
+            const T x1 = cmul(in[j + half], w);
+            const int o = ((j - k) << 1) + k;
+            out[o] = cadd(x0, x1);
+            out[o + Ns] = csub(x0, x1);
         }
         __syncthreads();
         T* t = a;
         a = b;
         b = t;
-        rem /= R;
-        Ns *= R;
     }
     return a;
 }
@@ -548,51 +365,50 @@ __device__ __forceinline__ void bluestein_rows(T* a, int N, int M, int nfft, con
 // Bluestein instance (whose logN argument holds M).
 template <int MODE>
 __device__ __forceinline__ int row_stride(int N, int logN) {
-    return MODE >= 3 ? blu_stride(logN) : N;
+    return MODE >= 2 ? blu_stride(logN) : N;
 }
 // Rows of the block's LDS buffer: rpb, or blu_rows(M) for Bluestein (which
 // transforms all of them).
 template <typename T, int MODE>
 __device__ __forceinline__ int lds_rows(int rpb, int logN) {
-    return MODE >= 3 ? blu_rows<T>(logN) : rpb;
+    return MODE >= 2 ? blu_rows<T>(logN) : rpb;
 }
 // LDS position of the flat index i = row * N + column.
 template <int MODE>
 __device__ __forceinline__ int lds_row(int i, int N, int RS, unsigned long long mN) {
-    if constexpr (MODE >= 3) {
+    if constexpr (MODE >= 2) {
         const int r = div16(i, mN);
         return r * RS + (i - r * N);
     } else {
         return i;
     }
 }
-// Stage the FFT twiddles in LDS at b (MODE 1: past the second buffer; the
-// others transform in place): N values, M for Bluestein.
+// Stage the FFT twiddles in LDS at b, past the rows the in-place instances
+// (MODE >= 1) transform: N values, M for Bluestein.
 template <typename T, int MODE>
-__device__ __forceinline__ const T* stage_fft_twiddles(T* b, int rpb, int N, int logN, const T* __restrict__ tw) {
-    T* twl = MODE == 1 ? b + rpb * N : b;
-    const int n = MODE >= 3 ? logN : N;
+__device__ __forceinline__ const T* stage_fft_twiddles(T* b, int N, int logN, const T* __restrict__ tw) {
+    T* twl = b;
+    const int n = MODE >= 2 ? logN : N;
     for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = tw[i];
     return twl;
 }
 
-// The FFT stage of the per-field kernels: MODE 0 radix-2 passes
-// (power-of-two N), 1 the two-buffer mixed-radix passes, 2 the in-place
-// register passes (mr_reg_fft), 3 and 4 Bluestein with M a power of two and
-// M of small primes (bluestein_rows; logN holds M, tw the LDS copy of the M
-// twiddles, gtab the global tables).
-// Returns the buffer holding the result.
+// The FFT stage of the per-field kernels: MODE 0 radix-2 passes between two
+// buffers (power-of-two N), 1 the in-place register mixed-radix passes
+// (mr_reg_fft), 2 and 3 Bluestein with M a power of two and M of small
+// primes (bluestein_rows; logN holds M, tw the LDS copy of the M twiddles,
+// gtab the global tables).  Returns the buffer holding the result.
 template <typename T, int SIGN, int MODE>
 __device__ __forceinline__ T* fft_rows(T* a, T* b, int N, int logN, int pow2, int nfft, const T* __restrict__ tw,
                                        const T* __restrict__ gtab) {
-    if constexpr (MODE >= 3) {
-        bluestein_rows<T, SIGN, MODE == 3>(a, N, logN, nfft, tw, gtab);
+    if constexpr (MODE >= 2) {
+        bluestein_rows<T, SIGN, MODE == 2>(a, N, logN, nfft, tw, gtab);
         return a;
-    } else if constexpr (MODE == 2) {
+    } else if constexpr (MODE == 1) {
         mr_reg_fft<T, SIGN>(a, N, nfft, tw);
         return a;
     } else {
-        return block_fft<T, SIGN, MODE == 1>(a, b, N, logN, pow2, nfft, tw);
+        return radix2_fft<T, SIGN>(a, b, N, logN, nfft, tw);
     }
 }
 
@@ -640,7 +456,7 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) voi
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, N, logN, tw);
     __syncthreads();
     T* r = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const unsigned long long mnr = magic16(nr);
@@ -694,7 +510,7 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) voi
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, N, logN, tw);
     __syncthreads();
     T* V = fft_rows<T, -1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const unsigned long long mB = magic16(B);
@@ -1265,7 +1081,7 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) voi
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, N, logN, tw);
     __syncthreads();
     T* u = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const R inv_s2pi = (R)0.39894228040143267794;  // 1/sqrt(2 pi)
@@ -1360,7 +1176,7 @@ __global__ __launch_bounds__(MODE ? kMrThreads : 256) S2_FFT_OCCUPANCY(MODE) voi
         }
     }
     const T* twp = tw;
-    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, rpb, N, logN, tw);
+    if constexpr (MODE != 0) twp = stage_fft_twiddles<T, MODE>(b, N, logN, tw);
     __syncthreads();
     T* y = fft_rows<T, +1, MODE>(a, b, N, logN, pow2, nr, twp, tw);
     const size_t base = ((size_t)f * N + j0) * N;
@@ -1644,20 +1460,14 @@ static int ilog2_exact(int n) {
     return ((1 << l) == n) ? l : -1;
 }
 
-// The per-field FFT/DCT kernels come in three instances (MODE of
-// fft_rows): 0 for power-of-two N (radix-2 passes, 256 threads, 2048 values
-// per block in two buffers), 2 for N whose prime factors are at most
-// kRegMaxPrime (in-place register passes, 512 threads, rm_elems<T>() values
-// per block), 1 for the other N (two-buffer passes, 512 threads, S2_MR_ELEMS
-// FP64 / S2_MR_ELEMS32 FP32 values per block).  The mixed-radix instances
-// also hold the N twiddles in LDS.  Separate instances keep the mixed-radix
-// passes' registers out of the power-of-two kernels.
-#ifndef S2_MR_ELEMS
-#define S2_MR_ELEMS 2048
-#endif
-#ifndef S2_MR_ELEMS32
-#define S2_MR_ELEMS32 4096
-#endif
+// The per-field FFT/DCT kernels come in four instances (MODE of fft_rows):
+// 0 for power-of-two N (radix-2 passes, 256 threads, 2048 values per block
+// in two buffers), 1 for N whose prime factors are at most kRegMaxPrime
+// (in-place register passes, 512 threads, rm_elems<T>() values per block), 2
+// and 3 for the other N (Bluestein with a power-of-two or a 7-smooth
+// convolution length M, 512 threads, blu_rows(M) rows per block).  The
+// in-place instances also hold their twiddles in LDS.  Separate instances
+// keep each path's registers out of the others.
 template <typename T>
 struct FftCfg {
     int mode;    // instance
@@ -1689,29 +1499,23 @@ static FftCfg<T> fft_cfg(const DevConsts& c) {
     }
     if (rm_supported(N)) {
         const int rpb = std::max(1, std::min(rm_elems<T>() / N, (kLdsValues - N) / N));
-        return {2, rpb, c.logN, sizeof(T) * ((size_t)rpb * N + N), tw};
+        return {1, rpb, c.logN, sizeof(T) * ((size_t)rpb * N + N), tw};
     }
-    if (c.blu_M > 0) {
-        const int M = c.blu_M, rpb = blu_rows<T>(M);
-        const T* g = reinterpret_cast<const T*>(f64 ? (const void*)c.blu : (const void*)c.blu32);
-        return {is_pow2(M) ? 3 : 4, rpb, M, sizeof(T) * ((size_t)rpb * blu_stride(M) + M), g};
-    }
-    const int elems = f64 ? S2_MR_ELEMS : S2_MR_ELEMS32;
-    const int rpb = std::max(1, std::min(elems / N, (kLdsValues - N) / (2 * N)));
-    return {1, rpb, c.logN, sizeof(T) * (2 * (size_t)rpb * N + N), tw};
+    if (c.blu_M == 0) throw ArgError{"no FFT tables for this ring length"};  // consts_create builds them
+    const int M = c.blu_M, rpb = blu_rows<T>(M);
+    const T* g = reinterpret_cast<const T*>(f64 ? (const void*)c.blu : (const void*)c.blu32);
+    return {is_pow2(M) ? 2 : 3, rpb, M, sizeof(T) * ((size_t)rpb * blu_stride(M) + M), g};
 }
 
-// FFT_INSTANCES(k, T): the five instances of a per-field FFT/DCT kernel.
-#define FFT_INSTANCES(k, T) k<T, 0>, k<T, 1>, k<T, 2>, k<T, 3>, k<T, 4>
-template <typename T, typename K0, typename K1, typename K2, typename K3, typename K4, typename... A>
-static void fft_launch(const FftCfg<T>& g, K0 k0, K1 k1, K2 k2, K3 k3, K4 k4, dim3 grid, hipStream_t s,
-                       A... args) {
+// FFT_INSTANCES(k, T): the four instances of a per-field FFT/DCT kernel.
+#define FFT_INSTANCES(k, T) k<T, 0>, k<T, 1>, k<T, 2>, k<T, 3>
+template <typename T, typename K0, typename K1, typename K2, typename K3, typename... A>
+static void fft_launch(const FftCfg<T>& g, K0 k0, K1 k1, K2 k2, K3 k3, dim3 grid, hipStream_t s, A... args) {
     switch (g.mode) {
         case 0: launch_lds(k0, grid, 256, g.lds, s, args...); break;
         case 1: launch_lds(k1, grid, kMrThreads, g.lds, s, args...); break;
         case 2: launch_lds(k2, grid, kMrThreads, g.lds, s, args...); break;
-        case 3: launch_lds(k3, grid, kMrThreads, g.lds, s, args...); break;
-        default: launch_lds(k4, grid, kMrThreads, g.lds, s, args...); break;
+        default: launch_lds(k3, grid, kMrThreads, g.lds, s, args...); break;
     }
 }
 
@@ -2014,12 +1818,9 @@ static int reg_rpb(int N) { return 8 * kThreads / N; }
 static size_t reg_lds(int N) { return sizeof(double2) * ((size_t)reg_rpb(N) * (N + 1) + N); }
 
 // FP32 ring FFTs of grids whose ring length is not a power of two (the mixed
-// form's first and last stages at such bandwidths), on the mixed-radix
-// instances; same order-major FP32 intermediate X as the register kernels.
-// Measured at bw = 255 x 1024 fields on the two-buffer instance: 2048 values
-// per block 2.92 / 2.95 ms per forward / inverse FFT launch, 4096 values
-// 2.67 / 2.85 ms, 8192 values 3.89 / 4.21 ms (one block per CU);
-// 1024-thread blocks slower at both sizes.
+// form's first and last stages at such bandwidths), on the register
+// mixed-radix or Bluestein instances; same order-major FP32 intermediate X
+// as the power-of-two register kernels.
 void ring_fft_fwd_f32(const DevConsts& c, const double* re, const double* im, float2* X, int F, hipStream_t s) {
     const int N = c.N;
     const FftCfg<float2> fg = fft_cfg<float2>(c);
