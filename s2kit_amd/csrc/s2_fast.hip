@@ -208,8 +208,15 @@ __device__ __forceinline__ void mma_run(typename Mma<R>::acc (&acc)[CT], const R
     }
 }
 
-// Orders m of a GEMM launch: grid (column blocks, roundup8(bw)); blocks past
-// the last order return at once.
+// Output tiles of a block: its waves take every (kThreads/64 * gridDim.z)-th
+// tile, slice blockIdx.z of the tiles when grid.z splits an order's column
+// block over several blocks (calls of few columns, whose blocks are each one
+// order's table stream).  A tile's sums do not depend on the split.
+__device__ __forceinline__ int item0(int wave) { return wave + (kThreads / 64) * blockIdx.z; }
+__device__ __forceinline__ int item_step() { return (kThreads / 64) * gridDim.z; }
+
+// Orders m of a GEMM launch: grid (column blocks, roundup8(bw), tile slices);
+// blocks past the last order return at once.
 __device__ __forceinline__ bool gemm_block(int B, int& bx, int& m) {
     xcd_block_m(bx, m);
     return m < B;
@@ -277,7 +284,7 @@ __device__ __forceinline__ void gemm_fwd_semi(R* __restrict__ Bd, const R* __res
         if (g.Lp == 0) continue;
         const int Li16 = (g.Lp + 15) & ~15;
         const int ntiles = (Li16 >> 4) * nctg;
-        for (int item = wave; item < ntiles; item += kThreads / 64) {
+        for (int item = item0(wave); item < ntiles; item += item_step()) {
             const int it = item / nctg, ct = item - it * nctg;
             const int i0 = it * 16, c0 = ct * 16 * CT;
             const int ilast = min(i0 + 15, g.Lp - 1);
@@ -338,7 +345,7 @@ __device__ __forceinline__ void gemm_fwd_naive(R* __restrict__ Bd, const typenam
         __syncthreads();
         const int Li16 = (g.Lp + 15) & ~15;
         const int ntiles = (Li16 >> 4) * nctg;
-        for (int item = wave; item < ntiles; item += kThreads / 64) {
+        for (int item = item0(wave); item < ntiles; item += item_step()) {
             const int it = item / nctg, ct = item - it * nctg;
             const int i0 = it * 16, c0 = ct * 16 * CT;
             typename Mma<R>::acc acc[CT] = {};
@@ -437,7 +444,7 @@ __device__ __forceinline__ void gemm_inv_semi(R* __restrict__ Cd, const double* 
         const int Li4 = p ? Li4_1 : Li4_0;
         const int cnt = p ? G.hO : G.hE;
         const int r0 = g.Lp > 0 ? row_size(m, m + p) : 0;
-        for (int item = wave; item < ntt * nctg; item += kThreads / 64) {
+        for (int item = item0(wave); item < ntt * nctg; item += item_step()) {
             const int tt = item / nctg, ct = item - tt * nctg;
             const int t0 = tt * 16, c0 = ct * 16 * CT;
             const int istart = max(0, t0 - r0 + 1) & ~3;
@@ -483,7 +490,7 @@ __device__ __forceinline__ void gemm_inv_naive(R* __restrict__ Cd, const double*
     const int nctg = (ncb >> 4) / CT;
     const int ntt = B16 >> 4;
     const bool mirror = fmt == 1 && m > 0;
-    for (int item = wave; item < ntt * nctg; item += kThreads / 64) {
+    for (int item = item0(wave); item < ntt * nctg; item += item_step()) {
         const int tt = item / nctg, ct = item - tt * nctg;
         const int t0 = tt * 16, c0 = ct * 16 * CT;
         typename Mma<R>::acc e[CT] = {}, o[CT] = {};
@@ -631,12 +638,17 @@ bool fast_supported(int bw, int cutoff, int prec) {
     return gemm_ncb(bw, prec) >= 16;  // GEMM LDS tile of at least 16 columns
 }
 
+// Tile slices per order and column block: enough blocks for the chip
+// (1024) when the call has few columns, at most 8.
+static int gemm_slices(int nblocks) { return std::max(1, std::min(8, 1024 / std::max(1, nblocks))); }
+
 template <typename R, bool GEN, int CT>
 static void fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre,
                        double* cim, int F, Fmt fmt, int ncb, hipStream_t s) {
     using T = typename Cplx<R>::type;
     const int B = c.bw, cols = 4 * F;
-    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
+    const int nx = (cols + ncb - 1) / ncb, ny = (B + 7) & ~7;
+    const dim3 grid(nx, ny, gemm_slices(nx * ny));
     launch_lds(k_gemm_fwd<R, GEN, CT>, grid, kThreads, (size_t)ncb * gemm_row(B, g.prec), s, (const R*)Cb, (const T*)X,
                (const double*)c.w_naive, (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
                (int)fmt);
@@ -647,7 +659,8 @@ static void inv_launch(const DevConsts& c, const GemmTables& g, const double* cr
                        void* X, int F, Fmt fmt, int ncb, hipStream_t s) {
     using T = typename Cplx<R>::type;
     const int B = c.bw, cols = 4 * F;
-    const dim3 grid((cols + ncb - 1) / ncb, (B + 7) & ~7);
+    const int nx = (cols + ncb - 1) / ncb, ny = (B + 7) & ~7;
+    const dim3 grid(nx, ny, gemm_slices(nx * ny));
     launch_lds(k_gemm_inv<R, GEN, CT>, grid, kThreads, (size_t)ncb * gemm_row(B, g.prec), s, cre, cim, (const R*)g.ai,
                (const GemmMeta*)g.meta, (R*)Cb, (T*)X, B, g.cut, F, ncb, (int)fmt);
 }
@@ -695,6 +708,22 @@ void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre,
         (two ? inv_launch<double, false, 2> : inv_launch<double, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
 }
 
+// Calls of few fields: the default ring-FFT/DCT blocks (512 threads, 8 FP64 or
+// 16 FP32 rings) would give fewer than 256 blocks, one per CU (one field at
+// bw=256: 64), so one-wave blocks (64 threads; 128 for the longest rings)
+// take one or two rings each: eight times the blocks, each block's passes
+// synchronised within one or two waves.  Measured per fwd+inv pair of one
+// field: bw=64 46.9 -> 31.7 us, 128 51.7 -> 37.8, 256 72.1 -> 64.2; at four
+// fields (256 default blocks) the one-wave blocks were slower (92.9 -> 100.3
+// us at bw=256).  Returns the block's threads, 0 for the default blocks.
+template <int V>
+static int few_field_threads(int N, int F) {
+    if ((long)F * N * N >= 512L * V * 256) return 0;
+    if (N <= 64 * V) return 64;
+    if (N <= 128 * V) return 128;
+    return 0;
+}
+
 template <typename R>
 static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
                      double* cim, int F, Fmt fmt, void* Xv, void* Cbv, hipStream_t s, StageTimer* tm) {
@@ -708,10 +737,17 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
     const T* tw = (const T*)(sizeof(R) == 8 ? (const void*)c.tw : (const void*)c.tw32);
     const T* ph = (const T*)(sizeof(R) == 8 ? (const void*)c.ph : (const void*)c.ph32);
     const R* ws = (const R*)(sizeof(R) == 8 ? (const void*)c.w_semi : (const void*)c.w_semi32);
+    const int few = few_field_threads<V>(N, F);
+    const int rpf = few ? V * few / N : 0;  // rings per one-wave block
+    const size_t ldf = few ? sizeof(T) * ((size_t)rpf * (N + 1) + N) : 0;
     {
         Span sp(tm, ST_PHI_FWD, s);
         // 512-thread blocks (twice the rings, full 128-byte row segments) when a block still holds whole rows
-        if ((long)N * N >= 512L * V) {
+        if (few == 64) {
+            launch_lds(k_fphi_fwd<R, V, 64>, dim3(N / rpf, F), 64, ldf, s, re, im, X, lgN, tw);
+        } else if (few == 128) {
+            launch_lds(k_fphi_fwd<R, V, 128>, dim3(N / rpf, F), 128, ldf, s, re, im, X, lgN, tw);
+        } else if ((long)N * N >= 512L * V) {
             const int rp2 = V * 512 / N;
             const size_t l2 = sizeof(T) * ((size_t)rp2 * (N + 1) + N);
             if (k_fphi_fwd_nt(rp2, sizeof(T)))
@@ -726,7 +762,13 @@ static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, 
     const int cut = g.cut, nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
     if (cut > 0) {
         Span sp(tm, ST_DCT_FWD, s);
-        if ((long)N * N >= 512L * V) {
+        if (few == 64) {
+            launch_lds(k_fdct_fwd<R, V, 64>, dim3((nrows + rpf - 1) / rpf, F), 64, ldf, s, (const T*)X, Cb, ws, lgN,
+                       cut, nrows, tw, ph);
+        } else if (few == 128) {
+            launch_lds(k_fdct_fwd<R, V, 128>, dim3((nrows + rpf - 1) / rpf, F), 128, ldf, s, (const T*)X, Cb, ws, lgN,
+                       cut, nrows, tw, ph);
+        } else if ((long)N * N >= 512L * V) {
             const int rp2 = V * 512 / N;
             launch_lds(k_fdct_fwd<R, V, 512>, dim3((nrows + rp2 - 1) / rp2, F), 512,
                        sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const T*)X, Cb, ws, lgN, cut, nrows, tw, ph);
@@ -754,6 +796,9 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     const T* tw = (const T*)(sizeof(R) == 8 ? (const void*)c.tw : (const void*)c.tw32);
     const T* ph = (const T*)(sizeof(R) == 8 ? (const void*)c.ph : (const void*)c.ph32);
     const R* sv = (const R*)(sizeof(R) == 8 ? (const void*)c.sinv : (const void*)c.sinv32);
+    const int few = few_field_threads<V>(N, F);
+    const int rpf = few ? V * few / N : 0;  // rings per one-wave block
+    const size_t ldf = few ? sizeof(T) * ((size_t)rpf * (N + 1) + N) : 0;
     {
         Span sp(tm, ST_LEG_INV, s);
         gemm_inv_launch(c, g, cre, cim, Cb, X, F, fmt, s);
@@ -762,7 +807,13 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     const int cut = g.cut, nrows = (fmt == Fmt::Complex) ? 2 * cut - 1 : cut;
     if (cut > 0) {
         Span sp(tm, ST_DCT_INV, s);
-        if ((long)N * N >= 512L * V) {
+        if (few == 64) {
+            launch_lds(k_fdct_inv<R, V, 64>, dim3((nrows + rpf - 1) / rpf, F), 64, ldf, s, (const R*)Cb, X, sv, lgN,
+                       cut, nrows, (int)fmt, tw, ph);
+        } else if (few == 128) {
+            launch_lds(k_fdct_inv<R, V, 128>, dim3((nrows + rpf - 1) / rpf, F), 128, ldf, s, (const R*)Cb, X, sv, lgN,
+                       cut, nrows, (int)fmt, tw, ph);
+        } else if ((long)N * N >= 512L * V) {
             const int rp2 = V * 512 / N;
             launch_lds(k_fdct_inv<R, V, 512>, dim3((nrows + rp2 - 1) / rp2, F), 512,
                        sizeof(T) * ((size_t)rp2 * (N + 1) + N), s, (const R*)Cb, X, sv, lgN, cut, nrows, (int)fmt, tw,
@@ -774,7 +825,11 @@ static void inv_impl(const DevConsts& c, const GemmTables& g, const double* cre,
     }
     {
         Span sp(tm, ST_PHI_INV, s);
-        if ((long)N * N >= 512L * V) {
+        if (few == 64) {
+            launch_lds(k_fphi_inv<R, V, 64>, dim3(N / rpf, F), 64, ldf, s, (const T*)X, re, im, lgN, tw);
+        } else if (few == 128) {
+            launch_lds(k_fphi_inv<R, V, 128>, dim3(N / rpf, F), 128, ldf, s, (const T*)X, re, im, lgN, tw);
+        } else if ((long)N * N >= 512L * V) {
             const int rp2 = V * 512 / N;
             launch_lds(k_fphi_inv<R, V, 512>, dim3(N / rp2, F), 512, sizeof(T) * ((size_t)rp2 * (N + 1) + N), s,
                        (const T*)X, re, im, lgN, tw);
