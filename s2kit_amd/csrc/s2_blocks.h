@@ -85,10 +85,25 @@ __device__ __forceinline__ void dftR(T* x) {
         dft2<SIGN>(x);
 }
 
+// Twiddle exp(-2 pi i t / N) from tw: tw[t], or with TW2 a two-level table
+// of 2^(lgN - L) + 2^L values, hi[u] = exp(-2 pi i u 2^L / N) then
+// lo[v] = exp(-2 pi i v / N), L = tw2_bits(lgN): hi[t >> L] * lo[t mod 2^L]
+// (one rounding more, about 2 sqrt(N) values of LDS instead of N).
+__host__ __device__ constexpr int tw2_bits(int lgN) { return (lgN + 1) / 2; }
+template <bool TW2, typename T>
+__device__ __forceinline__ T twiddle(const T* tw, int t, int lgN) {
+    if constexpr (TW2) {
+        const int L = tw2_bits(lgN);
+        return cmul(tw[t >> L], tw[(1 << (lgN - L)) + (t & ((1 << L) - 1))]);
+    } else {
+        return tw[t];
+    }
+}
+
 // One Stockham pass of radix R over the E = V*NTH values of the block of NTH
 // threads (E/N transforms of length N, row stride S).  Sub-transform size
 // Ns = 2^lgNs.
-template <typename T, int SIGN, int V, int R, int LGR, int NTH>
+template <typename T, int SIGN, int V, int R, int LGR, int NTH, bool TW2 = false>
 __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T* tw) {
     constexpr int PER = V / R;
     const int lgNR = lgN - LGR;
@@ -113,7 +128,7 @@ __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T
         if (lgNs > 0) {  // the first pass has k = 0: unit twiddles
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                T w = tw[r * step];
+                T w = twiddle<TW2>(tw, r * step, lgN);
                 if (SIGN > 0) w.y = -w.y;
                 x[u * R + r] = cmul(x[u * R + r], w);
             }
@@ -127,18 +142,18 @@ __device__ __forceinline__ void fft_pass(T* a, int lgN, int S, int lgNs, const T
 }
 
 // In-place FFTs of all E/N rows of the block's LDS tile (natural order in and out).
-template <typename T, int SIGN, int V, int NTH = kThreads>
+template <typename T, int SIGN, int V, int NTH = kThreads, bool TW2 = false>
 __device__ __forceinline__ void fft_block(T* a, int lgN, int S, const T* tw) {
     int lgNs = 0, rem = lgN;
     while (rem >= 3) {
-        fft_pass<T, SIGN, V, 8, 3, NTH>(a, lgN, S, lgNs, tw);
+        fft_pass<T, SIGN, V, 8, 3, NTH, TW2>(a, lgN, S, lgNs, tw);
         lgNs += 3;
         rem -= 3;
     }
     if (rem == 2)
-        fft_pass<T, SIGN, V, 4, 2, NTH>(a, lgN, S, lgNs, tw);
+        fft_pass<T, SIGN, V, 4, 2, NTH, TW2>(a, lgN, S, lgNs, tw);
     else if (rem == 1)
-        fft_pass<T, SIGN, V, 2, 1, NTH>(a, lgN, S, lgNs, tw);
+        fft_pass<T, SIGN, V, 2, 1, NTH, TW2>(a, lgN, S, lgNs, tw);
 }
 
 __device__ __forceinline__ int frow_of(int q, int N, int cut) { return q < cut ? q : N - 2 * cut + 1 + q; }

@@ -319,6 +319,19 @@ __host__ __device__ constexpr int blu_rows(int M) {
                       : imax(1, imin(rm_elems<T>() / M, (81920 / (int)sizeof(T) - M) / M));
 }
 
+// A power-of-two M keeps its M twiddles in LDS while the block stays within
+// 80 KiB (two blocks per CU); beyond that it stages the two-level table of
+// twiddle() instead (bw=509 x 64 fields FP64: 2.33 -> 1.53 ms per FFT/DCT
+// launch; where the full table fits, the extra complex product per twiddle
+// cost up to 5%).
+template <typename T>
+__host__ __device__ constexpr bool blu_tw2(int M) {
+    return is_pow2(M) && (size_t)sizeof(T) * ((size_t)blu_rows<T>(M) * blu_stride(M) + M) > 81920;
+}
+__host__ __device__ constexpr int tw2_values(int M) {
+    return (M >> tw2_bits(__builtin_ctz(M))) + (1 << tw2_bits(__builtin_ctz(M)));
+}
+
 template <typename T, int SIGN, bool P2>
 __device__ __forceinline__ void bluestein_rows(T* a, int N, int M, int nfft, const T* twl, const T* __restrict__ g) {
     constexpr int V = rm_elems<T>() / kMrThreads;
@@ -340,8 +353,12 @@ __device__ __forceinline__ void bluestein_rows(T* a, int N, int M, int nfft, con
     __syncthreads();
     // Both FFTs forward (one instance of the passes per kernel): the inverse
     // FFT of z is conj(FFT(conj z)), the conjugations folded into the products.
-    if constexpr (p2) fft_block<T, -1, V, kMrThreads>(a, __builtin_ctz(M), S, twl);
-    else mr_reg_fft<T, -1, kBluMaxPrime>(a, M, rows, twl);
+    if constexpr (p2) {
+        if (blu_tw2<T>(M)) fft_block<T, -1, V, kMrThreads, true>(a, __builtin_ctz(M), S, twl);
+        else fft_block<T, -1, V, kMrThreads, false>(a, __builtin_ctz(M), S, twl);
+    } else {
+        mr_reg_fft<T, -1, kBluMaxPrime>(a, M, rows, twl);
+    }
     for (int i = threadIdx.x; i < tot; i += kMrThreads) {
         const int r = div16(i, mM), k = i - r * M;
         T c = bh[SIGN < 0 ? k : (k == 0 ? 0 : M - k)];
@@ -349,8 +366,12 @@ __device__ __forceinline__ void bluestein_rows(T* a, int N, int M, int nfft, con
         a[r * S + k] = cconj(cmul(a[r * S + k], c));
     }
     __syncthreads();
-    if constexpr (p2) fft_block<T, -1, V, kMrThreads>(a, __builtin_ctz(M), S, twl);
-    else mr_reg_fft<T, -1, kBluMaxPrime>(a, M, rows, twl);
+    if constexpr (p2) {
+        if (blu_tw2<T>(M)) fft_block<T, -1, V, kMrThreads, true>(a, __builtin_ctz(M), S, twl);
+        else fft_block<T, -1, V, kMrThreads, false>(a, __builtin_ctz(M), S, twl);
+    } else {
+        mr_reg_fft<T, -1, kBluMaxPrime>(a, M, rows, twl);
+    }
     const unsigned long long mN = magic16(N);
     for (int i = threadIdx.x; i < nfft * N; i += kMrThreads) {
         const int r = div16(i, mN), k = i - r * N;
@@ -384,12 +405,18 @@ __device__ __forceinline__ int lds_row(int i, int N, int RS, unsigned long long 
     }
 }
 // Stage the FFT twiddles in LDS at b, past the rows the in-place instances
-// (MODE >= 1) transform: N values, M for Bluestein.
+// (MODE >= 1) transform: N values, M for Bluestein, or the two-level table of
+// twiddle() (hi then lo) where blu_tw2.
 template <typename T, int MODE>
 __device__ __forceinline__ const T* stage_fft_twiddles(T* b, int N, int logN, const T* __restrict__ tw) {
     T* twl = b;
-    const int n = MODE >= 2 ? logN : N;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = tw[i];
+    if (MODE == 2 && blu_tw2<T>(logN)) {
+        const int M = logN, lgM = __builtin_ctz(M), L = tw2_bits(lgM), nhi = M >> L;
+        for (int i = threadIdx.x; i < nhi + (1 << L); i += blockDim.x) twl[i] = i < nhi ? tw[i << L] : tw[i - nhi];
+    } else {
+        const int n = MODE >= 2 ? logN : N;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = tw[i];
+    }
     return twl;
 }
 
@@ -1504,7 +1531,8 @@ static FftCfg<T> fft_cfg(const DevConsts& c) {
     if (c.blu_M == 0) throw ArgError{"no FFT tables for this ring length"};  // consts_create builds them
     const int M = c.blu_M, rpb = blu_rows<T>(M);
     const T* g = reinterpret_cast<const T*>(f64 ? (const void*)c.blu : (const void*)c.blu32);
-    return {is_pow2(M) ? 2 : 3, rpb, M, sizeof(T) * ((size_t)rpb * blu_stride(M) + M), g};
+    const int ntw = blu_tw2<T>(M) ? tw2_values(M) : M;  // LDS twiddles
+    return {is_pow2(M) ? 2 : 3, rpb, M, sizeof(T) * ((size_t)rpb * blu_stride(M) + ntw), g};
 }
 
 // FFT_INSTANCES(k, T): the four instances of a per-field FFT/DCT kernel.
