@@ -424,3 +424,31 @@ def test_autotune_keeps_results(env, bw, prec):
     for a, b in zip(before, after):
         assert torch.equal(a, b)
     plan.close()
+
+
+@pytest.mark.parametrize("bw,prec", [(256, 0), (128, 0), (64, 0), (256, 1), (100, 0)])
+def test_few_field_calls_match_batched(env, bw, prec):
+    # Calls of few fields run the ring FFTs and DCTs on one-wave blocks and
+    # slice each order's GEMM tiles over several blocks; the butterflies and
+    # every tile's sums are the same, so one field per call reproduces the
+    # batched call bit for bit (the batch of 8 at bw >= 256 runs the default
+    # 512-thread blocks and unsliced GEMMs).
+    S, torch, dev = env
+    from s2kit_amd import synth
+    F, n = 8, 2 * bw
+    cre, cim = synth.power_law_coeffs(bw, F, 29, dev)
+    outs = []
+    for per_call in (F, 1):
+        plan = S.Plan(bw, per_call, precision=prec)
+        gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+        gim = torch.empty_like(gre)
+        c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+        for f0 in range(0, F, per_call):
+            sl = slice(f0, f0 + per_call)
+            plan.inverse(cre[sl], cim[sl], gre[sl], gim[sl])
+            plan.forward(gre[sl], gim[sl], c2r[sl], c2i[sl])
+        torch.cuda.synchronize()
+        plan.close()
+        outs.append((gre, gim, c2r, c2i))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
