@@ -106,9 +106,12 @@ def test_plan_tables_match_oracle(env, O, bw, cutoff):
     plan.close()
 
 
-def test_plan_conv_and_zonal(env, O):
+@pytest.mark.parametrize("bw", [64, 45, 23, 59])
+def test_plan_conv_and_zonal(env, O, bw):
+    # 45: 90 = 2*3^2*5 rings (register mixed-radix FFTs); 23 and 59: Bluestein
+    # (46 rings, convolution length 96 = 2^5*3; 118 rings, length 256)
     S, torch, dev = env
-    bw, F = 64, 2
+    F = 2
     plan = S.Plan(bw, F)
     rng = np.random.default_rng(31)
     n = 2 * bw
