@@ -144,9 +144,6 @@ __host__ __device__ inline GGeo ggeo(int B, bool gen) {
 #ifndef S2_GEMM_RUN
 #define S2_GEMM_RUN 8
 #endif
-#ifndef S2_GEMM_PIPE
-#define S2_GEMM_PIPE 1
-#endif
 // S K steps of MFMAs for CT column tiles on table values g already loaded.
 template <typename R, int CT, int S>
 __device__ __forceinline__ void mma_batch(typename Mma<R>::acc (&acc)[CT], const R* lds, int cs, const R (&g)[S],
@@ -181,8 +178,9 @@ __device__ __forceinline__ void mma_run(typename Mma<R>::acc (&acc)[CT], const R
                                         const R* __restrict__ glb, size_t stride, int k0, int k1) {
     const int kq = (threadIdx.x & 63) >> 4;
     int k = k0;
-#if S2_GEMM_PIPE
     // two table buffers: batch k + 4S is in flight while batch k's MFMAs run
+    // (one buffer, each batch waited before its MFMAs: FP32 bw=256 forward
+    // GEMM 1.84 ms against 1.49)
     R ga[S], gb[S];
     if (k + 4 * S <= k1) table_batch<R, S>(ga, glb, stride, k, kq);
     while (k + 4 * S <= k1) {
@@ -194,13 +192,6 @@ __device__ __forceinline__ void mma_run(typename Mma<R>::acc (&acc)[CT], const R
         mma_batch<R, CT, S>(acc, lds, cs, gb, k, kq);
         k += 4 * S;
     }
-#else
-    for (; k + 4 * S <= k1; k += 4 * S) {
-        R g[S];
-        table_batch<R, S>(g, glb, stride, k, kq);
-        mma_batch<R, CT, S>(acc, lds, cs, g, k, kq);
-    }
-#endif
     for (; k < k1; k += 4) {
         const R g = glb[(size_t)(k + kq) * stride];
 #pragma unroll

@@ -204,22 +204,15 @@ constexpr int rm_per() {
 }
 
 // Radix of the next pass over the remaining length rem: 8, 4 or 2 while
-// rem is even, then the smallest odd prime factor.  COMPOSITE also takes 15
-// and 9 whole in registers (fewer passes), measured on MI355X and left off:
-// FP64 bw=263 (Bluestein length 1080) -7%, bw=37 (length 147) +9% per step
-// (the smooth-length instance went from 86 to 124 VGPRs), bw=255 and 100
-// unchanged; in FP32 their O(R) conjugate-pair sums cost accuracy (the mixed
-// form at bw = 255 went from below 1e-6 to 1.02e-6).
-#ifndef S2_MR_COMPOSITE
-#define S2_MR_COMPOSITE 0
-#endif
-template <bool COMPOSITE>
+// rem is even, then the smallest odd prime factor.  (Odd composites 15 and 9
+// whole in registers, fewer passes, measured on MI355X: FP64 bw=263 -7%,
+// bw=37 +9% per step as the smooth-length instance went from 86 to 124
+// VGPRs, bw=255 and 100 unchanged; in FP32 their O(R) conjugate-pair sums
+// took the mixed form at bw = 255 from below 1e-6 to 1.02e-6.  Not kept.)
 __device__ __forceinline__ int mr_reg_radix(int rem) {
     if (rem % 8 == 0) return 8;
     if (rem % 4 == 0) return 4;
     if (rem % 2 == 0) return 2;
-    if (COMPOSITE && rem % 15 == 0) return 15;
-    if (COMPOSITE && rem % 9 == 0) return 9;
     int R = 3;
     while (rem % R) R += 2;
     return R;
@@ -233,15 +226,12 @@ __device__ __forceinline__ void mr_reg_fft(T* a, int N, int nfft, const T* __res
     constexpr int E = rm_elems<T>();
     int rem = N, Ns = 1;
     while (rem > 1) {
-        constexpr bool comp = S2_MR_COMPOSITE && sizeof(T) == 16;
-        const int R = mr_reg_radix<comp>(rem);
+        const int R = mr_reg_radix(rem);
         switch (R) {
             case 2: mr_reg_pass<T, SIGN, 2, rm_per<E, 2>()>(a, N, Ns, nfft, tw); break;
             case 3: mr_reg_pass<T, SIGN, 3, rm_per<E, 3>()>(a, N, Ns, nfft, tw); break;
             case 4: mr_reg_pass<T, SIGN, 4, rm_per<E, 4>()>(a, N, Ns, nfft, tw); break;
             case 8: mr_reg_pass<T, SIGN, 8, rm_per<E, 8>()>(a, N, Ns, nfft, tw); break;
-            case 9: if constexpr (comp) mr_reg_pass<T, SIGN, 9, rm_per<E, 9>()>(a, N, Ns, nfft, tw); break;
-            case 15: if constexpr (comp) mr_reg_pass<T, SIGN, 15, rm_per<E, 15>()>(a, N, Ns, nfft, tw); break;
             case 5: mr_reg_pass<T, SIGN, 5, rm_per<E, 5>()>(a, N, Ns, nfft, tw); break;
             case 7: mr_reg_pass<T, SIGN, 7, rm_per<E, 7>()>(a, N, Ns, nfft, tw); break;
             case 11: if constexpr (PMAX >= 11) mr_reg_pass<T, SIGN, 11, rm_per<E, 11>()>(a, N, Ns, nfft, tw); break;
