@@ -20,8 +20,16 @@
 // Operands.  A (data or coefficients) goes from global memory straight into
 // registers in the MFMA operand layout; the K index is permuted so that each
 // lane reads S consecutive rings (degrees) per chunk.  B (the table) streams
-// through two LDS buffers: the next chunk's loads are issued before the
-// current chunk's MFMAs and stored after them, one barrier per chunk.
+// through two LDS buffers, one barrier per chunk: the next chunk is copied
+// global -> LDS directly (global_load_lds, no staging registers) before the
+// current chunk's MFMAs, and the forward reads its table fragments one tile
+// group ahead of their MFMAs.  The registers this frees give three waves per
+// SIMD (forward up to n0 = 8, inverse).  Measured against register staging,
+// fragments read up front and two waves per SIMD, results bit-identical:
+// bw=256 x 1024 forward GEMM 0.944 -> 0.873 ms, inverse 0.961 -> 0.905 ms
+// (step 4.12 -> 4.00 ms); bw=512 x 128 forward 0.798 -> 0.764 ms (its n0 = 16
+// instance's spills 118 -> 48 VGPRs; at three waves it spilled 155 and
+// slowed to 0.91 ms, so it keeps two), inverse 0.797 -> 0.738 ms.
 #include <algorithm>
 #include <cmath>
 
@@ -33,10 +41,10 @@ namespace s2 {
 typedef float fvec2 __attribute__((ext_vector_type(2)));
 
 #ifndef S2_MIX_WPE_FWD
-#define S2_MIX_WPE_FWD 2
+#define S2_MIX_WPE_FWD 3  // waves per SIMD; the n0 = 16 instance (bw > 256) keeps 2
 #endif
 #ifndef S2_MIX_WPE_INV
-#define S2_MIX_WPE_INV 2
+#define S2_MIX_WPE_INV 3
 #endif
 #ifndef S2_MIX_GJMAX
 #define S2_MIX_GJMAX 4
@@ -115,8 +123,8 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
     constexpr int NT = 2 * N0, G4 = NT / 4, R2 = NT % 4;  // R2: 0, or 2 tiles in a last pair
     constexpr int W = 16 * NT;
     constexpr int KC = mix_kc(N0), S = KC / 4;
-    constexpr int NV = KC * W / 4;  // float4 per table chunk
-    constexpr int PT = (NV + kMixThreads - 1) / kMixThreads;
+    constexpr int NV = KC * W / 4;  // float4 per table chunk (a whole number of KiB)
+    static_assert(NV % 64 == 0, "table chunks go to LDS in 1 KiB wave transfers");
     const int N = 2 * B;
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf;
@@ -136,13 +144,14 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
     const fvec4* tsrc = reinterpret_cast<const fvec4*>(NF + g0.offF);
 
     fvec4 xa[S / 2], xb[S / 2];  // rings j0 .. j0+S-1 and N-j0-S .. N-j0-1 (pairs of complex values)
-    fvec4 tv[PT];
     auto fetch = [&](int ch) __attribute__((always_inline)) {
-#pragma unroll
-        for (int u = 0; u < PT; ++u) {  // a partial last round re-reads the chunk's last float4
-            const int e = (NV % kMixThreads == 0) ? tid + u * kMixThreads : min(tid + u * kMixThreads, NV - 1);
-            tv[u] = tsrc[(size_t)ch * NV + e];
-        }
+        // the table chunk straight into its LDS buffer (global_load_lds, no
+        // staging registers), 1 KiB per wave instruction
+        const unsigned char* src = reinterpret_cast<const unsigned char*>(tsrc + (size_t)ch * NV);
+        unsigned char* dst = reinterpret_cast<unsigned char*>(sm + (ch & 1) * (KC * W));
+        for (int u = wave; u < NV / 64; u += kMixThreads / 64)
+            __builtin_amdgcn_global_load_lds((const void*)(src + u * 1024 + lane * 16),
+                                             (__attribute__((address_space(3))) void*)(dst + u * 1024), 16, 0, 0);
         const int j0 = ch * KC + kq * S;
         const fvec4* pa = reinterpret_cast<const fvec4*>(xr + j0);
         const fvec4* pb = reinterpret_cast<const fvec4*>(xr + N - j0 - S);
@@ -151,12 +160,6 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
             xa[h] = pa[h];
             xb[h] = pb[h];
         }
-    };
-    auto stash = [&](float* tb) __attribute__((always_inline)) {
-#pragma unroll
-        for (int u = 0; u < PT; ++u)
-            if (NV % kMixThreads == 0 || tid + u * kMixThreads < NV)
-                reinterpret_cast<fvec4*>(tb)[tid + u * kMixThreads] = tv[u];
     };
     float a0[S], a1[S];  // folded operands: x_j + x_{N-1-j}, x_j - x_{N-1-j}
     auto fold = [&]() __attribute__((always_inline)) {
@@ -177,7 +180,7 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
     for (int t = 0; t < NT; ++t) acc[t] = fvec4{0.f, 0.f, 0.f, 0.f};
     constexpr int TB = KC * W;  // floats per table buffer
     fetch(0);
-    stash(sm);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS transfers have landed
     __syncthreads();
     const int nch = mix_bp(B) / KC;  // rings past bw meet zero table rows
     for (int ch = 0; ch < nch; ++ch) {
@@ -188,19 +191,32 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const float* r = tb + s * W;
-            fvec4 tq[G4 > 0 ? G4 : 1];
+            // table fragments streamed one tile group ahead of their MFMAs
+            // (two float4 live instead of G4)
             fvec2 tp;
-#pragma unroll
-            for (int g = 0; g < G4; ++g) tq[g] = *reinterpret_cast<const fvec4*>(r + 64 * g + 4 * x16);
             if (R2) tp = *reinterpret_cast<const fvec2*>(r + 64 * G4 + 2 * x16);
+            fvec4 tcur = *reinterpret_cast<const fvec4*>(r + 4 * x16);
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const float a = t < N0 ? a0[s] : a1[s];
-                const float bv = (t < 4 * G4) ? tq[t >> 2][t & 3] : tp[t & 1];
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[t], 0, 0, 0);
+            for (int g = 0; g < G4; ++g) {
+                const fvec4 tnext = *reinterpret_cast<const fvec4*>(r + 64 * (g + 1 < G4 ? g + 1 : g) + 4 * x16);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int t = 4 * g + v;
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(t < N0 ? a0[s] : a1[s], tcur[v], acc[t], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                tcur = tnext;
+            }
+            if (R2) {
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    const int t = 4 * G4 + v;
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(t < N0 ? a0[s] : a1[s], tp[v], acc[t], 0, 0, 0);
+                }
             }
         }
-        if (more) stash(sm + ((ch + 1) & 1) * TB);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS transfers (chunk ch + 1) have landed
         __syncthreads();
     }
 
@@ -208,7 +224,7 @@ __device__ __forceinline__ void mix_fwd_body(const float2* __restrict__ X, const
 }
 
 template <int MAXN0>
-__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(S2_MIX_WPE_FWD))) void k_mix_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
+__global__ __launch_bounds__(kMixThreads) __attribute__((amdgpu_waves_per_eu(MAXN0 > 8 ? 2 : S2_MIX_WPE_FWD))) void k_mix_fwd(const float2* __restrict__ X, const float* __restrict__ NF,
                                                         const GemmMeta* __restrict__ meta, double* __restrict__ cre,
                                                         double* __restrict__ cim, int B, int F, int fmt, int nb) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -296,10 +312,18 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     // rows -m of odd m carry the sign (-1)^m: applied to the operand
     const unsigned nsign = ((((wave * 16 + x16) % cpf) >> 1) && (m & 1)) ? 0x80000000u : 0u;
     dpair8 cv[S];  // degree offsets d0 + 2h (even parity), d0 + 2h + 1 (odd)
-    fvec4 tv[PT];
+    constexpr int TB = 2 * KC * JW;  // floats per table buffer
     auto fetch = [&](int ch) __attribute__((always_inline)) {
+        // the table chunk gathered straight into its LDS buffer (global_load_lds:
+        // a global address per lane, 64 lanes' float4 contiguous in LDS): float4
+        // e of the chunk at position e
+        unsigned char* dst = reinterpret_cast<unsigned char*>(sm + (ch & 1) * TB);
 #pragma unroll
-        for (int u = 0; u < PT; ++u) tv[u] = *reinterpret_cast<const fvec4*>(tsrc[u] + (size_t)ch * KC * Bp);
+        for (int u = 0; u < PT; ++u)
+            if (FULL || u * kMixThreads + wave * 64 < NV)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(tsrc[u] + (size_t)ch * KC * Bp),
+                    (__attribute__((address_space(3))) void*)(dst + (u * kMixThreads + wave * 64) * 16), 16, 0, 0);
         const int d0 = 2 * (ch * KC + kq * S);
         if (d0 + 2 * S <= L) {
 #pragma unroll
@@ -309,11 +333,6 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
             for (int h = 0; h < S; ++h)
                 cv[h] = dpair8{d0 + 2 * h < L ? cp[d0 + 2 * h] : 0.0, d0 + 2 * h + 1 < L ? cp[d0 + 2 * h + 1] : 0.0};
         }
-    };
-    auto stash = [&](float* tb) __attribute__((always_inline)) {
-#pragma unroll
-        for (int u = 0; u < PT; ++u)
-            if (FULL || tid < NV) reinterpret_cast<fvec4*>(tb)[tid + u * kMixThreads] = tv[u];
     };
     float a[2][S];
     auto convert = [&]() __attribute__((always_inline)) {
@@ -329,9 +348,8 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
     for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int t = 0; t < JT; ++t) acc[p][t] = fvec4{0.f, 0.f, 0.f, 0.f};
-    constexpr int TB = 2 * KC * JW;  // floats per table buffer
     fetch(0);
-    stash(sm);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS transfers have landed
     __syncthreads();
     const int nch = g0.ki / KC;
     for (int ch = 0; ch < nch; ++ch) {
@@ -372,7 +390,7 @@ __device__ __forceinline__ void mix_inv_body(const double* __restrict__ cre, con
                             __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][s], tq[b][p][g][v], acc[p][TV * g + v], 0, 0, 0);
                 }
         }
-        if (more) stash(sm + ((ch + 1) & 1) * TB);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS transfers (chunk ch + 1) have landed
         __syncthreads();
     }
 
