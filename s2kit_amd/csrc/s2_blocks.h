@@ -491,6 +491,22 @@ struct Mma<float> {
 
 inline int lg2(int n) { return __builtin_ctz(n); }
 
+// Calls of few fields: the default ring-FFT/DCT blocks (512 threads, 8 FP64 or
+// 16 FP32 rings) would give fewer than 256 blocks, one per CU (one field at
+// bw=256: 64), so one-wave blocks (64 threads; 128 for the longest rings)
+// take one or two rings each: eight times the blocks, each block's passes
+// synchronised within one or two waves.  Measured per fwd+inv pair of one
+// field: bw=64 46.9 -> 31.7 us, 128 51.7 -> 37.8, 256 72.1 -> 64.2; at four
+// fields (256 default blocks) the one-wave blocks were slower (92.9 -> 100.3
+// us at bw=256).  Returns the block's threads, 0 for the default blocks.
+template <int V>
+inline int few_field_threads(int N, int F) {
+    if ((long)F * N * N >= 512L * V * 256) return 0;
+    if (N <= 64 * V) return 64;
+    if (N <= 128 * V) return 128;
+    return 0;
+}
+
 // Launch with `lds` bytes of dynamic LDS; above 64 KiB the kernel must opt
 // in (gfx950 has 160 KiB per workgroup).
 template <typename K, typename... A>

@@ -699,22 +699,6 @@ void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre,
         (two ? inv_launch<double, false, 2> : inv_launch<double, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
 }
 
-// Calls of few fields: the default ring-FFT/DCT blocks (512 threads, 8 FP64 or
-// 16 FP32 rings) would give fewer than 256 blocks, one per CU (one field at
-// bw=256: 64), so one-wave blocks (64 threads; 128 for the longest rings)
-// take one or two rings each: eight times the blocks, each block's passes
-// synchronised within one or two waves.  Measured per fwd+inv pair of one
-// field: bw=64 46.9 -> 31.7 us, 128 51.7 -> 37.8, 256 72.1 -> 64.2; at four
-// fields (256 default blocks) the one-wave blocks were slower (92.9 -> 100.3
-// us at bw=256).  Returns the block's threads, 0 for the default blocks.
-template <int V>
-static int few_field_threads(int N, int F) {
-    if ((long)F * N * N >= 512L * V * 256) return 0;
-    if (N <= 64 * V) return 64;
-    if (N <= 128 * V) return 128;
-    return 0;
-}
-
 template <typename R>
 static void fwd_impl(const DevConsts& c, const GemmTables& g, const double* re, const double* im, double* cre,
                      double* cim, int F, Fmt fmt, void* Xv, void* Cbv, hipStream_t s, StageTimer* tm) {
