@@ -120,6 +120,17 @@ def kernels_gen(bw, form):
             "dct_inv": f"k_dct_inv<double2, {mode}>", "phi_inv": f"k_phi_inv<double2, {mode}>"}
 
 
+def kernel_map(form, bw, nfields):
+    """Kernel behind each timed stage of a batched call of nfields fields on
+    an all-seminaive plan.  The FP64/FP32 GEMMs run K-chunked (64 columns per
+    block) at bw in [128, 256] with 64 or more fields."""
+    k = dict(KERNELS[form] if bw & (bw - 1) == 0 else kernels_gen(bw, form))
+    if form in ("fp64", "fp32") and 128 <= bw <= 256 and nfields >= 64:
+        r = "double" if form == "fp64" else "float"
+        k["leg_fwd"], k["leg_inv"] = f"k_gemm_fwd_kc<{r}>", f"k_gemm_inv_kc<{r}>"
+    return k
+
+
 
 def measured_traffic(form, bw, nfields, stage):
     """HBM bytes per launch of `stage`'s kernel from the committed rocprofv3
@@ -685,7 +696,7 @@ def main():
     achieved = stage_bytes(dom, bw, F, elem, args.precision) / avg_s
     roofline = {
         "bound": "hbm",
-        "kernel": (KERNELS[args.precision] if bw & (bw - 1) == 0 else kernels_gen(bw, args.precision)).get(dom, dom),
+        "kernel": kernel_map(args.precision, bw, F).get(dom, dom),
         "stage": dom,
         "achieved": achieved / 1e9,
         "peak": HBM_PEAK / 1e9,
@@ -699,7 +710,7 @@ def main():
 
     # ---- MFMA utilisation of the Legendre GEMMs ---------------------------
     peak, issue = MFMA_PEAK[args.precision]
-    kmap = KERNELS[args.precision] if bw & (bw - 1) == 0 else kernels_gen(bw, args.precision)
+    kmap = kernel_map(args.precision, bw, F)
     mfma = {}
     for st_ in ("leg_fwd", "leg_inv"):
         if st_ in stages:

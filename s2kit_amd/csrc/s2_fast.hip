@@ -532,6 +532,248 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// K-chunked seminaive GEMMs (every order seminaive, many columns)
+// ---------------------------------------------------------------------------
+// The kernels above hold whole data rows in LDS: at bw=256 FP64 a column takes
+// 2 KiB, so four blocks per CU leave 16 columns per block and every table value
+// loaded feeds one MFMA.  Here a block owns kKcCols = 64 columns of one order
+// and the data stream through two LDS chunk buffers of kKc = 16 K positions
+// (staged in registers one chunk ahead); wave w of eight owns the 16-wide
+// tile w of each parity with all four column tiles, so a table value feeds four
+// MFMAs and a block reads its order's table once per 64 columns.  The table
+// values of the next chunk are loaded before the current chunk's MFMAs.  Each
+// accumulator takes its K steps in the same ascending order as the kernels
+// above: the results are the same.
+constexpr int kKcCols = 64, kKc = 16, kKcThreads = 512, kKcWaves = kKcThreads / 64, kKcNS = 1, kKcRS = kKc + 2;
+constexpr int kKcPer = kKcCols * kKc / kKcThreads;  // chunk values staged per thread
+constexpr int kKcSteps = kKc / 4;                   // K steps of 4 per chunk
+
+// Data column col of order m: the start of its Cb row (field, +-m, re/im).
+template <typename R>
+__device__ __forceinline__ const R* kc_row(const R* __restrict__ Cb, int col, int m, int N, int B, int cpf) {
+    const int f = col / cpf, rem = col - f * cpf;
+    const int r = (rem >> 1) ? N - m : m;
+    return Cb + (((size_t)f * N + r) * 2 + (rem & 1)) * B;
+}
+
+// Forward (seminaive.c:183-197): D'[c][i] = sum_t data[c][p*hE + t] AF(m,p)[t][i].
+template <typename R>
+__global__ __launch_bounds__(kKcThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_fwd_kc(const R* __restrict__ Cb, const R* __restrict__ AF,
+                                                          const GemmMeta* __restrict__ meta, double* __restrict__ cre,
+                                                          double* __restrict__ cim, int B, int F, int fmt) {
+    __shared__ R Bd[2][kKcCols * kKcRS];
+    int bx, m;
+    if (!gemm_block(B, bx, m)) return;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf, cbase = bx * kKcCols;
+    if (cbase >= ncols) return;
+    const int N = 2 * B, hE = (B + 1) / 2, hO = B / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, kq = lane >> 4, ln = lane & 15;
+    const int fc = tid / (kKc / kKcPer), fk = kKcPer * (tid % (kKc / kKcPer));  // staging: column fc, positions fk ..
+    const bool fcol = cbase + fc < ncols;
+    const R* frow = fcol ? kc_row(Cb, cbase + fc, m, N, B, cpf) : Cb;
+    for (int p = 0; p < 2; ++p) {
+        const GemmMeta g = meta[m * 2 + p];
+        if (g.Lp == 0) continue;
+        const int Li16 = (g.Lp + 15) & ~15, ntile = Li16 >> 4;
+        const int cnt = p ? hO : hE;
+        const R* src = frow + p * hE;
+        const R* ga = AF + g.offF;  // this parity's table (32-bit offsets from here)
+        // K extent of each of this wave's tiles (multiples of 4), and of the block
+        int kend[kKcNS];
+#pragma unroll
+        for (int s = 0; s < kKcNS; ++s) {
+            const int it = wave + kKcWaves * s;
+            kend[s] = it < ntile ? (row_size(m, m + p + 2 * min(16 * it + 15, g.Lp - 1)) + 3) & ~3 : 0;
+        }
+        const int nq = ((row_size(m, m + p + 2 * (g.Lp - 1)) + 3) / 4 * 4 + kKc - 1) / kKc;
+        typename Mma<R>::acc acc[kKcNS][4] = {};
+        R v[kKcPer], ta[kKcNS][kKcSteps], tb[kKcNS][kKcSteps];
+        auto stage = [&](int q) __attribute__((always_inline)) {
+#pragma unroll
+            for (int e = 0; e < kKcPer; ++e) {
+                const int t = q * kKc + fk + e;
+                v[e] = (fcol && t < cnt) ? src[t] : (R)0;
+            }
+        };
+        auto table = [&](int q, R (&tv)[kKcNS][kKcSteps]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int s = 0; s < kKcNS; ++s)
+#pragma unroll
+                for (int u = 0; u < kKcSteps; ++u) {
+                    const int k = q * kKc + 4 * u;
+                    tv[s][u] = k < kend[s] ? ga[(k + kq) * Li16 + 16 * (wave + kKcWaves * s) + ln] : (R)0;
+                }
+        };
+        auto put = [&](int b) __attribute__((always_inline)) {
+#pragma unroll
+            for (int e = 0; e < kKcPer; ++e) Bd[b][fc * kKcRS + fk + e] = v[e];
+        };
+        auto chunk = [&](int q, const R (&tv)[kKcNS][kKcSteps]) __attribute__((always_inline)) {
+            const R* bd = Bd[q & 1] + ln * kKcRS + kq;
+#pragma unroll
+            for (int s = 0; s < kKcNS; ++s)
+#pragma unroll
+                for (int u = 0; u < kKcSteps; ++u)
+                    if (q * kKc + 4 * u < kend[s]) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            acc[s][c] = Mma<R>::mma(bd[16 * c * kKcRS + 4 * u], tv[s][u], acc[s][c]);
+                    }
+        };
+        stage(0);
+        table(0, ta);
+        put(0);
+        __syncthreads();
+        for (int q = 0; q < nq; q += 2) {
+            if (q + 1 < nq) {
+                stage(q + 1);
+                table(q + 1, tb);
+            }
+            chunk(q, ta);
+            if (q + 1 < nq) put(1);
+            __syncthreads();
+            if (q + 1 >= nq) break;
+            if (q + 2 < nq) {
+                stage(q + 2);
+                table(q + 2, ta);
+            }
+            chunk(q + 1, tb);
+            if (q + 2 < nq) put(0);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int s = 0; s < kKcNS; ++s) {
+            const int i = 16 * (wave + kKcWaves * s) + ln;
+            if (i >= g.Lp) continue;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = cbase + 16 * c + Mma<R>::row(lane, r);
+                    if (col < ncols) fwd_store((double)acc[s][c][r], col, m, m + p + 2 * i, B, cpf, fmt, cre, cim);
+                }
+        }
+    }
+}
+
+// Inverse (seminaive.c:74-98): D'[c][t] = sum_i coef[c][m+p+2i] AI(m,p)[i][t],
+// into Cb row (field, +-m, re/im) at p*hE + t.
+template <typename R>
+__global__ __launch_bounds__(kKcThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_inv_kc(const double* __restrict__ cre,
+                                                          const double* __restrict__ cim, const R* __restrict__ AI,
+                                                          const GemmMeta* __restrict__ meta, R* __restrict__ Cb, int B,
+                                                          int F, int fmt) {
+    __shared__ R Cd[2][kKcCols * kKcRS];
+    int bx, m;
+    if (!gemm_block(B, bx, m)) return;
+    const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
+    const int ncols = F * cpf, cbase = bx * kKcCols;
+    if (cbase >= ncols) return;
+    const int N = 2 * B, hE = (B + 1) / 2, hO = B / 2, Tp16 = (hE + 15) & ~15, ntt = Tp16 >> 4;
+    const size_t B2 = (size_t)B * B;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, kq = lane >> 4, ln = lane & 15;
+    const int fc = tid / (kKc / kKcPer), fk = kKcPer * (tid % (kKc / kKcPer));  // staging: column fc, positions fk ..
+    const bool fcol = cbase + fc < ncols;
+    int fmm = m;
+    const double* fsrc = cre;
+    if (fcol) {
+        const int col = cbase + fc;
+        const int f = col / cpf, rem = col - f * cpf;
+        fmm = (rem >> 1) ? -m : m;
+        fsrc = ((rem & 1) ? cim : cre) + f * B2;
+    }
+    for (int p = 0; p < 2; ++p) {
+        // a parity without degrees (Lp = 0) still stores its zero cosine
+        // coefficients: the DCT reads every position below its count
+        const GemmMeta g = meta[m * 2 + p];
+        const int Li4 = (g.Lp + 3) & ~3;
+        const int cnt = p ? hO : hE;
+        const int r0 = row_size(m, m + p);
+        const R* ga = AI + g.offI;  // this parity's table (32-bit offsets from here)
+        // K start of each of this wave's output tiles (multiples of 4); K ends at Li4
+        int kst[kKcNS];
+#pragma unroll
+        for (int s = 0; s < kKcNS; ++s) {
+            const int tt = wave + kKcWaves * s;
+            kst[s] = tt < ntt ? max(0, 16 * tt - r0 + 1) & ~3 : Li4;
+        }
+        const int nq = (Li4 + kKc - 1) / kKc;
+        typename Mma<R>::acc acc[kKcNS][4] = {};
+        R v[kKcPer], ta[kKcNS][kKcSteps], tb[kKcNS][kKcSteps];
+        auto stage = [&](int q) __attribute__((always_inline)) {
+#pragma unroll
+            for (int e = 0; e < kKcPer; ++e) {
+                const int i = q * kKc + fk + e;
+                v[e] = (fcol && i < g.Lp) ? (R)fsrc[index_of_coeff(fmm, m + p + 2 * i, B)] : (R)0;
+            }
+        };
+        auto active = [&](int s, int k) __attribute__((always_inline)) { return k >= kst[s] && k < Li4; };
+        auto table = [&](int q, R (&tv)[kKcNS][kKcSteps]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int s = 0; s < kKcNS; ++s)
+#pragma unroll
+                for (int u = 0; u < kKcSteps; ++u) {
+                    const int k = q * kKc + 4 * u;
+                    tv[s][u] = active(s, k) ? ga[(k + kq) * Tp16 + 16 * (wave + kKcWaves * s) + ln] : (R)0;
+                }
+        };
+        auto put = [&](int b) __attribute__((always_inline)) {
+#pragma unroll
+            for (int e = 0; e < kKcPer; ++e) Cd[b][fc * kKcRS + fk + e] = v[e];
+        };
+        auto chunk = [&](int q, const R (&tv)[kKcNS][kKcSteps]) __attribute__((always_inline)) {
+            const R* cd = Cd[q & 1] + ln * kKcRS + kq;
+#pragma unroll
+            for (int s = 0; s < kKcNS; ++s)
+#pragma unroll
+                for (int u = 0; u < kKcSteps; ++u)
+                    if (active(s, q * kKc + 4 * u)) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            acc[s][c] = Mma<R>::mma(cd[16 * c * kKcRS + 4 * u], tv[s][u], acc[s][c]);
+                    }
+        };
+        stage(0);
+        table(0, ta);
+        put(0);
+        __syncthreads();
+        for (int q = 0; q < nq; q += 2) {
+            if (q + 1 < nq) {
+                stage(q + 1);
+                table(q + 1, tb);
+            }
+            chunk(q, ta);
+            if (q + 1 < nq) put(1);
+            __syncthreads();
+            if (q + 1 >= nq) break;
+            if (q + 2 < nq) {
+                stage(q + 2);
+                table(q + 2, ta);
+            }
+            chunk(q + 1, tb);
+            if (q + 2 < nq) put(0);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int s = 0; s < kKcNS; ++s) {
+            const int t = 16 * (wave + kKcWaves * s) + ln;
+            if (t >= cnt) continue;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = cbase + 16 * c + Mma<R>::row(lane, r);
+                    if (col >= ncols) continue;
+                    const int f = col / cpf, rem = col - f * cpf;
+                    const int row = (rem >> 1) ? N - m : m;
+                    Cb[(((size_t)f * N + row) * 2 + (rem & 1)) * B + p * hE + t] = acc[s][c][r];
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
@@ -673,8 +915,30 @@ static void gemm_tiles(const DevConsts& c, const GemmTables& g, int& ncb, bool& 
     two = (g.t_ncb ? g.t_ct : gemm_ct(ncb, c.bw)) == 2;
 }
 
+// The K-chunked kernels take calls of many fields on plans whose orders are
+// all seminaive, bw in [128, 256] (at most eight 16-wide tiles per parity:
+// one per wave; at bw = 64 they were slower, 31.5 -> 34.0 us forward at 256
+// fields).
+static bool kc_ok(const DevConsts& c, const GemmTables& g, int F) {
+    return g.cut == c.bw && c.bw >= 128 && c.bw <= 256 && F >= 64;
+}
+
+static dim3 kc_grid(int B, int F) { return dim3((4 * F + kKcCols - 1) / kKcCols, (B + 7) & ~7); }
+
 void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre, double* cim,
                      int F, Fmt fmt, hipStream_t s) {
+    if (kc_ok(c, g, F)) {
+        if (g.prec != 0)
+            k_gemm_fwd_kc<float><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>((const float*)Cb, (const float*)g.af,
+                                                                     (const GemmMeta*)g.meta, cre, cim, c.bw, F,
+                                                                     (int)fmt);
+        else
+            k_gemm_fwd_kc<double><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>((const double*)Cb, (const double*)g.af,
+                                                                      (const GemmMeta*)g.meta, cre, cim, c.bw, F,
+                                                                      (int)fmt);
+        S2_HIP(hipGetLastError());
+        return;
+    }
     int ncb;
     bool two;
     gemm_tiles(c, g, ncb, two);
@@ -688,6 +952,18 @@ void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, co
 
 void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
                      int F, Fmt fmt, hipStream_t s) {
+    if (kc_ok(c, g, F)) {
+        if (g.prec != 0)
+            k_gemm_inv_kc<float><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>(cre, cim, (const float*)g.ai,
+                                                                     (const GemmMeta*)g.meta, (float*)Cb, c.bw, F,
+                                                                     (int)fmt);
+        else
+            k_gemm_inv_kc<double><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>(cre, cim, (const double*)g.ai,
+                                                                      (const GemmMeta*)g.meta, (double*)Cb, c.bw, F,
+                                                                      (int)fmt);
+        S2_HIP(hipGetLastError());
+        return;
+    }
     int ncb;
     bool two;
     gemm_tiles(c, g, ncb, two);
