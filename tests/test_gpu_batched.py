@@ -455,3 +455,78 @@ def test_few_field_calls_match_batched(env, bw, prec):
         outs.append((gre, gim, c2r, c2i))
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("bw,prec", [(256, 0), (128, 0), (255, 0), (256, 1), (128, 1)])
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_k_chunked_gemms_match_whole_row_gemms(env, O, bw, prec, fmt):
+    # Calls of 64 or more fields at bw in [128, 256] run the K-chunked GEMMs
+    # (64 columns per block, data streamed through LDS chunks); calls of 8
+    # fields the whole-row GEMMs.  Both take every accumulator's K steps in
+    # the same order, so the results agree bit for bit; two fields are also
+    # checked against the oracle (FP64).
+    S, torch, dev = env
+    F, per, n = 64, 8, 2 * bw
+    print(f"k-chunked bw={bw} prec={prec} fmt={fmt}", flush=True)
+    kind = 2 if fmt == 1 else 0
+    coeffs = [O.make_coeffs(300 + f, bw, kind) for f in range(F)]
+    cre = to_dev(torch, dev, [c[0] for c in coeffs])
+    cim = to_dev(torch, dev, [c[1] for c in coeffs])
+    outs = []
+    for per_call in (F, per):
+        plan = S.Plan(bw, per_call, precision=prec)
+        gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+        gim = torch.empty_like(gre)
+        c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+        for f0 in range(0, F, per_call):
+            sl = slice(f0, f0 + per_call)
+            plan.inverse(cre[sl], cim[sl], gre[sl], gim[sl], data_format=fmt)
+            plan.forward(gre[sl], gim[sl], c2r[sl], c2i[sl], data_format=fmt)
+        torch.cuda.synchronize()
+        plan.close()
+        outs.append((gre, gim, c2r, c2i))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+    if prec == 0:
+        gre, gim, c2r, c2i = outs[0]
+        for f in (0, F - 1):
+            og = O.inv_fst(coeffs[f][0], coeffs[f][1], bw, data_format=fmt)
+            assert crel(gre[f].cpu().numpy().ravel(), gim[f].cpu().numpy().ravel(), *og) < 1e-11
+            oc = O.fst(gre[f].cpu().numpy().ravel(), gim[f].cpu().numpy().ravel(), bw, data_format=fmt)
+            assert crel(c2r[f].cpu().numpy(), c2i[f].cpu().numpy(), *oc) < 1e-11
+
+
+@pytest.mark.parametrize("bw,F,prec", [(128, 64, 0), (128, 8, 0), (256, 64, 1), (255, 64, 0)])
+def test_real_inverse_does_not_depend_on_earlier_calls(env, bw, F, prec):
+    # Every call writes all the plan scratch its later stages read: a REAL
+    # inverse gives the same bits on a fresh plan and after COMPLEX and REAL
+    # calls on the same plan have left their intermediates behind (an order
+    # whose odd parity has no degree still stores its zero cosine row).
+    S, torch, dev = env
+    from s2kit_amd import synth
+    n = 2 * bw
+    cre, cim = synth.power_law_coeffs(bw, F, 41, dev)
+    p0 = S.Plan(bw, F, precision=prec)
+    gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    gim = torch.empty_like(gre)
+    p0.inverse(cre, cim, gre, gim)
+    grr, gri = gre.clone(), torch.zeros_like(gim)
+    crr, cri = torch.empty_like(cre), torch.empty_like(cim)
+    p0.forward(grr, gri, crr, cri, data_format=S.REAL)
+    p0.close()
+
+    def real_inverse(plan):
+        a, b = torch.empty_like(gre), torch.empty_like(gim)
+        plan.inverse(crr, cri, a, b, data_format=S.REAL)
+        torch.cuda.synchronize()
+        return a, b
+
+    plan = S.Plan(bw, F, precision=prec)
+    fresh = real_inverse(plan)
+    c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
+    plan.inverse(cre, cim, gre, gim)
+    plan.forward(gre, gim, c2r, c2i)
+    plan.forward(grr, gri, c2r, c2i, data_format=S.REAL)
+    later = real_inverse(plan)
+    plan.close()
+    assert torch.equal(fresh[0], later[0]) and torch.equal(fresh[1], later[1])
