@@ -13,9 +13,13 @@
 // Tables (cospml.c:161-362, pml.c:41-79, weights.c:32-46) are generated on
 // the GPU in FP64 by k_gen_cospml / k_transpose / k_gen_pml / k_weights.
 //
-// FFTs are Stockham passes in LDS: radix 2 for power-of-two lengths, mixed
-// radix (one pass per prime factor, radix 4 for pairs of 2s) otherwise.  Both DCTs use one complex FFT of length N = 2B on a
-// reordered sequence (Makhoul); the DCTs are real-linear with real kernels,
+// The ring FFTs of these kernels come in four instances: radix-2 Stockham
+// passes in LDS for power-of-two lengths, in-place register passes of radix
+// 2/4/8 and odd primes up to 19 for lengths with no larger prime factor, and
+// Bluestein's convolution (power-of-two or 7-smooth length) otherwise;
+// power-of-two grids of 64 .. 2048 points take the register radix-8 kernels
+// of s2_blocks.h instead.  Both DCTs use one complex FFT of length N = 2B on
+// a reordered sequence (Makhoul); the DCTs are real-linear with real kernels,
 // so one complex row carries the real and imaginary transforms together.
 #include <cmath>
 #include <cstdio>
