@@ -825,8 +825,10 @@ __device__ __forceinline__ double sum_row(double v) {
 
 // Forward (DLTSemi's projections, seminaive.c:183-197) for orders m < cutoff:
 // row l of order m against the order's cosine coefficients Cb[f][+-m][k].
-template <int NF, int MAXI, int LPR>
-__global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict__ Cb, const double* __restrict__ T,
+// NTH threads: 512 from bw = 512 on (twice the row groups per block for the
+// same data fill: bw=512, one field, 59 -> 53 us; the inverse lost, 65 -> 72).
+template <int NF, int MAXI, int LPR, int NTH>
+__global__ __launch_bounds__(NTH) void k_leg_fwd_rows(const double2* __restrict__ Cb, const double* __restrict__ T,
                                                       const long* __restrict__ toff, double* __restrict__ cre,
                                                       double* __restrict__ cim, int N, int B, int F, int fmt, int rpb,
                                                       int planar) {
@@ -855,7 +857,7 @@ __global__ __launch_bounds__(256) void k_leg_fwd_rows(const double2* __restrict_
     const size_t B2 = (size_t)B * B;
     const double sg = (m & 1) ? -1.0 : 1.0;
     const int lend = min(L, l0 + rpb);
-    for (int li = l0 + g; li < lend; li += 256 / LPR) {
+    for (int li = l0 + g; li < lend; li += NTH / LPR) {
         const int l = m + li;
         const int rs = row_size(m, l);
         const double* tp = tm + table_offset(m, l);
@@ -1923,10 +1925,16 @@ void forward_fp64(const DevConsts& c, const DevTables& t, const double* re, cons
             const int rpb = rows_per_block(B);
             const dim3 grid((B + rpb - 1) / rpb, cut, (F + nf - 1) / nf);
             const size_t lds = sizeof(double2) * 2 * B * nf;
-            rows_launch(B, nf, true, [&](auto kern) { kern<<<grid, 256, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb, planar); },
-                        [](auto nf_, auto mx, auto lp) {
-                            return k_leg_fwd_rows<decltype(nf_)::value, decltype(mx)::value, decltype(lp)::value>;
-                        });
+            if (B >= 512)
+                rows_launch(B, nf, true, [&](auto kern) { kern<<<grid, 512, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb, planar); },
+                            [](auto nf_, auto mx, auto lp) {
+                                return k_leg_fwd_rows<decltype(nf_)::value, decltype(mx)::value, decltype(lp)::value, 512>;
+                            });
+            else
+                rows_launch(B, nf, true, [&](auto kern) { kern<<<grid, 256, lds, s>>>(Cb, t.fwd, t.off, cre, cim, N, B, F, f, rpb, planar); },
+                            [](auto nf_, auto mx, auto lp) {
+                                return k_leg_fwd_rows<decltype(nf_)::value, decltype(mx)::value, decltype(lp)::value, 256>;
+                            });
         } else {
             const int fc = std::max(1, 65536 / (2 * B * (int)sizeof(double2)));
             k_leg_fwd<<<dim3(cut, (F + fc - 1) / fc), 256, sizeof(double2) * 2 * B * std::min(fc, F), s>>>(
