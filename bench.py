@@ -123,11 +123,13 @@ def kernels_gen(bw, form):
 def kernel_map(form, bw, nfields):
     """Kernel behind each timed stage of a batched call of nfields fields on
     an all-seminaive plan.  The FP64/FP32 GEMMs run K-chunked (64 columns per
-    block) at bw in [128, 256] with 64 or more fields."""
+    block; 512 threads up to bw = 256, 1024 above) at bw in [128, 512] with 64
+    or more fields."""
     k = dict(KERNELS[form] if bw & (bw - 1) == 0 else kernels_gen(bw, form))
-    if form in ("fp64", "fp32") and 128 <= bw <= 256 and nfields >= 64:
+    if form in ("fp64", "fp32") and 128 <= bw <= 512 and nfields >= 64:
         r = "double" if form == "fp64" else "float"
-        k["leg_fwd"], k["leg_inv"] = f"k_gemm_fwd_kc<{r}>", f"k_gemm_inv_kc<{r}>"
+        t = 512 if bw <= 256 else 1024
+        k["leg_fwd"], k["leg_inv"] = f"k_gemm_fwd_kc<{r}, {t}>", f"k_gemm_inv_kc<{r}, {t}>"
     return k
 
 

@@ -538,14 +538,14 @@ __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict_
 // 2 KiB, so four blocks per CU leave 16 columns per block and every table value
 // loaded feeds one MFMA.  Here a block owns kKcCols = 64 columns of one order
 // and the data stream through two LDS chunk buffers of kKc = 16 K positions
-// (staged in registers one chunk ahead); wave w of eight owns the 16-wide
-// tile w of each parity with all four column tiles, so a table value feeds four
+// (staged in registers one chunk ahead); wave w owns the 16-wide tile w of
+// each parity (eight waves up to bw = 256, sixteen at bw = 512: one wave per
+// tile) with all four column tiles, so a table value feeds four
 // MFMAs and a block reads its order's table once per 64 columns.  The table
 // values of the next chunk are loaded before the current chunk's MFMAs.  Each
 // accumulator takes its K steps in the same ascending order as the kernels
 // above: the results are the same.
-constexpr int kKcCols = 64, kKc = 16, kKcThreads = 512, kKcWaves = kKcThreads / 64, kKcNS = 1, kKcRS = kKc + 2;
-constexpr int kKcPer = kKcCols * kKc / kKcThreads;  // chunk values staged per thread
+constexpr int kKcCols = 64, kKc = 16, kKcNS = 1, kKcRS = kKc + 2;
 constexpr int kKcSteps = kKc / 4;                   // K steps of 4 per chunk
 
 // Data column col of order m: the start of its Cb row (field, +-m, re/im).
@@ -557,8 +557,8 @@ __device__ __forceinline__ const R* kc_row(const R* __restrict__ Cb, int col, in
 }
 
 // Forward (seminaive.c:183-197): D'[c][i] = sum_t data[c][p*hE + t] AF(m,p)[t][i].
-template <typename R>
-__global__ __launch_bounds__(kKcThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_fwd_kc(const R* __restrict__ Cb, const R* __restrict__ AF,
+template <typename R, int NTH>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_fwd_kc(const R* __restrict__ Cb, const R* __restrict__ AF,
                                                           const GemmMeta* __restrict__ meta, double* __restrict__ cre,
                                                           double* __restrict__ cim, int B, int F, int fmt) {
     __shared__ R Bd[2][kKcCols * kKcRS];
@@ -569,6 +569,7 @@ __global__ __launch_bounds__(kKcThreads) __attribute__((amdgpu_waves_per_eu(4)))
     if (cbase >= ncols) return;
     const int N = 2 * B, hE = (B + 1) / 2, hO = B / 2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, kq = lane >> 4, ln = lane & 15;
+    constexpr int kKcWaves = NTH / 64, kKcPer = kKcCols * kKc / NTH;  // waves; chunk values staged per thread
     const int fc = tid / (kKc / kKcPer), fk = kKcPer * (tid % (kKc / kKcPer));  // staging: column fc, positions fk ..
     const bool fcol = cbase + fc < ncols;
     const R* frow = fcol ? kc_row(Cb, cbase + fc, m, N, B, cpf) : Cb;
@@ -659,8 +660,8 @@ __global__ __launch_bounds__(kKcThreads) __attribute__((amdgpu_waves_per_eu(4)))
 
 // Inverse (seminaive.c:74-98): D'[c][t] = sum_i coef[c][m+p+2i] AI(m,p)[i][t],
 // into Cb row (field, +-m, re/im) at p*hE + t.
-template <typename R>
-__global__ __launch_bounds__(kKcThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_inv_kc(const double* __restrict__ cre,
+template <typename R, int NTH>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_inv_kc(const double* __restrict__ cre,
                                                           const double* __restrict__ cim, const R* __restrict__ AI,
                                                           const GemmMeta* __restrict__ meta, R* __restrict__ Cb, int B,
                                                           int F, int fmt) {
@@ -673,6 +674,7 @@ __global__ __launch_bounds__(kKcThreads) __attribute__((amdgpu_waves_per_eu(4)))
     const int N = 2 * B, hE = (B + 1) / 2, hO = B / 2, Tp16 = (hE + 15) & ~15, ntt = Tp16 >> 4;
     const size_t B2 = (size_t)B * B;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, kq = lane >> 4, ln = lane & 15;
+    constexpr int kKcWaves = NTH / 64, kKcPer = kKcCols * kKc / NTH;  // waves; chunk values staged per thread
     const int fc = tid / (kKc / kKcPer), fk = kKcPer * (tid % (kKc / kKcPer));  // staging: column fc, positions fk ..
     const bool fcol = cbase + fc < ncols;
     int fmm = m;
@@ -916,12 +918,14 @@ static void gemm_tiles(const DevConsts& c, const GemmTables& g, int& ncb, bool& 
 }
 
 // The K-chunked kernels take calls of many fields on plans whose orders are
-// all seminaive, bw in [128, 256] (at most eight 16-wide tiles per parity:
+// all seminaive, bw in [128, 512] (at most sixteen 16-wide tiles per parity:
 // one per wave; at bw = 64 they were slower, 31.5 -> 34.0 us forward at 256
 // fields).
 static bool kc_ok(const DevConsts& c, const GemmTables& g, int F) {
-    return g.cut == c.bw && c.bw >= 128 && c.bw <= 256 && F >= 64;
+    return g.cut == c.bw && c.bw >= 128 && c.bw <= 512 && F >= 64;
 }
+// threads of a K-chunked block: one wave per 16-wide tile of a parity
+static int kc_threads(int B) { return B <= 256 ? 512 : 1024; }
 
 static dim3 kc_grid(int B, int F) { return dim3((4 * F + kKcCols - 1) / kKcCols, (B + 7) & ~7); }
 
@@ -929,11 +933,11 @@ void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, co
                      int F, Fmt fmt, hipStream_t s) {
     if (kc_ok(c, g, F)) {
         if (g.prec != 0)
-            k_gemm_fwd_kc<float><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>((const float*)Cb, (const float*)g.af,
+            (kc_threads(c.bw) == 512 ? k_gemm_fwd_kc<float, 512> : k_gemm_fwd_kc<float, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>((const float*)Cb, (const float*)g.af,
                                                                      (const GemmMeta*)g.meta, cre, cim, c.bw, F,
                                                                      (int)fmt);
         else
-            k_gemm_fwd_kc<double><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>((const double*)Cb, (const double*)g.af,
+            (kc_threads(c.bw) == 512 ? k_gemm_fwd_kc<double, 512> : k_gemm_fwd_kc<double, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>((const double*)Cb, (const double*)g.af,
                                                                       (const GemmMeta*)g.meta, cre, cim, c.bw, F,
                                                                       (int)fmt);
         S2_HIP(hipGetLastError());
@@ -954,11 +958,11 @@ void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre,
                      int F, Fmt fmt, hipStream_t s) {
     if (kc_ok(c, g, F)) {
         if (g.prec != 0)
-            k_gemm_inv_kc<float><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>(cre, cim, (const float*)g.ai,
+            (kc_threads(c.bw) == 512 ? k_gemm_inv_kc<float, 512> : k_gemm_inv_kc<float, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>(cre, cim, (const float*)g.ai,
                                                                      (const GemmMeta*)g.meta, (float*)Cb, c.bw, F,
                                                                      (int)fmt);
         else
-            k_gemm_inv_kc<double><<<kc_grid(c.bw, F), kKcThreads, 0, s>>>(cre, cim, (const double*)g.ai,
+            (kc_threads(c.bw) == 512 ? k_gemm_inv_kc<double, 512> : k_gemm_inv_kc<double, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>(cre, cim, (const double*)g.ai,
                                                                       (const GemmMeta*)g.meta, (double*)Cb, c.bw, F,
                                                                       (int)fmt);
         S2_HIP(hipGetLastError());

@@ -457,10 +457,10 @@ def test_few_field_calls_match_batched(env, bw, prec):
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("bw,prec", [(256, 0), (128, 0), (255, 0), (256, 1), (128, 1)])
+@pytest.mark.parametrize("bw,prec", [(256, 0), (128, 0), (255, 0), (256, 1), (128, 1), (512, 0), (512, 1), (300, 0)])
 @pytest.mark.parametrize("fmt", [0, 1])
 def test_k_chunked_gemms_match_whole_row_gemms(env, O, bw, prec, fmt):
-    # Calls of 64 or more fields at bw in [128, 256] run the K-chunked GEMMs
+    # Calls of 64 or more fields at bw in [128, 512] run the K-chunked GEMMs
     # (64 columns per block, data streamed through LDS chunks); calls of 8
     # fields the whole-row GEMMs.  Both take every accumulator's K steps in
     # the same order, so the results agree bit for bit; two fields are also
