@@ -362,12 +362,12 @@ __global__ __launch_bounds__(kThreads) void k_gemm_fwd(const R* __restrict__ Cb,
                                                        const typename Cplx<R>::type* __restrict__ X,
                                                        const double* __restrict__ wn, const R* __restrict__ AF,
                                                        const GemmMeta* __restrict__ meta, double* __restrict__ cre,
-                                                       double* __restrict__ cim, int B, int cut, int F, int ncb,
-                                                       int fmt) {
+                                                       double* __restrict__ cim, int B, int cut, int mlo, int F,
+                                                       int ncb, int fmt) {
     extern __shared__ __align__(16) unsigned char smem[];
     R* Bd = reinterpret_cast<R*>(smem);
     int bx, m;
-    if (!gemm_block(B, bx, m)) return;
+    if (!gemm_block(B, bx, m) || m < mlo) return;  // orders below mlo: the K-chunked launch
     const int cbase = bx * ncb;
     if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
     const GGeo G = ggeo<R>(B, GEN);
@@ -517,11 +517,11 @@ template <typename R, bool GEN, int CT>
 __global__ __launch_bounds__(kThreads) void k_gemm_inv(const double* __restrict__ cre, const double* __restrict__ cim,
                                                        const R* __restrict__ AI, const GemmMeta* __restrict__ meta,
                                                        R* __restrict__ Cb, typename Cplx<R>::type* __restrict__ X,
-                                                       int B, int cut, int F, int ncb, int fmt) {
+                                                       int B, int cut, int mlo, int F, int ncb, int fmt) {
     extern __shared__ __align__(16) unsigned char smem[];
     R* Cd = reinterpret_cast<R*>(smem);
     int bx, m;
-    if (!gemm_block(B, bx, m)) return;
+    if (!gemm_block(B, bx, m) || m < mlo) return;  // orders below mlo: the K-chunked launch
     const int cbase = bx * ncb;
     if (cbase >= F * ((m > 0 && fmt == 0) ? 4 : 2)) return;
     const GGeo G = ggeo<R>(B, GEN);
@@ -560,10 +560,10 @@ __device__ __forceinline__ const R* kc_row(const R* __restrict__ Cb, int col, in
 template <typename R, int NTH>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_fwd_kc(const R* __restrict__ Cb, const R* __restrict__ AF,
                                                           const GemmMeta* __restrict__ meta, double* __restrict__ cre,
-                                                          double* __restrict__ cim, int B, int F, int fmt) {
+                                                          double* __restrict__ cim, int B, int cut, int F, int fmt) {
     __shared__ R Bd[2][kKcCols * kKcRS];
     int bx, m;
-    if (!gemm_block(B, bx, m)) return;
+    if (!gemm_block(B, bx, m) || m >= cut) return;  // seminaive orders only
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf, cbase = bx * kKcCols;
     if (cbase >= ncols) return;
@@ -671,10 +671,10 @@ template <typename R, int NTH>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_inv_kc(const double* __restrict__ cre,
                                                           const double* __restrict__ cim, const R* __restrict__ AI,
                                                           const GemmMeta* __restrict__ meta, R* __restrict__ Cb, int B,
-                                                          int F, int fmt) {
+                                                          int cut, int F, int fmt) {
     __shared__ R Cd[2][kKcCols * kKcRS];
     int bx, m;
-    if (!gemm_block(B, bx, m)) return;
+    if (!gemm_block(B, bx, m) || m >= cut) return;  // seminaive orders only
     const int cpf = (m > 0 && fmt == 0) ? 4 : 2;
     const int ncols = F * cpf, cbase = bx * kKcCols;
     if (cbase >= ncols) return;
@@ -889,25 +889,25 @@ static int gemm_slices(int nblocks) { return std::max(1, std::min(8, 1024 / std:
 
 template <typename R, bool GEN, int CT>
 static void fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, const void* X, double* cre,
-                       double* cim, int F, Fmt fmt, int ncb, hipStream_t s) {
+                       double* cim, int F, Fmt fmt, int ncb, int mlo, hipStream_t s) {
     using T = typename Cplx<R>::type;
     const int B = c.bw, cols = 4 * F;
     const int nx = (cols + ncb - 1) / ncb, ny = (B + 7) & ~7;
     const dim3 grid(nx, ny, gemm_slices(nx * ny));
     launch_lds(k_gemm_fwd<R, GEN, CT>, grid, kThreads, (size_t)ncb * gemm_row(B, g.prec), s, (const R*)Cb, (const T*)X,
-               (const double*)c.w_naive, (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, F, ncb,
+               (const double*)c.w_naive, (const R*)g.af, (const GemmMeta*)g.meta, cre, cim, B, g.cut, mlo, F, ncb,
                (int)fmt);
 }
 
 template <typename R, bool GEN, int CT>
 static void inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb,
-                       void* X, int F, Fmt fmt, int ncb, hipStream_t s) {
+                       void* X, int F, Fmt fmt, int ncb, int mlo, hipStream_t s) {
     using T = typename Cplx<R>::type;
     const int B = c.bw, cols = 4 * F;
     const int nx = (cols + ncb - 1) / ncb, ny = (B + 7) & ~7;
     const dim3 grid(nx, ny, gemm_slices(nx * ny));
     launch_lds(k_gemm_inv<R, GEN, CT>, grid, kThreads, (size_t)ncb * gemm_row(B, g.prec), s, cre, cim, (const R*)g.ai,
-               (const GemmMeta*)g.meta, (R*)Cb, (T*)X, B, g.cut, F, ncb, (int)fmt);
+               (const GemmMeta*)g.meta, (R*)Cb, (T*)X, B, g.cut, mlo, F, ncb, (int)fmt);
 }
 
 std::vector<std::pair<int, int>> gemm_tile_choices(int bw, int prec) {
@@ -927,12 +927,13 @@ static void gemm_tiles(const DevConsts& c, const GemmTables& g, int& ncb, bool& 
     two = (g.t_ncb ? g.t_ct : gemm_ct(ncb, c.bw)) == 2;
 }
 
-// The K-chunked kernels take calls of many fields on plans whose orders are
-// all seminaive, bw in [128, 512] (at most sixteen 16-wide tiles per parity:
+// The K-chunked kernels take the seminaive orders (m < cutoff) of calls of
+// many fields, bw in [128, 512] (at most sixteen 16-wide tiles per parity:
 // one per wave; at bw = 64 they were slower, 31.5 -> 34.0 us forward at 256
-// fields).
+// fields); the naive orders of a plan with cutoff < bw follow in a second
+// launch of the whole-row kernels.
 static bool kc_ok(const DevConsts& c, const GemmTables& g, int F) {
-    return g.cut == c.bw && c.bw >= 128 && c.bw <= 512 && F >= 64;
+    return g.cut > 0 && c.bw >= 128 && c.bw <= 512 && F >= 64;
 }
 // threads of a K-chunked block: one wave per 16-wide tile of a parity
 static int kc_threads(int B) { return B <= 256 ? 512 : 1024; }
@@ -944,24 +945,25 @@ void gemm_fwd_launch(const DevConsts& c, const GemmTables& g, const void* Cb, co
     if (kc_ok(c, g, F)) {
         if (g.prec != 0)
             (kc_threads(c.bw) == 512 ? k_gemm_fwd_kc<float, 512> : k_gemm_fwd_kc<float, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>((const float*)Cb, (const float*)g.af,
-                                                                     (const GemmMeta*)g.meta, cre, cim, c.bw, F,
+                                                                     (const GemmMeta*)g.meta, cre, cim, c.bw, g.cut, F,
                                                                      (int)fmt);
         else
             (kc_threads(c.bw) == 512 ? k_gemm_fwd_kc<double, 512> : k_gemm_fwd_kc<double, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>((const double*)Cb, (const double*)g.af,
-                                                                      (const GemmMeta*)g.meta, cre, cim, c.bw, F,
+                                                                      (const GemmMeta*)g.meta, cre, cim, c.bw, g.cut, F,
                                                                       (int)fmt);
         S2_HIP(hipGetLastError());
-        return;
+        if (g.cut == c.bw) return;
     }
+    const int mlo = kc_ok(c, g, F) ? g.cut : 0;
     int ncb;
     bool two;
     gemm_tiles(c, g, ncb, two);
     if (g.prec != 0)
-        (two ? fwd_launch<float, false, 2> : fwd_launch<float, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
+        (two ? fwd_launch<float, false, 2> : fwd_launch<float, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, mlo, s);
     else if (g.gen)
-        (two ? fwd_launch<double, true, 2> : fwd_launch<double, true, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
+        (two ? fwd_launch<double, true, 2> : fwd_launch<double, true, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, mlo, s);
     else
-        (two ? fwd_launch<double, false, 2> : fwd_launch<double, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, s);
+        (two ? fwd_launch<double, false, 2> : fwd_launch<double, false, 1>)(c, g, Cb, X, cre, cim, F, fmt, ncb, mlo, s);
 }
 
 void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre, const double* cim, void* Cb, void* X,
@@ -969,24 +971,25 @@ void gemm_inv_launch(const DevConsts& c, const GemmTables& g, const double* cre,
     if (kc_ok(c, g, F)) {
         if (g.prec != 0)
             (kc_threads(c.bw) == 512 ? k_gemm_inv_kc<float, 512> : k_gemm_inv_kc<float, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>(cre, cim, (const float*)g.ai,
-                                                                     (const GemmMeta*)g.meta, (float*)Cb, c.bw, F,
+                                                                     (const GemmMeta*)g.meta, (float*)Cb, c.bw, g.cut, F,
                                                                      (int)fmt);
         else
             (kc_threads(c.bw) == 512 ? k_gemm_inv_kc<double, 512> : k_gemm_inv_kc<double, 1024>)<<<kc_grid(c.bw, F), kc_threads(c.bw), 0, s>>>(cre, cim, (const double*)g.ai,
-                                                                      (const GemmMeta*)g.meta, (double*)Cb, c.bw, F,
+                                                                      (const GemmMeta*)g.meta, (double*)Cb, c.bw, g.cut, F,
                                                                       (int)fmt);
         S2_HIP(hipGetLastError());
-        return;
+        if (g.cut == c.bw) return;
     }
+    const int mlo = kc_ok(c, g, F) ? g.cut : 0;
     int ncb;
     bool two;
     gemm_tiles(c, g, ncb, two);
     if (g.prec != 0)
-        (two ? inv_launch<float, false, 2> : inv_launch<float, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
+        (two ? inv_launch<float, false, 2> : inv_launch<float, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, mlo, s);
     else if (g.gen)
-        (two ? inv_launch<double, true, 2> : inv_launch<double, true, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
+        (two ? inv_launch<double, true, 2> : inv_launch<double, true, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, mlo, s);
     else
-        (two ? inv_launch<double, false, 2> : inv_launch<double, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, s);
+        (two ? inv_launch<double, false, 2> : inv_launch<double, false, 1>)(c, g, cre, cim, Cb, X, F, fmt, ncb, mlo, s);
 }
 
 template <typename R>

@@ -457,24 +457,28 @@ def test_few_field_calls_match_batched(env, bw, prec):
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("bw,prec", [(256, 0), (128, 0), (255, 0), (256, 1), (128, 1), (512, 0), (512, 1), (300, 0)])
+@pytest.mark.parametrize("bw,prec,cutoff", [(256, 0, 256), (128, 0, 128), (255, 0, 255), (256, 1, 256), (128, 1, 128),
+                                            (512, 0, 512), (512, 1, 512), (300, 0, 300), (256, 0, 64), (255, 0, 100),
+                                            (512, 0, 128), (128, 1, 32)])
 @pytest.mark.parametrize("fmt", [0, 1])
-def test_k_chunked_gemms_match_whole_row_gemms(env, O, bw, prec, fmt):
-    # Calls of 64 or more fields at bw in [128, 512] run the K-chunked GEMMs
-    # (64 columns per block, data streamed through LDS chunks); calls of 8
-    # fields the whole-row GEMMs.  Both take every accumulator's K steps in
-    # the same order, so the results agree bit for bit; two fields are also
-    # checked against the oracle (FP64).
+def test_k_chunked_gemms_match_whole_row_gemms(env, O, bw, prec, cutoff, fmt):
+    # Calls of 64 or more fields at bw in [128, 512] run the seminaive orders
+    # on the K-chunked GEMMs (64 columns per block, data streamed through LDS
+    # chunks) and, at cutoff < bw, the naive orders on the whole-row GEMMs in
+    # a second launch; calls of 8 fields run every order on the whole-row
+    # GEMMs.  Both take every accumulator's K steps in the same order, so the
+    # results agree bit for bit; two fields are also checked against the
+    # oracle (FP64).
     S, torch, dev = env
     F, per, n = 64, 8, 2 * bw
-    print(f"k-chunked bw={bw} prec={prec} fmt={fmt}", flush=True)
+    print(f"k-chunked bw={bw} prec={prec} cutoff={cutoff} fmt={fmt}", flush=True)
     kind = 2 if fmt == 1 else 0
     coeffs = [O.make_coeffs(300 + f, bw, kind) for f in range(F)]
     cre = to_dev(torch, dev, [c[0] for c in coeffs])
     cim = to_dev(torch, dev, [c[1] for c in coeffs])
     outs = []
     for per_call in (F, per):
-        plan = S.Plan(bw, per_call, precision=prec)
+        plan = S.Plan(bw, per_call, precision=prec, cutoff=cutoff)
         gre = torch.empty((F, n, n), dtype=torch.float64, device=dev)
         gim = torch.empty_like(gre)
         c2r, c2i = torch.empty_like(cre), torch.empty_like(cim)
@@ -490,9 +494,9 @@ def test_k_chunked_gemms_match_whole_row_gemms(env, O, bw, prec, fmt):
     if prec == 0:
         gre, gim, c2r, c2i = outs[0]
         for f in (0, F - 1):
-            og = O.inv_fst(coeffs[f][0], coeffs[f][1], bw, data_format=fmt)
+            og = O.inv_fst(coeffs[f][0], coeffs[f][1], bw, data_format=fmt, cutoff=cutoff)
             assert crel(gre[f].cpu().numpy().ravel(), gim[f].cpu().numpy().ravel(), *og) < 1e-11
-            oc = O.fst(gre[f].cpu().numpy().ravel(), gim[f].cpu().numpy().ravel(), bw, data_format=fmt)
+            oc = O.fst(gre[f].cpu().numpy().ravel(), gim[f].cpu().numpy().ravel(), bw, data_format=fmt, cutoff=cutoff)
             assert crel(c2r[f].cpu().numpy(), c2i[f].cpu().numpy(), *oc) < 1e-11
 
 
