@@ -46,3 +46,28 @@ def test_legendre_flops_per_pair(bw, pair_flops):
     # tabulated from the TableSize formula (cospml.c:39-59) independently
     assert 2 * bench.legendre_flops(bw, 1, "fp64") == pair_flops
     assert bench.legendre_flops(bw, 3, "mixed") == 3 * 4 * bw ** 3
+
+
+def test_kernel_map_names_the_launched_gemms():
+    # the kernels the host launches per form, bandwidth and field count
+    # (s2_fast.hip kc_ok / kc_threads, s2_kernels.hip fft_cfg), as the
+    # roofline object and the profile summaries name them
+    k = bench.kernel_map("mixed", 256, 1024)
+    assert (k["leg_fwd"], k["leg_inv"]) == ("k_mix_fwd", "k_mix_inv")
+    k = bench.kernel_map("fp64", 256, 1024)
+    assert (k["leg_fwd"], k["leg_inv"]) == ("k_gemm_fwd_kc<double, 512>", "k_gemm_inv_kc<double, 512>")
+    assert bench.kernel_map("fp32", 512, 128)["leg_fwd"] == "k_gemm_fwd_kc<float, 1024>"
+    # fewer than 64 fields, or bw below 128: the whole-row GEMMs
+    assert bench.kernel_map("fp64", 256, 32)["leg_fwd"].startswith("k_gemm_fwd<double, false")
+    assert bench.kernel_map("fp64", 64, 1024)["leg_fwd"].startswith("k_gemm_fwd<double, false")
+    # bw=255: 510 = 2*3*5*17 rings on the register mixed-radix instance;
+    # bw=263: 526 = 2*263, Bluestein over 1080 = 2^3 3^3 5 points
+    k = bench.kernel_map("fp64", 255, 256)
+    assert k["phi_fwd"] == "k_phi_fwd<double2, 1>" and k["leg_fwd"] == "k_gemm_fwd_kc<double, 512>"
+    assert bench.kernel_map("fp64", 263, 256)["phi_fwd"] == "k_phi_fwd<double2, 3>"
+    assert bench.kernel_map("fp64", 263, 256)["leg_fwd"] == "k_gemm_fwd_kc<double, 1024>"
+    # the committed profile summaries name the same kernels
+    for form, bw, f in (("fp64", 256, 1024), ("fp64", 255, 256), ("fp64", 263, 256)):
+        text = open(os.path.join(ROOT, "profiles", f"kernel_stats_{form}_bw{bw}_f{f}.txt")).read()
+        for st in ("leg_fwd", "leg_inv"):
+            assert bench.kernel_map(form, bw, f)[st] in text, (form, bw, st)
