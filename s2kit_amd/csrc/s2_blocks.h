@@ -352,25 +352,55 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
     const int q0 = bx * rpb;
     {
         // V[k] = conj(ph[k]) * (X[k] - i X[N-k]) with X zero from index B on:
-        // X[k] for k < B, -i X[N-k] for k > B, 0 at k = B -- one value each
-        R cr[V], ci[V];
+        // X[k] for k < B, -i X[N-k] for k > B, 0 at k = B.
+        if constexpr (sizeof(R) == 8) {
+            // A thread loads the coefficient of each of its k < B once and forms
+            // V at k and at N - k (at k = 0: V[0] and the zero V[B]).  Loading per
+            // output position reads every coefficient twice: FP64 bw=256 x 1024
+            // 1.72 -> 1.59 ms with the pairing; FP32 is 12% slower with it.
+            R cr[V / 2], ci[V / 2];
 #pragma unroll
-        for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * NTH;
-            const int q = q0 + (idx >> lgN), k = idx & (N - 1);
-            const R* row = Cb + ((size_t)f * N + frow_of(min(q, nrows - 1), N, cut)) * 2 * B;
-            const int ks = k < B ? k : (k > B ? N - k : 0);
-            cr[u] = row[cpos(ks, half)];
-            ci[u] = row[B + cpos(ks, half)];
-        }
+            for (int u = 0; u < V / 2; ++u) {
+                const int idx = threadIdx.x + u * NTH;
+                const int q = q0 + (idx >> (lgN - 1)), k = idx & (B - 1);
+                const R* row = Cb + ((size_t)f * N + frow_of(min(q, nrows - 1), N, cut)) * 2 * B;
+                cr[u] = row[cpos(k, half)];
+                ci[u] = row[B + cpos(k, half)];
+            }
 #pragma unroll
-        for (int u = 0; u < V; ++u) {
-            const int idx = threadIdx.x + u * NTH;
-            const int q = q0 + (idx >> lgN), rl = idx >> lgN, k = idx & (N - 1);
-            const bool live = q < nrows && k != B;
-            const R tr = live ? (k < B ? cr[u] : ci[u]) : (R)0;
-            const R ti = live ? (k < B ? ci[u] : -cr[u]) : (R)0;
-            a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(tr, ti));
+            for (int u = 0; u < V / 2; ++u) {
+                const int idx = threadIdx.x + u * NTH;
+                const int rl = idx >> (lgN - 1), k = idx & (B - 1);
+                const bool live = q0 + rl < nrows;
+                const R ar = live ? cr[u] : (R)0, ai = live ? ci[u] : (R)0;   // position k
+                const R br = live ? ci[u] : (R)0, bi = live ? -cr[u] : (R)0;  // position N - k
+                a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(ar, ai));
+                if (k == 0)
+                    a[rl * S + B] = cmul(cconj(ph[B]), mk<T>((R)0, (R)0));
+                else
+                    a[rl * S + N - k] = cmul(cconj(ph[N - k]), mk<T>(br, bi));
+            }
+        } else {
+            // one value per output position
+            R cr[V], ci[V];
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                const int idx = threadIdx.x + u * NTH;
+                const int q = q0 + (idx >> lgN), k = idx & (N - 1);
+                const R* row = Cb + ((size_t)f * N + frow_of(min(q, nrows - 1), N, cut)) * 2 * B;
+                const int ks = k < B ? k : (k > B ? N - k : 0);
+                cr[u] = row[cpos(ks, half)];
+                ci[u] = row[B + cpos(ks, half)];
+            }
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                const int idx = threadIdx.x + u * NTH;
+                const int q = q0 + (idx >> lgN), rl = idx >> lgN, k = idx & (N - 1);
+                const bool live = q < nrows && k != B;
+                const R tr = live ? (k < B ? cr[u] : ci[u]) : (R)0;
+                const R ti = live ? (k < B ? ci[u] : -cr[u]) : (R)0;
+                a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(tr, ti));
+            }
         }
     }
     // sin(theta_j) of the output phase, loaded while the FFT runs (FP64 B=256
