@@ -467,9 +467,9 @@ def cpu_worker(args):
     return time.perf_counter() - t0
 
 
-# A one-GPU box allots this many host cores to its GPU; the CPU baseline runs
-# one process per core of that share (os.cpu_count() there reports the whole
-# host, which other GPUs' jobs share).
+# A shared multi-GPU host allots this many cores to each GPU; the CPU baseline
+# runs one process per core of that share (os.cpu_count() reports the whole
+# host, which the other GPUs' jobs share).
 CPU_SHARE_PER_GPU = 16
 
 
