@@ -1,8 +1,8 @@
 """ctypes wrapper over the CPU FP64 oracle (oracle/libs2oracle.so).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()``
-and by bench.py's cpu_baseline leg, always as the checker or the timed CPU
-port -- never by the product package ``s2kit_amd``.
+and by bench.py's cpu_baseline leg, always as the reference the results are
+compared with or the timed CPU port -- never by the product package ``s2kit_amd``.
 
 Each wrapper names the reference function it restates; the C source carries
 the file:line citations.

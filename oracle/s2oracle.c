@@ -1,7 +1,7 @@
 /*
  * s2oracle.c -- CPU FP64 restatement of S2kit 1.1's memoised seminaive SHT.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the
+ * TEST INFRASTRUCTURE ONLY: what the HIP product path is tested against, and the
  * "port" CPU baseline.  Nothing in the product library links or calls it.
  *
  * Written from the mathematics and from the reference's documented behaviour
