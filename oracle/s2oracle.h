@@ -2,8 +2,8 @@
  * s2oracle.h -- CPU FP64 restatement of S2kit's memoised seminaive spherical
  * harmonic transform.
  *
- * TEST INFRASTRUCTURE ONLY.  The HIP product path is tested against this
- * library (s2kit_amd/csrc) and the "port" CPU baseline timed by bench.py.  It is
+ * TEST INFRASTRUCTURE ONLY.  The HIP product path (s2kit_amd/csrc) is tested
+ * against this library, which is also the "port" CPU baseline timed by bench.py.  It is
  * never linked into, loaded by, or called from the product library.
  *
  * Every function cites the reference file:line whose behaviour it restates.
