@@ -215,7 +215,11 @@ __device__ __forceinline__ void xcd_block_m(int& bx, int& m) {
 __device__ __forceinline__ int cpos(int k, int half) { return (k & 1) ? half + (k >> 1) : (k >> 1); }
 
 // Copy the N twiddles exp(-2 pi i t/N) into LDS (they are read by every
-// butterfly of every pass).
+// butterfly of every pass).  Called before a kernel's data loads, its wait
+// for the twiddles holds every data load back by one memory round trip, so
+// most instances call it after issuing them (bw=256 x 1024: FP64 FFTs and
+// DCTs 3-7% faster, the FP32 forward FFT 1.08 -> 1.05 ms); the FP32
+// instances noted at their call sites measured faster with it first.
 template <typename T, int NTH = kThreads>
 __device__ __forceinline__ T* stage_twiddles(unsigned char* smem, const T* __restrict__ tw, int N) {
     T* t = reinterpret_cast<T*>(smem);
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(NTH) void k_fphi_fwd(const double* __restrict__ re,
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1;
-    const T* twl = stage_twiddles<T, NTH>(smem, tw, N);
+    const T* twl = reinterpret_cast<const T*>(smem);
     T* a = reinterpret_cast<T*>(smem) + N;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
@@ -249,6 +253,7 @@ __global__ __launch_bounds__(NTH) void k_fphi_fwd(const double* __restrict__ re,
             vr[u] = ld_stream<NTF>(re + base + threadIdx.x + u * NTH);
             vi[u] = ld_stream<NTF>(im + base + threadIdx.x + u * NTH);
         }
+        stage_twiddles<T, NTH>(smem, tw, N);
 #pragma unroll
         for (int u = 0; u < V; ++u) {
             const int idx = threadIdx.x + u * NTH;
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(NTH) void k_fdct_fwd(const typename Cplx<R>::type* 
                                                   const typename Cplx<R>::type* __restrict__ ph) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    const T* twl = stage_twiddles<T, NTH>(smem, tw, 1 << lgN);
+    const T* twl = reinterpret_cast<const T*>(smem);
     T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
     constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
@@ -290,6 +295,9 @@ __global__ __launch_bounds__(NTH) void k_fdct_fwd(const typename Cplx<R>::type* 
     int bx, f;
     xcd_block(bx, f);
     const int q0 = bx * rpb;
+    // FP32 stages the twiddles before its loads (behind them: bw=512 x 64
+    // 0.24 -> 0.26 ms, bw=256 x 1024 0.80 -> 0.77 ms)
+    if constexpr (sizeof(R) == 4) stage_twiddles<T, NTH>(smem, tw, N);
     {
         R xr[V], xi[V], ww[V];
 #pragma unroll
@@ -303,6 +311,7 @@ __global__ __launch_bounds__(NTH) void k_fdct_fwd(const typename Cplx<R>::type* 
             xi[u] = xv.y;
             ww[u] = q < nrows ? w_semi[(am & 1) * N + j] : (R)0;
         }
+        if constexpr (sizeof(R) == 8) stage_twiddles<T, NTH>(smem, tw, N);
 #pragma unroll
         for (int u = 0; u < V; ++u) {
             const int idx = threadIdx.x + u * NTH;
@@ -342,7 +351,7 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
                                                   const typename Cplx<R>::type* __restrict__ ph) {
     using T = typename Cplx<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    const T* twl = stage_twiddles<T, NTH>(smem, tw, 1 << lgN);
+    const T* twl = reinterpret_cast<const T*>(smem);
     T* a = reinterpret_cast<T*>(smem) + (1 << lgN);
     constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1, B = N >> 1, half = B >> 1;
@@ -367,6 +376,7 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
                 cr[u] = row[cpos(k, half)];
                 ci[u] = row[B + cpos(k, half)];
             }
+            stage_twiddles<T, NTH>(smem, tw, N);
 #pragma unroll
             for (int u = 0; u < V / 2; ++u) {
                 const int idx = threadIdx.x + u * NTH;
@@ -381,7 +391,9 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
                     a[rl * S + N - k] = cmul(cconj(ph[N - k]), mk<T>(br, bi));
             }
         } else {
-            // one value per output position
+            // one value per output position; FP32 stages the twiddles first
+            // (behind its loads this instance ran 1.10 -> 1.39 ms at bw=256 x 1024)
+            stage_twiddles<T, NTH>(smem, tw, N);
             R cr[V], ci[V];
 #pragma unroll
             for (int u = 0; u < V; ++u) {
@@ -443,12 +455,17 @@ __global__ __launch_bounds__(NTH) void k_fphi_inv(const typename Cplx<R>::type* 
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int E = V * NTH;
     const int N = 1 << lgN, S = N + 1, B = N >> 1;
-    const T* twl = stage_twiddles<T, NTH>(smem, tw, N);
+    const T* twl = reinterpret_cast<const T*>(smem);
     T* a = reinterpret_cast<T*>(smem) + N;
     const int lgrpb = __builtin_ctz(E) - lgN, rpb = 1 << lgrpb;
     int bx, f;
     xcd_block(bx, f);
     const int j0 = bx * rpb;
+    // twiddles behind the loads for FP64 and the 16-byte-store instance; the
+    // FP32 8-byte-store instance stages them first (behind: bw=512 x 64
+    // 0.35 -> 0.39 ms)
+    constexpr bool kLate = sizeof(R) == 8 || W16;
+    if constexpr (!kLate) stage_twiddles<T, NTH>(smem, tw, N);
     {
         T v[V];
 #pragma unroll
@@ -457,6 +474,7 @@ __global__ __launch_bounds__(NTH) void k_fphi_inv(const typename Cplx<R>::type* 
             const int r = idx >> lgrpb;
             v[u] = (r == B) ? mk<T>(0, 0) : X[((size_t)f * N + r) * N + j0 + (idx & (rpb - 1))];
         }
+        if constexpr (kLate) stage_twiddles<T, NTH>(smem, tw, N);
 #pragma unroll
         for (int u = 0; u < V; ++u) {
             const int idx = threadIdx.x + u * NTH;
