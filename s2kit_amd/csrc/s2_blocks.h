@@ -364,10 +364,14 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
         // X[k] for k < B, -i X[N-k] for k > B, 0 at k = B.
         if constexpr (sizeof(R) == 8) {
             // A thread loads the coefficient of each of its k < B once and forms
-            // V at k and at N - k (at k = 0: V[0] and the zero V[B]).  Loading per
-            // output position reads every coefficient twice: FP64 bw=256 x 1024
-            // 1.72 -> 1.59 ms with the pairing; FP32 is 12% slower with it.
+            // V at k and at its partner position N - k (B for k = 0, where V is
+            // zero).  Loading per output position reads every coefficient twice:
+            // FP64 bw=256 x 1024 1.72 -> 1.59 ms with the pairing; FP32 is 12%
+            // slower with it.  The phase factors of both positions are loaded
+            // with the coefficients: loaded in the second loop, each iteration
+            // waited for its own (a branch there kept them from being batched).
             R cr[V / 2], ci[V / 2];
+            T pa[V / 2], pb[V / 2];
 #pragma unroll
             for (int u = 0; u < V / 2; ++u) {
                 const int idx = threadIdx.x + u * NTH;
@@ -375,20 +379,19 @@ __global__ __launch_bounds__(NTH) void k_fdct_inv(const R* __restrict__ Cb, type
                 const R* row = Cb + ((size_t)f * N + frow_of(min(q, nrows - 1), N, cut)) * 2 * B;
                 cr[u] = row[cpos(k, half)];
                 ci[u] = row[B + cpos(k, half)];
+                pa[u] = ph[k];
+                pb[u] = ph[k == 0 ? B : N - k];
             }
             stage_twiddles<T, NTH>(smem, tw, N);
 #pragma unroll
             for (int u = 0; u < V / 2; ++u) {
                 const int idx = threadIdx.x + u * NTH;
                 const int rl = idx >> (lgN - 1), k = idx & (B - 1);
-                const bool live = q0 + rl < nrows;
-                const R ar = live ? cr[u] : (R)0, ai = live ? ci[u] : (R)0;   // position k
-                const R br = live ? ci[u] : (R)0, bi = live ? -cr[u] : (R)0;  // position N - k
-                a[rl * S + k] = cmul(cconj(ph[k]), mk<T>(ar, ai));
-                if (k == 0)
-                    a[rl * S + B] = cmul(cconj(ph[B]), mk<T>((R)0, (R)0));
-                else
-                    a[rl * S + N - k] = cmul(cconj(ph[N - k]), mk<T>(br, bi));
+                const bool live = q0 + rl < nrows, lb = live && k != 0;
+                const R ar = live ? cr[u] : (R)0, ai = live ? ci[u] : (R)0;  // position k
+                const R br = lb ? ci[u] : (R)0, bi = lb ? -cr[u] : (R)0;     // partner position
+                a[rl * S + k] = cmul(cconj(pa[u]), mk<T>(ar, ai));
+                a[rl * S + (k == 0 ? B : N - k)] = cmul(cconj(pb[u]), mk<T>(br, bi));
             }
         } else {
             // one value per output position; FP32 stages the twiddles first
