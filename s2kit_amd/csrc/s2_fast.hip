@@ -590,11 +590,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int nq = ((row_size(m, m + p + 2 * (g.Lp - 1)) + 3) / 4 * 4 + kKc - 1) / kKc;
         typename Mma<R>::acc acc[kKcNS][4] = {};
         R v[kKcPer], ta[kKcNS][kKcSteps], tb[kKcNS][kKcSteps];
+        bool vok[kKcPer];
         auto stage = [&](int q) __attribute__((always_inline)) {
 #pragma unroll
             for (int e = 0; e < kKcPer; ++e) {
                 const int t = q * kKc + fk + e;
-                v[e] = (fcol && t < cnt) ? src[t] : (R)0;
+                vok[e] = fcol && t < cnt;
+                v[e] = *(vok[e] ? src + t : Cb);  // unconditional load (see table); zeroed in put
             }
         };
         auto table = [&](int q, R (&tv)[kKcNS][kKcSteps]) __attribute__((always_inline)) {
@@ -602,13 +604,19 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int s = 0; s < kKcNS; ++s)
 #pragma unroll
                 for (int u = 0; u < kKcSteps; ++u) {
+                    // Every lane loads, from the array's first entry when the
+                    // entry is past its tile (a load under a branch left the
+                    // compiler unable to count loads in flight, and it waited
+                    // for all of them, this prefetch included, before each
+                    // MFMA).  Such values are never used: chunk() runs an
+                    // MFMA only under the same condition.
                     const int k = q * kKc + 4 * u;
-                    tv[s][u] = k < kend[s] ? ga[(k + kq) * Li16 + 16 * (wave + kKcWaves * s) + ln] : (R)0;
+                    tv[s][u] = *(k < kend[s] ? ga + ((k + kq) * Li16 + 16 * (wave + kKcWaves * s) + ln) : AF);
                 }
         };
         auto put = [&](int b) __attribute__((always_inline)) {
 #pragma unroll
-            for (int e = 0; e < kKcPer; ++e) Bd[b][fc * kKcRS + fk + e] = v[e];
+            for (int e = 0; e < kKcPer; ++e) Bd[b][fc * kKcRS + fk + e] = vok[e] ? v[e] : (R)0;
         };
         auto chunk = [&](int q, const R (&tv)[kKcNS][kKcSteps]) __attribute__((always_inline)) {
             const R* bd = Bd[q & 1] + ln * kKcRS + kq;
@@ -627,18 +635,21 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k
         put(0);
         __syncthreads();
         for (int q = 0; q < nq; q += 2) {
-            if (q + 1 < nq) {
-                stage(q + 1);
-                table(q + 1, tb);
-            }
+            // The next chunk's loads, issued unconditionally (past the last
+            // chunk they read the arrays' first entries, unused): with the
+            // count of loads in flight fixed, each MFMA waits only for the
+            // table value it reads, loaded one chunk earlier, and the empty
+            // asm keeps the loads ahead of this chunk's MFMAs.
+            stage(q + 1);
+            table(q + 1, tb);
+            asm volatile("" ::: "memory");
             chunk(q, ta);
             if (q + 1 < nq) put(1);
             __syncthreads();
             if (q + 1 >= nq) break;
-            if (q + 2 < nq) {
-                stage(q + 2);
-                table(q + 2, ta);
-            }
+            stage(q + 2);
+            table(q + 2, ta);
+            asm volatile("" ::: "memory");
             chunk(q + 1, tb);
             if (q + 2 < nq) put(0);
             __syncthreads();
@@ -710,11 +721,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int nq = (Li4 + kKc - 1) / kKc;
         typename Mma<R>::acc acc[kKcNS][4] = {};
         R v[kKcPer], ta[kKcNS][kKcSteps], tb[kKcNS][kKcSteps];
+        bool vok[kKcPer];
         auto stage = [&](int q) __attribute__((always_inline)) {
 #pragma unroll
             for (int e = 0; e < kKcPer; ++e) {
                 const int i = q * kKc + fk + e;
-                v[e] = (fcol && i < g.Lp) ? (R)fsrc[index_of_coeff(fmm, m + p + 2 * i, B)] : (R)0;
+                vok[e] = fcol && i < g.Lp;
+                v[e] = *(vok[e] ? fsrc + index_of_coeff(fmm, m + p + 2 * i, B) : cre);  // unconditional; zeroed in put
             }
         };
         auto active = [&](int s, int k) __attribute__((always_inline)) { return k >= kst[s] && k < Li4; };
@@ -723,13 +736,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k
             for (int s = 0; s < kKcNS; ++s)
 #pragma unroll
                 for (int u = 0; u < kKcSteps; ++u) {
-                    const int k = q * kKc + 4 * u;
-                    tv[s][u] = active(s, k) ? ga[(k + kq) * Tp16 + 16 * (wave + kKcWaves * s) + ln] : (R)0;
+                    const int k = q * kKc + 4 * u;  // unconditional load, used only where active, as in the forward
+                    tv[s][u] = *(active(s, k) ? ga + ((k + kq) * Tp16 + 16 * (wave + kKcWaves * s) + ln) : AI);
                 }
         };
         auto put = [&](int b) __attribute__((always_inline)) {
 #pragma unroll
-            for (int e = 0; e < kKcPer; ++e) Cd[b][fc * kKcRS + fk + e] = v[e];
+            for (int e = 0; e < kKcPer; ++e) Cd[b][fc * kKcRS + fk + e] = vok[e] ? v[e] : (R)0;
         };
         auto chunk = [&](int q, const R (&tv)[kKcNS][kKcSteps]) __attribute__((always_inline)) {
             const R* cd = Cd[q & 1] + ln * kKcRS + kq;
@@ -748,18 +761,21 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k
         put(0);
         __syncthreads();
         for (int q = 0; q < nq; q += 2) {
-            if (q + 1 < nq) {
-                stage(q + 1);
-                table(q + 1, tb);
-            }
+            // The next chunk's loads, issued unconditionally (past the last
+            // chunk they read the arrays' first entries, unused): with the
+            // count of loads in flight fixed, each MFMA waits only for the
+            // table value it reads, loaded one chunk earlier, and the empty
+            // asm keeps the loads ahead of this chunk's MFMAs.
+            stage(q + 1);
+            table(q + 1, tb);
+            asm volatile("" ::: "memory");
             chunk(q, ta);
             if (q + 1 < nq) put(1);
             __syncthreads();
             if (q + 1 >= nq) break;
-            if (q + 2 < nq) {
-                stage(q + 2);
-                table(q + 2, ta);
-            }
+            stage(q + 2);
+            table(q + 2, ta);
+            asm volatile("" ::: "memory");
             chunk(q + 1, tb);
             if (q + 2 < nq) put(0);
             __syncthreads();
